@@ -1,0 +1,162 @@
+"""Benchmark: Goldilocks inner-product sumcheck (InnerProductProver, MSB),
+v = 28, two 2^28-element tables uniform in [0, p) from fixed seeds, all 28
+rounds with the SHAKE256 Fiat-Shamir transcript, on one MI355X.
+
+A step is one full proof (28 rounds + finalize + final value) over tables
+already resident in HBM.  Metric: evals/s = 2^v / T_proof (the reference's
+criterion convention, Throughput::Elements(2^v), benches/sumcheck.rs:28,80).
+
+Prints one JSON line.  Besides the contract fields it carries
+  roofline     - the fused fold+evaluate kernel's achieved bandwidth:
+                 algorithmic bytes of all its launches in the timed region
+                 divided by their summed HIP-event durations (events recorded
+                 on the prover's stream around each launch);
+  cpu_baseline - the oracle's restatement of the reference's
+                 InnerProductProver (parallel folds, serial evaluate), timed on
+                 this host's cores on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "efficient-sumcheck_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SEED_A, SEED_B = 0x5EED0002, 0x5EED0003
+DOMSEP = b"effsc-hip/bench/ip"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(v, tables=2):
+    # per table: round 0 reads N; round j >= 1 reads L_j and writes L_j/2;
+    # finalize folds 2 -> 1 (reads 2, writes 1)
+    n = 1 << v
+    words = n  # round 0
+    L = n
+    while L >= 2:
+        words += L + L // 2
+        L //= 2
+    return 8 * tables * words
+
+
+def cpu_baseline(v_sample, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n = 1 << v_sample
+    a = O.uniform(n, SEED_A)
+    b = O.uniform(n, SEED_B)
+    t, _, _, _ = O.bench_ip(a, b, v_sample, threads)
+    return n / t, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--v", type=int, default=28)
+    ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size (default: v)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import effsc
+    from effsc import _native as N
+
+    L = N.lib()
+    v = args.v
+    n = 1 << v
+    dev = 0
+    ptrs = []
+    for seed in (SEED_A, SEED_B):
+        p = N.u64p()
+        N.check(L.effsc_hip_device_alloc(dev, n, C.byref(p)))
+        N.check(L.effsc_hip_fill_uniform(dev, p, n, seed, 0))
+        ptrs.append(p)
+    addr = [C.cast(p, C.c_void_p).value for p in ptrs]
+    prover = effsc.InnerProductProver(addr[0], addr[1], borrow=True, n=n, device=dev)
+    log(f"bench: inner product v={v}, tables resident ({2 * n * 8 / 2**30:.1f} GiB)")
+
+    def step():
+        prover.reset()
+        return effsc.sumcheck(prover, v, effsc.Transcript(DOMSEP))
+
+    for _ in range(args.warmup):
+        step()
+    N.check(L.effsc_hip_synchronize(dev))
+    N.check(L.effsc_hip_profile(prover.h, 1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = step()
+    N.check(L.effsc_hip_synchronize(dev))
+    elapsed = time.perf_counter() - t0
+    launches = (C.c_uint64 * 3)()
+    ms = (C.c_double * 3)()
+    nbytes = (C.c_double * 3)()
+    N.check(L.effsc_hip_profile_read(prover.h, launches, ms, nbytes))
+    N.check(L.effsc_hip_profile(prover.h, 0))
+    # the profiled region adds event records; time an unprofiled region too
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    N.check(L.effsc_hip_synchronize(dev))
+    elapsed_plain = time.perf_counter() - t1
+    T = min(elapsed, elapsed_plain) / args.steps
+
+    fused_gbs = nbytes[0] / (ms[0] * 1e-3) / 1e9 if ms[0] > 0 else 0.0
+    eval_gbs = nbytes[1] / (ms[1] * 1e-3) / 1e9 if ms[1] > 0 else 0.0
+    alg = algorithmic_bytes(v)
+    log(f"bench: {T * 1e3:.3f} ms/proof; fused kernel {launches[0]} launches {ms[0]:.2f} ms "
+        f"{fused_gbs:.0f} GB/s; round-0 evaluate {eval_gbs:.0f} GB/s; final {proof.final_value}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cv = args.cpu_v if args.cpu_v is not None else v
+        log(f"bench: CPU baseline (oracle port of the reference IP prover), v={cv}, {threads} threads")
+        rate, secs = cpu_baseline(cv, threads)
+        cpu = {"value": rate, "unit": "evals/s", "cores": threads, "kind": "port",
+               "sample": f"one full InnerProductProver proof at v={cv} ({secs:.2f} s): two parallel folds "
+                         f"per round, serial evaluate, scalar Montgomery arithmetic"}
+
+    out = {
+        "metric": "Goldilocks sumcheck evals/s (InnerProductProver MSB, v=28)",
+        "value": n / T,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"inner_product_msb_v{v}", "v": v, "field": "goldilocks", "tables": 2,
+                   "transcript": "shake256-fs", "input": "uniform [0,p) counter-based, seeds 0x5EED0002/3"},
+        "gbps_algorithmic": alg / T / 1e9,
+        "roofline": {"bound": "hbm", "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fused_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fused<Fp,Fp,2,2> (fold + evaluate, rounds 1..v-1)",
+                     "launches_timed": int(launches[0]), "kernel_ms_total": ms[0],
+                     "round0_eval_gbs": eval_gbs},
+        "cpu_baseline": cpu,
+    }
+    prover.close()
+    for p in ptrs:
+        L.effsc_hip_device_free(dev, p)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

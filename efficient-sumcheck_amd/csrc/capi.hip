@@ -1,0 +1,1212 @@
+// C ABI of the gfx950 sumcheck prover: prover handles, round dispatch, the
+// host protocol loop (runner, verifier, transcripts) and utilities.
+// Declarations and reference citations: include/effsc_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "effsc_hip.h"
+#include "kernels.hpp"
+#include "shake.hpp"
+
+using namespace effsc_hip;
+using gl::Fp;
+using gl::Fp3;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                               \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(EFFSC_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(expr)                 \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != EFFSC_HIP_OK) return rc_; \
+  } while (0)
+
+enum Kind { K_ML = 1, K_IP = 2, K_PRODUCT = 3, K_EQ = 4 };
+
+size_t next_pow2(size_t n) {
+  size_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+int g_grid_max = 0;  // 0: derived from the device's CU count
+
+int grid_max_for(int dev) {
+  if (g_grid_max > 0) return g_grid_max;
+  if (const char* e = std::getenv("EFFSC_HIP_GRID_MAX")) {
+    int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  return cus * 8;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Prover handle
+// ---------------------------------------------------------------------------
+struct effsc_hip_prover {
+  int kind = 0;
+  int ext = 1, in_ext = 1, K = 1, deg = 1, repr = 0, dev = 0;
+  uint64_t w = EFFSC_HIP_W_FROM_RAW7;
+  size_t n = 0, len = 0;
+  bool in_work = false, own_in = true;
+  uint64_t* in[kMaxK] = {};
+  size_t in_stride = 0;
+  uint64_t* work[kMaxK] = {};
+  size_t work_stride = 0;
+  hipStream_t stream = nullptr;
+  Reduce red = {};
+  uint64_t* h_res = nullptr;  // pinned, kMaxK * 3 words
+  int grid_max = 2048;
+  // EqFactored state (src/provers/eq_factored.rs:67-83)
+  size_t v = 0, v_l = 0, rounds_elapsed = 0;
+  uint64_t* eq_l = nullptr;  // ext planes, stride eq_l_stride
+  uint64_t* eq_r = nullptr;
+  size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
+  uint64_t* eq_inner = nullptr;  // per-row partial sums for the left phase
+  // Kernel timing (effsc_hip_profile): HIP events around each launch on the
+  // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold.
+  bool prof_on = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int prof_pending = -1;
+  double prof_pending_bytes = 0;
+  uint64_t prof_launches[3] = {0, 0, 0};
+  double prof_ms[3] = {0, 0, 0};
+  double prof_bytes[3] = {0, 0, 0};
+};
+
+namespace {
+
+int alloc_dev(uint64_t** p, size_t words) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(words, 1) * 8);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(EFFSC_HIP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return EFFSC_HIP_OK;
+}
+
+// Brackets a kernel launch with events when profiling; the elapsed time is
+// collected at the next stream synchronisation (prof_collect).
+int prof_begin(effsc_hip_prover* h) {
+  if (h->prof_on) HIPCHK(hipEventRecord(h->ev0, h->stream));
+  return EFFSC_HIP_OK;
+}
+int prof_end(effsc_hip_prover* h, int slot, double bytes) {
+  if (!h->prof_on) return EFFSC_HIP_OK;
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->prof_pending = slot;
+  h->prof_pending_bytes = bytes;
+  return EFFSC_HIP_OK;
+}
+int prof_collect(effsc_hip_prover* h) {
+  if (!h->prof_on || h->prof_pending < 0) return EFFSC_HIP_OK;
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  int s = h->prof_pending;
+  h->prof_launches[s]++;
+  h->prof_ms[s] += ms;
+  h->prof_bytes[s] += h->prof_pending_bytes;
+  h->prof_pending = -1;
+  return EFFSC_HIP_OK;
+}
+
+unsigned blocks_for(const effsc_hip_prover* h, size_t units) {
+  size_t b = (units + kBlock - 1) / kBlock;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(b, (size_t)h->grid_max));
+}
+
+template <class F, int K>
+Views<F, K> views(uint64_t* const* planes, size_t stride) {
+  Views<F, K> v;
+  for (int i = 0; i < K; i++) v.t[i] = View<F>{planes[i], stride};
+  return v;
+}
+
+template <class F>
+F from_words(const uint64_t* w) {
+  return gl::load_words<F>(w);
+}
+
+// -------------------------------------------------------------- launchers
+template <class FI, class FO, int K>
+struct Ops {
+  static int eval(effsc_hip_prover* h, uint64_t* const* planes, size_t stride, size_t half, size_t paired) {
+    Views<FI, K> t = views<FI, K>(planes, stride);
+    bool vec = (paired == half) && (half % 2 == 0) && (stride % 2 == 0);
+    TRY(prof_begin(h));
+    if (vec) {
+      unsigned g = blocks_for(h, half / 2);
+      hipLaunchKernelGGL((k_eval<FI, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, t, half, paired, h->w, h->red);
+    } else {
+      unsigned g = blocks_for(h, half);
+      hipLaunchKernelGGL((k_eval<FI, K, 1>), dim3(g), dim3(kBlock), 0, h->stream, t, half, paired, h->w, h->red);
+    }
+    HIPCHK(hipGetLastError());
+    // algorithmic bytes: every element of the K tables read once
+    TRY(prof_end(h, 1, 8.0 * K * FI::D * (double)(half + paired)));
+    return EFFSC_HIP_OK;
+  }
+  static int fused(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t q) {
+    Views<FI, K> in = views<FI, K>(src, src_stride);
+    Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
+    FO c = from_words<FO>(chal);
+    bool vec = (q % 2 == 0) && (src_stride % 2 == 0) && (h->work_stride % 2 == 0);
+    TRY(prof_begin(h));
+    if (vec) {
+      unsigned g = blocks_for(h, q / 2);
+      hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, q, h->w, h->red);
+    } else {
+      unsigned g = blocks_for(h, q);
+      hipLaunchKernelGGL((k_fused<FI, FO, K, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, q, h->w, h->red);
+    }
+    HIPCHK(hipGetLastError());
+    // algorithmic bytes: read 4q elements, write 2q elements, per table
+    TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D)));
+    return EFFSC_HIP_OK;
+  }
+  static int fold(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t half,
+                  size_t paired) {
+    Views<FI, K> in = views<FI, K>(src, src_stride);
+    Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
+    FO c = from_words<FO>(chal);
+    unsigned g = blocks_for(h, half);
+    TRY(prof_begin(h));
+    hipLaunchKernelGGL((k_fold<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, half, paired, h->w);
+    HIPCHK(hipGetLastError());
+    TRY(prof_end(h, 2, 8.0 * K * ((double)(half + paired) * FI::D + (double)half * FO::D)));
+    return EFFSC_HIP_OK;
+  }
+};
+
+template <template <class, class, int> class O, class FI, class FO, class Fn>
+int with_k(int K, Fn&& fn) {
+  switch (K) {
+    case 1: return fn(O<FI, FO, 1>{});
+    case 2: return fn(O<FI, FO, 2>{});
+    case 3: return fn(O<FI, FO, 3>{});
+    case 4: return fn(O<FI, FO, 4>{});
+  }
+  return fail(EFFSC_HIP_EINVAL, "unsupported table count");
+}
+
+// Calls fn(Ops<FI, FO, K>{}) for the runtime field degrees and table count.
+template <class Fn>
+int dispatch(int fi, int fo, int K, Fn&& fn) {
+  if (fi == 1 && fo == 1) return with_k<Ops, Fp, Fp>(K, fn);
+  if (fi == 1 && fo == 3) return with_k<Ops, Fp, Fp3>(K, fn);
+  if (fi == 3 && fo == 3) return with_k<Ops, Fp3, Fp3>(K, fn);
+  return fail(EFFSC_HIP_EINVAL, "unsupported field combination");
+}
+
+// ------------------------------------------------------------ round logic
+int src_field(const effsc_hip_prover* h) { return h->in_work ? h->ext : h->in_ext; }
+uint64_t* const* src_planes(const effsc_hip_prover* h) { return h->in_work ? h->work : h->in; }
+size_t src_stride(const effsc_hip_prover* h) { return h->in_work ? h->work_stride : h->in_stride; }
+
+// Half-split geometry of the current length: ML/IP pad implicitly to the next
+// power of two (src/inner_product_sumcheck.rs:165-172); the coefficient
+// prover pairs (k, k + n/2) (src/provers/coefficient.rs:131-140).
+void geometry(const effsc_hip_prover* h, size_t L, size_t* half, size_t* paired) {
+  if (h->kind == K_PRODUCT) {
+    *half = L / 2;
+    *paired = *half;
+  } else {
+    *half = next_pow2(L) >> 1;
+    *paired = L - *half;
+  }
+}
+
+// Reads element `idx` of every table (current state), lifted to the prover's
+// field, as canonical words: out[i * ext + c].
+int read_elements(effsc_hip_prover* h, size_t idx, uint64_t* out) {
+  int fs = src_field(h);
+  uint64_t* const* planes = src_planes(h);
+  size_t stride = src_stride(h);
+  for (int i = 0; i < h->K; i++) {
+    for (int c = 0; c < h->ext; c++) {
+      uint64_t x = 0;
+      if (c < fs) HIPCHK(hipMemcpyAsync(&h->h_res[0], planes[i] + c * stride + idx, 8, hipMemcpyDeviceToHost, h->stream));
+      if (c < fs) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        x = h->h_res[0];
+      }
+      out[i * h->ext + c] = x;
+    }
+  }
+  return EFFSC_HIP_OK;
+}
+
+int fetch_result(effsc_hip_prover* h, int fi, uint64_t* out_vals) {
+  // out_vals: K values of the prover's field (ext words each)
+  HIPCHK(hipMemcpyAsync(h->h_res, h->red.result, (size_t)h->K * fi * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
+  for (int i = 0; i < h->K; i++)
+    for (int c = 0; c < h->ext; c++) out_vals[i * h->ext + c] = c < fi ? h->h_res[i * fi + c] : 0;
+  return EFFSC_HIP_OK;
+}
+
+// Fold the current tables with challenge c (ext words).  If `fuse` and the
+// length allows, also evaluates the next round polynomial into red.result
+// and sets *evaluated.
+int fold_step(effsc_hip_prover* h, const uint64_t* c, bool fuse, bool* evaluated) {
+  *evaluated = false;
+  size_t L = h->len;
+  if (L <= 1) return EFFSC_HIP_OK;
+  size_t half, paired;
+  geometry(h, L, &half, &paired);
+  int fi = src_field(h);
+  uint64_t* const* src = src_planes(h);
+  size_t sst = src_stride(h);
+  if (fuse && is_pow2(L) && L >= 4) {
+    TRY(dispatch(fi, h->ext, h->K, [&](auto op) { return decltype(op)::fused(h, src, sst, c, L / 4); }));
+    *evaluated = true;
+  } else {
+    TRY(dispatch(fi, h->ext, h->K, [&](auto op) { return decltype(op)::fold(h, src, sst, c, half, paired); }));
+  }
+  h->len = half;
+  h->in_work = true;
+  return EFFSC_HIP_OK;
+}
+
+// Round polynomial values (K values, ext words each) of the current state.
+int evaluate(effsc_hip_prover* h, bool already, uint64_t* vals) {
+  size_t L = h->len;
+  const int E = h->ext;
+  std::fill(vals, vals + (size_t)h->K * E, 0);
+  if (already) return fetch_result(h, h->ext, vals);
+  if (h->kind != K_PRODUCT && L <= 1) {
+    if (L == 0) return EFFSC_HIP_OK;
+    // n == 1: ML -> [v0]; IP -> [a0 b0, 0] (src/provers/inner_product.rs:74-81)
+    uint64_t el[kMaxK * 3];
+    TRY(read_elements(h, 0, el));
+    if (h->kind == K_ML) {
+      std::copy(el, el + E, vals);
+    } else if (E == 1) {
+      vals[0] = gl::mul(el[0], el[1]);
+    } else {
+      Fp3 p = gl::mul(from_words<Fp3>(el), from_words<Fp3>(el + 3), h->w);
+      gl::store_words(p, vals);
+    }
+    return EFFSC_HIP_OK;
+  }
+  size_t half, paired;
+  geometry(h, L, &half, &paired);
+  if (half == 0) return EFFSC_HIP_OK;
+  int fi = src_field(h);
+  uint64_t* const* src = src_planes(h);
+  size_t sst = src_stride(h);
+  TRY(dispatch(fi, fi, h->K, [&](auto op) { return decltype(op)::eval(h, src, sst, half, paired); }));
+  return fetch_result(h, fi, vals);
+}
+
+// EvalsInfty wire from the K product values (h(0), h(inf), h(2), ...): the
+// kernels already produce exactly that order; ML keeps one value.
+void to_wire(const effsc_hip_prover* h, const uint64_t* vals, uint64_t* out) {
+  size_t nw = (size_t)std::max(h->deg, 1);
+  std::copy(vals, vals + nw * h->ext, out);
+}
+
+int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
+int eq_finalize(effsc_hip_prover* h, const uint64_t* chal);
+int eq_final_value(effsc_hip_prover* h, uint64_t* out);
+
+int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
+  if (h->kind == K_EQ) return eq_round(h, chal, out);
+  bool evaluated = false;
+  if (chal) TRY(fold_step(h, chal, true, &evaluated));
+  uint64_t vals[kMaxK * 3];
+  TRY(evaluate(h, evaluated, vals));
+  to_wire(h, vals, out);
+  return EFFSC_HIP_OK;
+}
+
+int finalize_impl(effsc_hip_prover* h, const uint64_t* chal) {
+  if (h->kind == K_EQ) return eq_finalize(h, chal);
+  bool evaluated = false;
+  return fold_step(h, chal, false, &evaluated);
+}
+
+int final_value_impl(effsc_hip_prover* h, uint64_t* out) {
+  if (h->kind == K_EQ) return eq_final_value(h, out);
+  const int E = h->ext;
+  std::fill(out, out + E, 0);
+  if (h->len != 1) return EFFSC_HIP_OK;
+  uint64_t el[kMaxK * 3];
+  TRY(read_elements(h, 0, el));
+  // ML: evals[0]; IP: a0 b0; product: prod of singletons
+  // (src/provers/multilinear.rs:73-79, inner_product.rs:120-126,
+  //  coefficient.rs:300-329 with pairs (x_i, 0): h(0) + h(1) = prod x_i)
+  if (E == 1) {
+    uint64_t acc = el[0];
+    for (int i = 1; i < h->K; i++) acc = gl::mul(acc, el[i]);
+    out[0] = acc;
+  } else {
+    Fp3 acc = from_words<Fp3>(el);
+    for (int i = 1; i < h->K; i++) acc = gl::mul(acc, from_words<Fp3>(el + 3 * i), h->w);
+    gl::store_words(acc, out);
+  }
+  return EFFSC_HIP_OK;
+}
+
+// ----------------------------------------------------- word conversions
+void words_in(const effsc_hip_prover* h, const uint64_t* src, uint64_t* dst, size_t n) {
+  for (size_t i = 0; i < n; i++) dst[i] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(src[i], gl::MONT_TO_CANON) : gl::canon(src[i]);
+}
+void words_out(const effsc_hip_prover* h, const uint64_t* src, uint64_t* dst, size_t n) {
+  for (size_t i = 0; i < n; i++) dst[i] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(src[i], gl::CANON_TO_MONT) : src[i];
+}
+
+// ---------------------------------------------------------- construction
+int check_cfg(const effsc_hip_config* cfg) {
+  if (!cfg) return fail(EFFSC_HIP_EINVAL, "null config");
+  if (cfg->ext != 1 && cfg->ext != 3) return fail(EFFSC_HIP_EINVAL, "ext must be 1 or 3");
+  if (cfg->input_ext != 1 && cfg->input_ext != cfg->ext)
+    return fail(EFFSC_HIP_EINVAL, "input_ext must be 1 or equal to ext");
+  if (cfg->repr != EFFSC_HIP_CANONICAL && cfg->repr != EFFSC_HIP_MONT64) return fail(EFFSC_HIP_EINVAL, "bad repr");
+  if ((cfg->flags & EFFSC_HIP_BORROW_INPUT) && cfg->repr != EFFSC_HIP_CANONICAL)
+    return fail(EFFSC_HIP_EINVAL, "borrowed input must be canonical");
+  return EFFSC_HIP_OK;
+}
+
+void destroy(effsc_hip_prover* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->dev);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int i = 0; i < kMaxK; i++) {
+    if (h->own_in && h->in[i]) (void)hipFree(h->in[i]);
+    if (h->work[i]) (void)hipFree(h->work[i]);
+  }
+  if (h->red.partials) (void)hipFree(h->red.partials);
+  if (h->red.counter) (void)hipFree(h->red.counter);
+  if (h->red.result) (void)hipFree(h->red.result);
+  if (h->eq_l) (void)hipFree(h->eq_l);
+  if (h->eq_r) (void)hipFree(h->eq_r);
+  if (h->eq_inner) (void)hipFree(h->eq_inner);
+  if (h->h_res) (void)hipHostFree(h->h_res);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+// Allocates stream, reduction scratch and work tables (K tables, ext planes of
+// `cap` elements).
+int setup_common(effsc_hip_prover* h, size_t cap) {
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->grid_max = grid_max_for(h->dev);
+  TRY(alloc_dev(&h->red.partials, (size_t)h->grid_max * kMaxK * 3));
+  uint64_t* ctr = nullptr;
+  TRY(alloc_dev(&ctr, 1));
+  h->red.counter = reinterpret_cast<unsigned*>(ctr);
+  HIPCHK(hipMemsetAsync(h->red.counter, 0, 8, h->stream));
+  TRY(alloc_dev(&h->red.result, kMaxK * 3));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_res), kMaxK * 3 * 8, hipHostMallocDefault));
+  h->work_stride = round_up(std::max<size_t>(cap, 1), 64);
+  for (int i = 0; i < h->K; i++) TRY(alloc_dev(&h->work[i], h->work_stride * h->ext));
+  return EFFSC_HIP_OK;
+}
+
+// Imports K input tables of n elements (in_ext words each, AoS, cfg repr)
+// into canonical device planes.
+int import_tables(effsc_hip_prover* h, const effsc_hip_config* cfg, const uint64_t* const* tables, size_t n) {
+  if (cfg->flags & EFFSC_HIP_BORROW_INPUT) {
+    h->own_in = false;
+    h->in_stride = n;
+    for (int i = 0; i < h->K; i++) {
+      if (!tables[i] && n) return fail(EFFSC_HIP_EINVAL, "null table");
+      if (reinterpret_cast<uintptr_t>(tables[i]) % 16) return fail(EFFSC_HIP_EINVAL, "borrowed table not 16-byte aligned");
+      h->in[i] = const_cast<uint64_t*>(tables[i]);
+    }
+    return EFFSC_HIP_OK;
+  }
+  h->own_in = true;
+  h->in_stride = round_up(std::max<size_t>(n, 1), 64);
+  const int d = h->in_ext;
+  uint64_t* staging = nullptr;
+  if (d > 1 && n) TRY(alloc_dev(&staging, n * d));
+  int rc = EFFSC_HIP_OK;
+  for (int i = 0; i < h->K && rc == EFFSC_HIP_OK; i++) {
+    rc = alloc_dev(&h->in[i], h->in_stride * d);
+    if (rc != EFFSC_HIP_OK || n == 0) continue;
+    if (!tables[i]) {
+      rc = fail(EFFSC_HIP_EINVAL, "null table");
+      break;
+    }
+    uint64_t* dst = d > 1 ? staging : h->in[i];
+    hipError_t e = hipMemcpyAsync(dst, tables[i], n * d * 8, hipMemcpyDefault, h->stream);
+    if (e != hipSuccess) {
+      rc = fail(EFFSC_HIP_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+      break;
+    }
+    unsigned g = blocks_for(h, n);
+    hipLaunchKernelGGL(k_import, dim3(g), dim3(kBlock), 0, h->stream, h->in[i], h->in_stride, dst, n, d,
+                       h->repr == EFFSC_HIP_MONT64 ? 1 : 0);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, std::string("import: ") + hipGetErrorString(e));
+  }
+  if (staging) (void)hipFree(staging);
+  return rc;
+}
+
+int new_product_like(const effsc_hip_config* cfg, int kind, int K, int deg, const uint64_t* const* tables, size_t n,
+                     effsc_hip_prover** out) {
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  *out = nullptr;
+  TRY(check_cfg(cfg));
+  if (K < 1 || K > kMaxK) return fail(EFFSC_HIP_EINVAL, "table count must be in 1..4");
+  auto* h = new effsc_hip_prover();
+  h->kind = kind;
+  h->ext = cfg->ext;
+  h->in_ext = cfg->input_ext;
+  h->K = K;
+  h->deg = deg;
+  h->repr = cfg->repr;
+  h->dev = cfg->device;
+  h->w = cfg->ext == 3 ? gl::canon(cfg->ext_w) : 0;
+  h->n = n;
+  h->len = n;
+  int rc = setup_common(h, next_pow2(std::max<size_t>(n, 1)) >> 1);
+  if (rc == EFFSC_HIP_OK) rc = import_tables(h, cfg, tables, n);
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// EqFactoredProver: sum_x eq(w, x) p(x) with eq split into eq(w_L, .) and
+// eq(w_R, .) (src/provers/eq_factored.rs:67-289).  Tables in the handle:
+// p -> in[0]/work[0] (K = 1), eq_l, eq_r.
+// ===========================================================================
+namespace {
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_eq_table(uint64_t* planes, size_t stride, const uint64_t* wpt, size_t v,
+                                                     uint64_t w) {
+  // table[x] = prod_j (x_j ? w_j : 1 - w_j), x_j = bit (v-1-j) of x
+  // (build_eq_table, src/provers/eq_factored.rs:217-239, MSB layout)
+  View<F> t{planes, stride};
+  const size_t n = (size_t)1 << v;
+  for (size_t x = gtid(); x < n; x += (size_t)gridDim.x * blockDim.x) {
+    F acc = gl::one<F>();
+    for (size_t j = 0; j < v; j++) {
+      F wj = gl::load_words<F>(wpt + j * F::D);
+      F f = ((x >> (v - 1 - j)) & 1) ? wj : gl::sub(gl::one<F>(), wj);
+      acc = gl::mul(acc, f, w);
+    }
+    t.st(x, acc);
+  }
+}
+
+// Left phase (rounds_elapsed < v_L): q0 = sum_a eqL_lo[a] sum_b eqR[b] p_lo[a m + b]
+// and q_inf = sum_a (eqL_hi[a] - eqL_lo[a]) sum_b eqR[b] (p_hi - p_lo)[a m + b]
+// (round_poly_left, src/provers/eq_factored.rs:139-172).  Each thread handles
+// index i = a m + b of the low half and contributes eqL-weighted terms
+// directly: eqL_lo[a] eqR[b] p_lo and (eqL_hi - eqL_lo)[a] eqR[b] (p_hi - p_lo).
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_eq_left(View<F> p, size_t p_half, View<F> eql, size_t eql_half,
+                                                    View<F> eqr, size_t m_log, uint64_t w, Reduce r) {
+  F acc[2] = {gl::zero<F>(), gl::zero<F>()};
+  const size_t mask = ((size_t)1 << m_log) - 1;
+  for (size_t i = gtid(); i < p_half; i += (size_t)gridDim.x * blockDim.x) {
+    size_t a = i >> m_log, b = i & mask;
+    F er = eqr.ld(b);
+    F pl = p.ld(i), ph = p.ld(p_half + i);
+    F el = eql.ld(a), eh = eql.ld(eql_half + a);
+    F t0 = gl::mul(er, pl, w);
+    F td = gl::mul(er, gl::sub(ph, pl), w);
+    acc[0] = gl::add(acc[0], gl::mul(el, t0, w));
+    acc[1] = gl::add(acc[1], gl::mul(gl::sub(eh, el), td, w));
+  }
+  finish<F, 2>(acc, r);
+}
+
+int eq_dispatch_left(effsc_hip_prover* h) {
+  uint64_t* const* src = src_planes(h);
+  size_t sst = src_stride(h);
+  size_t p_half = h->len >> 1;
+  size_t eql_half = h->eq_l_len >> 1;
+  size_t m_log = 0;
+  while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+  unsigned g = blocks_for(h, p_half);
+  if (h->ext == 1) {
+    hipLaunchKernelGGL(k_eq_left<Fp>, dim3(g), dim3(kBlock), 0, h->stream, View<Fp>{src[0], sst}, p_half,
+                       View<Fp>{h->eq_l, h->eq_l_stride}, eql_half, View<Fp>{h->eq_r, h->eq_r_stride}, m_log, h->w,
+                       h->red);
+  } else {
+    hipLaunchKernelGGL(k_eq_left<Fp3>, dim3(g), dim3(kBlock), 0, h->stream, View<Fp3>{src[0], sst}, p_half,
+                       View<Fp3>{h->eq_l, h->eq_l_stride}, eql_half, View<Fp3>{h->eq_r, h->eq_r_stride}, m_log,
+                       h->w, h->red);
+  }
+  HIPCHK(hipGetLastError());
+  return EFFSC_HIP_OK;
+}
+
+// Folds one eq half-table in place (ip::fold semantics on a pow2 table).
+int eq_fold_table(effsc_hip_prover* h, uint64_t* planes, size_t stride, size_t* len, const uint64_t* c) {
+  if (*len <= 1) return EFFSC_HIP_OK;
+  size_t half = *len >> 1;
+  unsigned g = blocks_for(h, half);
+  uint64_t* pl[kMaxK] = {planes};
+  if (h->ext == 1) {
+    Views<Fp, 1> v = views<Fp, 1>(pl, stride);
+    hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp>(c), half, half,
+                       h->w);
+  } else {
+    Views<Fp3, 1> v = views<Fp3, 1>(pl, stride);
+    hipLaunchKernelGGL((k_fold<Fp3, Fp3, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp3>(c), half,
+                       half, h->w);
+  }
+  HIPCHK(hipGetLastError());
+  *len = half;
+  return EFFSC_HIP_OK;
+}
+
+// Reads element 0 of an ext-plane table.
+int read0(effsc_hip_prover* h, const uint64_t* planes, size_t stride, int d, uint64_t* out) {
+  for (int c = 0; c < h->ext; c++) {
+    out[c] = 0;
+    if (c >= d) continue;
+    HIPCHK(hipMemcpyAsync(h->h_res, planes + c * stride, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    out[c] = h->h_res[0];
+  }
+  return EFFSC_HIP_OK;
+}
+
+template <class F>
+F fmul(const effsc_hip_prover* h, F a, F b) {
+  return gl::mul(a, b, h->w);
+}
+
+int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
+  const int E = h->ext;
+  if (chal) {
+    if (h->rounds_elapsed == 0) return fail(EFFSC_HIP_ESTATE, "challenge given before the first round");
+    size_t prev_var = h->rounds_elapsed - 1;
+    bool ev = false;
+    TRY(fold_step(h, chal, false, &ev));
+    if (prev_var < h->v_l)
+      TRY(eq_fold_table(h, h->eq_l, h->eq_l_stride, &h->eq_l_len, chal));
+    else
+      TRY(eq_fold_table(h, h->eq_r, h->eq_r_stride, &h->eq_r_len, chal));
+  }
+  std::fill(out, out + 2 * E, 0);
+  if (h->rounds_elapsed < h->v_l) {
+    TRY(eq_dispatch_left(h));
+    uint64_t vals[kMaxK * 3];
+    TRY(fetch_result(h, E, vals));
+    std::copy(vals, vals + 2 * E, out);
+  } else {
+    // round_poly_right (src/provers/eq_factored.rs:178-209): the inner-product
+    // round of (eq_r, p) scaled by the folded scalar eq_l[0].
+    uint64_t scalar[3];
+    TRY(read0(h, h->eq_l, h->eq_l_stride, E, scalar));
+    size_t n = h->len;
+    if (n <= 1) {
+      if (n == 1) {
+        uint64_t er[3], p0[3];
+        TRY(read0(h, h->eq_r, h->eq_r_stride, E, er));
+        TRY(read0(h, src_planes(h)[0], src_stride(h), src_field(h), p0));
+        if (E == 1) {
+          out[0] = gl::mul(gl::mul(scalar[0], er[0]), p0[0]);
+        } else {
+          Fp3 v = fmul(h, fmul(h, from_words<Fp3>(scalar), from_words<Fp3>(er)), from_words<Fp3>(p0));
+          gl::store_words(v, out);
+        }
+      }
+    } else {
+      size_t half = n >> 1;
+      uint64_t* planes[kMaxK] = {h->eq_r, src_planes(h)[0]};
+      // both tables must share a stride for the K = 2 view; eq_r and p use
+      // their own strides, so evaluate through per-table views
+      unsigned g = blocks_for(h, half);
+      if (E == 1) {
+        Views<Fp, 2> t;
+        t.t[0] = View<Fp>{planes[0], h->eq_r_stride};
+        t.t[1] = View<Fp>{planes[1], src_stride(h)};
+        hipLaunchKernelGGL((k_eval<Fp, 2, 1>), dim3(g), dim3(kBlock), 0, h->stream, t, half, half, h->w, h->red);
+      } else {
+        Views<Fp3, 2> t;
+        t.t[0] = View<Fp3>{planes[0], h->eq_r_stride};
+        t.t[1] = View<Fp3>{planes[1], src_stride(h)};
+        hipLaunchKernelGGL((k_eval<Fp3, 2, 1>), dim3(g), dim3(kBlock), 0, h->stream, t, half, half, h->w, h->red);
+      }
+      HIPCHK(hipGetLastError());
+      uint64_t vals[kMaxK * 3];
+      TRY(fetch_result(h, E, vals));
+      if (E == 1) {
+        out[0] = gl::mul(vals[0], scalar[0]);
+        out[1] = gl::mul(vals[1], scalar[0]);
+      } else {
+        Fp3 s = from_words<Fp3>(scalar);
+        gl::store_words(fmul(h, from_words<Fp3>(vals), s), out);
+        gl::store_words(fmul(h, from_words<Fp3>(vals + 3), s), out + 3);
+      }
+    }
+  }
+  h->rounds_elapsed++;
+  return EFFSC_HIP_OK;
+}
+
+int eq_finalize(effsc_hip_prover* h, const uint64_t* chal) {
+  // src/provers/eq_factored.rs:272-280
+  bool ev = false;
+  TRY(fold_step(h, chal, false, &ev));
+  size_t last_var = h->v == 0 ? (size_t)-1 : h->v - 1;
+  if (h->v > 0 && last_var < h->v_l)
+    TRY(eq_fold_table(h, h->eq_l, h->eq_l_stride, &h->eq_l_len, chal));
+  else
+    TRY(eq_fold_table(h, h->eq_r, h->eq_r_stride, &h->eq_r_len, chal));
+  return EFFSC_HIP_OK;
+}
+
+int eq_final_value(effsc_hip_prover* h, uint64_t* out) {
+  // src/provers/eq_factored.rs:282-288
+  const int E = h->ext;
+  std::fill(out, out + E, 0);
+  if (h->len != 1) return EFFSC_HIP_OK;
+  uint64_t p0[3], l0[3], r0[3];
+  TRY(read0(h, src_planes(h)[0], src_stride(h), src_field(h), p0));
+  TRY(read0(h, h->eq_l, h->eq_l_stride, E, l0));
+  TRY(read0(h, h->eq_r, h->eq_r_stride, E, r0));
+  if (E == 1) {
+    out[0] = gl::mul(gl::mul(p0[0], l0[0]), r0[0]);
+  } else {
+    Fp3 v = fmul(h, fmul(h, from_words<Fp3>(p0), from_words<Fp3>(l0)), from_words<Fp3>(r0));
+    gl::store_words(v, out);
+  }
+  return EFFSC_HIP_OK;
+}
+
+int build_eq(effsc_hip_prover* h, const uint64_t* wpt_canon_dev, size_t off, size_t vv, uint64_t** planes,
+             size_t* stride, size_t* len) {
+  size_t n = (size_t)1 << vv;
+  *stride = round_up(n, 64);
+  *len = n;
+  TRY(alloc_dev(planes, *stride * h->ext));
+  unsigned g = blocks_for(h, n);
+  if (h->ext == 1)
+    hipLaunchKernelGGL(k_eq_table<Fp>, dim3(g), dim3(kBlock), 0, h->stream, *planes, *stride,
+                       wpt_canon_dev + off * h->ext, vv, h->w);
+  else
+    hipLaunchKernelGGL(k_eq_table<Fp3>, dim3(g), dim3(kBlock), 0, h->stream, *planes, *stride,
+                       wpt_canon_dev + off * h->ext, vv, h->w);
+  HIPCHK(hipGetLastError());
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Exported entry points
+// ===========================================================================
+extern "C" {
+
+void effsc_hip_config_default(effsc_hip_config* cfg) {
+  if (!cfg) return;
+  cfg->ext = 1;
+  cfg->input_ext = 1;
+  cfg->repr = EFFSC_HIP_CANONICAL;
+  cfg->ext_w = EFFSC_HIP_W_FROM_RAW7;
+  cfg->device = 0;
+  cfg->flags = 0;
+}
+
+const char* effsc_hip_last_error(void) { return g_err.c_str(); }
+
+int effsc_hip_multilinear_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n, effsc_hip_prover** out) {
+  const uint64_t* t[1] = {evals};
+  return new_product_like(cfg, K_ML, 1, 1, t, n, out);
+}
+
+int effsc_hip_inner_product_new(const effsc_hip_config* cfg, const uint64_t* a, const uint64_t* b, size_t n,
+                                effsc_hip_prover** out) {
+  const uint64_t* t[2] = {a, b};
+  return new_product_like(cfg, K_IP, 2, 2, t, n, out);
+}
+
+int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
+                          effsc_hip_prover** out) {
+  if (out) *out = nullptr;
+  if (!tables) return fail(EFFSC_HIP_EINVAL, "null tables");
+  if (n && !is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "product prover tables must have power-of-two length");
+  return new_product_like(cfg, K_PRODUCT, k, k, tables, n, out);
+}
+
+int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point, size_t v, const uint64_t* p,
+                              size_t n, effsc_hip_prover** out) {
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  *out = nullptr;
+  TRY(check_cfg(cfg));
+  if (v >= 48) return fail(EFFSC_HIP_EINVAL, "too many variables");
+  if (n > ((size_t)1 << v))
+    return fail(EFFSC_HIP_EINVAL, "p_evals length exceeds 2^v");  // src/provers/eq_factored.rs:93-99
+  if (cfg->input_ext != cfg->ext) return fail(EFFSC_HIP_EINVAL, "eq-factored input must be in the prover's field");
+  if (cfg->flags & EFFSC_HIP_BORROW_INPUT) return fail(EFFSC_HIP_EINVAL, "eq-factored copies its input");
+  auto* h = new effsc_hip_prover();
+  h->kind = K_EQ;
+  h->ext = h->in_ext = cfg->ext;
+  h->K = 1;
+  h->deg = 2;
+  h->repr = cfg->repr;
+  h->dev = cfg->device;
+  h->w = cfg->ext == 3 ? gl::canon(cfg->ext_w) : 0;
+  size_t N = (size_t)1 << v;
+  h->n = N;
+  h->len = N;
+  h->v = v;
+  h->v_l = v / 2;
+  int rc = setup_common(h, N >> 1);
+  // p is zero-padded to 2^v: import the given prefix, clear the rest
+  if (rc == EFFSC_HIP_OK) {
+    h->own_in = true;
+    h->in_stride = round_up(N, 64);
+    rc = alloc_dev(&h->in[0], h->in_stride * h->ext);
+  }
+  if (rc == EFFSC_HIP_OK) {
+    hipError_t e = hipMemsetAsync(h->in[0], 0, h->in_stride * h->ext * 8, h->stream);
+    if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
+  }
+  uint64_t* staging = nullptr;
+  if (rc == EFFSC_HIP_OK) rc = alloc_dev(&staging, std::max<size_t>(n, v) * h->ext + 1);
+  if (rc == EFFSC_HIP_OK && n) {
+    hipError_t e = hipMemcpyAsync(staging, p, n * h->ext * 8, hipMemcpyDefault, h->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_import, dim3(blocks_for(h, n)), dim3(kBlock), 0, h->stream, h->in[0], h->in_stride,
+                         staging, n, h->ext, h->repr == EFFSC_HIP_MONT64 ? 1 : 0);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, std::string("import p: ") + hipGetErrorString(e));
+  }
+  if (rc == EFFSC_HIP_OK) {
+    // the point, canonical AoS on the device
+    std::vector<uint64_t> wc(std::max<size_t>(v, 1) * h->ext, 0);
+    for (size_t i = 0; i < v * h->ext; i++) {
+      uint64_t x = point[i];
+      wc[i] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(x, gl::MONT_TO_CANON) : gl::canon(x);
+    }
+    hipError_t e = hipMemcpyAsync(staging, wc.data(), wc.size() * 8, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
+    if (rc == EFFSC_HIP_OK) rc = build_eq(h, staging, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len);
+    if (rc == EFFSC_HIP_OK) rc = build_eq(h, staging, h->v_l, v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len);
+    if (rc == EFFSC_HIP_OK) {
+      e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
+    }
+  }
+  if (staging) (void)hipFree(staging);
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_degree(const effsc_hip_prover* h, size_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  *out = (size_t)h->deg;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_ext(const effsc_hip_prover* h, size_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  *out = (size_t)h->ext;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_round(effsc_hip_prover* h, const uint64_t* challenge, uint64_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  uint64_t c[3];
+  if (challenge) words_in(h, challenge, c, h->ext);
+  size_t nw = (size_t)std::max(h->deg, 1) * h->ext;
+  uint64_t res[kMaxK * 3];
+  TRY(round_impl(h, challenge ? c : nullptr, res));
+  words_out(h, res, out, nw);
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_finalize(effsc_hip_prover* h, const uint64_t* challenge) {
+  if (!h || !challenge) return fail(EFFSC_HIP_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  uint64_t c[3];
+  words_in(h, challenge, c, h->ext);
+  TRY(finalize_impl(h, c));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_final_value(effsc_hip_prover* h, uint64_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  uint64_t v[3];
+  TRY(final_value_impl(h, v));
+  words_out(h, v, out, h->ext);
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_table_count(const effsc_hip_prover* h, size_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  *out = h->kind == K_EQ ? 3 : (size_t)h->K;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_table_len(const effsc_hip_prover* h, size_t table, size_t* out) {
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  if (h->kind == K_EQ) {
+    if (table > 2) return fail(EFFSC_HIP_EINVAL, "table index out of range");
+    *out = table == 0 ? h->len : table == 1 ? h->eq_l_len : h->eq_r_len;
+    return EFFSC_HIP_OK;
+  }
+  if (table >= (size_t)h->K) return fail(EFFSC_HIP_EINVAL, "table index out of range");
+  *out = h->len;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_t cap_words, size_t* len) {
+  if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
+  size_t L = 0;
+  TRY(effsc_hip_table_len(h, table, &L));
+  if (len) *len = L;
+  const int E = h->ext;
+  if (!dst) return cap_words == 0 ? EFFSC_HIP_OK : fail(EFFSC_HIP_EINVAL, "null destination");
+  if (cap_words < L * E) return fail(EFFSC_HIP_EINVAL, "destination too small");
+  HIPCHK(hipSetDevice(h->dev));
+  const uint64_t* planes;
+  size_t stride;
+  int d;
+  if (h->kind == K_EQ && table > 0) {
+    planes = table == 1 ? h->eq_l : h->eq_r;
+    stride = table == 1 ? h->eq_l_stride : h->eq_r_stride;
+    d = E;
+  } else {
+    planes = src_planes(h)[table];
+    stride = src_stride(h);
+    d = src_field(h);
+  }
+  std::vector<uint64_t> buf(std::max<size_t>(L, 1) * d);
+  for (int c = 0; c < d && L; c++)
+    HIPCHK(hipMemcpyAsync(buf.data() + c * L, planes + c * stride, L * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < L; i++)
+    for (int c = 0; c < E; c++) {
+      uint64_t x = c < d ? buf[c * L + i] : 0;
+      dst[i * E + c] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(x, gl::CANON_TO_MONT) : x;
+    }
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_reset(effsc_hip_prover* h) {
+  if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
+  if (h->kind == K_EQ) return fail(EFFSC_HIP_ESTATE, "eq-factored prover folds its eq tables in place; rebuild it");
+  h->len = h->n;
+  h->in_work = false;
+  return EFFSC_HIP_OK;
+}
+
+void effsc_hip_free(effsc_hip_prover* h) { destroy(h); }
+
+int effsc_hip_profile(effsc_hip_prover* h, int enable) {
+  if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
+  HIPCHK(hipSetDevice(h->dev));
+  if (enable && !h->ev0) {
+    HIPCHK(hipEventCreate(&h->ev0));
+    HIPCHK(hipEventCreate(&h->ev1));
+  }
+  h->prof_on = enable != 0;
+  h->prof_pending = -1;
+  for (int i = 0; i < 3; i++) {
+    h->prof_launches[i] = 0;
+    h->prof_ms[i] = 0;
+    h->prof_bytes[i] = 0;
+  }
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_profile_read(const effsc_hip_prover* h, uint64_t* launches, double* ms, double* bytes) {
+  if (!h || !launches || !ms || !bytes) return fail(EFFSC_HIP_EINVAL, "null argument");
+  for (int i = 0; i < 3; i++) {
+    launches[i] = h->prof_launches[i];
+    ms[i] = h->prof_ms[i];
+    bytes[i] = h->prof_bytes[i];
+  }
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_synchronize(int device) {
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipDeviceSynchronize());
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size_t* new_len) {
+  if (!values && n) return fail(EFFSC_HIP_EINVAL, "null values");
+  HIPCHK(hipSetDevice(device));
+  if (n <= 1) {
+    if (new_len) *new_len = n;
+    return EFFSC_HIP_OK;
+  }
+  size_t half = next_pow2(n) >> 1, paired = n - half;
+  uint64_t* pl[kMaxK] = {values};
+  Views<Fp, 1> v = views<Fp, 1>(pl, n);
+  size_t b = std::min<size_t>((half + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3((unsigned)std::max<size_t>(b, 1)), dim3(kBlock), 0, nullptr, v, v,
+                     Fp{gl::canon(weight)}, half, paired, (uint64_t)0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  if (new_len) *new_len = half;
+  return EFFSC_HIP_OK;
+}
+
+// ----------------------------------------------------------------- transcripts
+struct effsc_hip_transcript {
+  int kind;  // 0: Fiat-Shamir, 1: test RNG
+  FsTranscript fs;
+  uint64_t rng;
+};
+
+effsc_hip_transcript* effsc_hip_transcript_new(const uint8_t* domsep, size_t len) {
+  auto* t = new effsc_hip_transcript();
+  t->kind = 0;
+  t->fs.init(domsep, domsep ? len : 0);
+  return t;
+}
+
+effsc_hip_transcript* effsc_hip_transcript_new_test(uint64_t seed) {
+  auto* t = new effsc_hip_transcript();
+  t->kind = 1;
+  t->rng = seed;
+  return t;
+}
+
+int effsc_hip_transcript_send(effsc_hip_transcript* t, int ext, const uint64_t* value) {
+  if (!t || !value || ext < 1 || ext > 3) return fail(EFFSC_HIP_EINVAL, "bad transcript argument");
+  if (t->kind == 0) t->fs.send(ext, value);
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_transcript_challenge(effsc_hip_transcript* t, int ext, uint64_t* out) {
+  if (!t || !out || ext < 1 || ext > 3) return fail(EFFSC_HIP_EINVAL, "bad transcript argument");
+  if (t->kind == 0) {
+    t->fs.challenge(ext, out);
+  } else {
+    // splitmix64; F::from_u64 reduces mod p (src/transcript/sanity.rs:42-44)
+    t->rng += 0x9E3779B97F4A7C15ull;
+    uint64_t z = t->rng;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[0] = gl::canon(z);
+    for (int i = 1; i < ext; i++) out[i] = 0;
+  }
+  return EFFSC_HIP_OK;
+}
+
+void effsc_hip_transcript_free(effsc_hip_transcript* t) { delete t; }
+
+// --------------------------------------------------------------------- runner
+int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, effsc_hip_hook hook,
+                       void* hook_ctx, uint64_t* round_polys, uint64_t* challenges, uint64_t* final_value) {
+  // src/runner.rs:29-69
+  if (!h || !t || !final_value || (num_rounds && (!round_polys || !challenges)))
+    return fail(EFFSC_HIP_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const int E = h->ext;
+  const size_t nw = (size_t)std::max(h->deg, 1);
+  uint64_t prev[3];
+  bool have = false;
+  for (size_t r = 0; r < num_rounds; r++) {
+    uint64_t* evals = round_polys + r * nw * E;
+    TRY(round_impl(h, have ? prev : nullptr, evals));
+    for (size_t i = 0; i < nw; i++) TRY(effsc_hip_transcript_send(t, E, evals + i * E));
+    if (hook) hook(hook_ctx, r, t);
+    TRY(effsc_hip_transcript_challenge(t, E, prev));
+    std::copy(prev, prev + E, challenges + r * E);
+    have = true;
+  }
+  if (have) {
+    TRY(finalize_impl(h, prev));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    TRY(prof_collect(h));
+  }
+  TRY(final_value_impl(h, final_value));
+  return EFFSC_HIP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------- verifier
+namespace {
+
+template <class F>
+F fconst(uint64_t x) {
+  return gl::lift_to<F>(Fp{gl::canon(x)});
+}
+
+// evaluate_from_evals + barycentric_weight: src/verifier.rs:146-200
+template <class F>
+F evaluate_from_evals(const std::vector<F>& ev, F r, uint64_t w) {
+  size_t d = ev.size();
+  if (d == 0) return gl::zero<F>();
+  if (d == 1) return ev[0];
+  if (d == 2) return gl::add(ev[0], gl::mul(r, gl::sub(ev[1], ev[0]), w));
+  std::vector<F> rm(d), prefix(d + 1, gl::one<F>()), suffix(d + 1, gl::one<F>());
+  for (size_t j = 0; j < d; j++) rm[j] = gl::sub(r, fconst<F>(j));
+  for (size_t i = 0; i < d; i++) prefix[i + 1] = gl::mul(prefix[i], rm[i], w);
+  for (size_t i = d; i-- > 0;) suffix[i] = gl::mul(suffix[i + 1], rm[i], w);
+  F result = gl::zero<F>();
+  for (size_t i = 0; i < d; i++) {
+    F num = gl::mul(prefix[i], suffix[i + 1], w);
+    F den = gl::one<F>();
+    for (size_t j = 0; j < d; j++) {
+      if (j == i) continue;
+      long long diff = (long long)i - (long long)j;
+      F df = diff > 0 ? fconst<F>((uint64_t)diff) : gl::sub(gl::zero<F>(), fconst<F>((uint64_t)(-diff)));
+      den = gl::mul(den, df, w);
+    }
+    if (!gl::is_zero(den)) result = gl::add(result, gl::mul(gl::mul(ev[i], num, w), gl::inv(den, w), w));
+  }
+  return result;
+}
+
+template <class F>
+int verify_t(uint64_t w, const uint64_t* claimed, size_t d, size_t num_rounds, effsc_hip_transcript* t,
+             const uint64_t* proof, size_t proof_words, effsc_hip_hook hook, void* ctx, uint64_t* challenges,
+             uint64_t* final_claim, size_t* err_round, int* err_kind) {
+  // src/verifier.rs:61-139
+  const int E = F::D;
+  F claim = gl::load_words<F>(claimed);
+  size_t cursor = 0;
+  for (size_t round = 0; round < num_rounds; round++) {
+    size_t n_wire = std::max<size_t>(d, 1);
+    std::vector<F> recv;
+    for (size_t i = 0; i < n_wire; i++) {
+      if (cursor + E > proof_words) {
+        if (err_round) *err_round = round;
+        if (err_kind) *err_kind = 1;
+        return fail(EFFSC_HIP_EVERIFY, "transcript error: proof exhausted");
+      }
+      recv.push_back(gl::load_words<F>(proof + cursor));
+      TRY(effsc_hip_transcript_send(t, E, proof + cursor));
+      cursor += E;
+    }
+    F h0 = recv[0];
+    F h1 = gl::sub(claim, h0);
+    if (hook && hook(ctx, round, t) != 0) {
+      if (err_round) *err_round = round;
+      if (err_kind) *err_kind = 2;
+      return fail(EFFSC_HIP_EVERIFY, "hook error");
+    }
+    uint64_t rw[3];
+    TRY(effsc_hip_transcript_challenge(t, E, rw));
+    std::copy(rw, rw + E, challenges + round * E);
+    F r = gl::load_words<F>(rw);
+    if (d == 0) {
+      claim = h0;
+      continue;
+    }
+    F h_inf = d >= 2 ? recv[1] : gl::sub(h1, h0);
+    std::vector<F> q;
+    q.push_back(h0);
+    q.push_back(gl::sub(h1, h_inf));
+    for (size_t i = 2; i < d; i++) {
+      F i_f = fconst<F>(i), i_d = gl::one<F>();
+      for (size_t j = 0; j < d; j++) i_d = gl::mul(i_d, i_f, w);
+      q.push_back(gl::sub(recv[i], gl::mul(h_inf, i_d, w)));
+    }
+    F q_r = evaluate_from_evals<F>(q, r, w);
+    F r_d = gl::one<F>();
+    for (size_t j = 0; j < d; j++) r_d = gl::mul(r_d, r, w);
+    claim = gl::add(q_r, gl::mul(h_inf, r_d, w));
+  }
+  gl::store_words(claim, final_claim);
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int effsc_hip_sumcheck_verify(int ext, uint64_t w, const uint64_t* claimed_sum, size_t degree,
+                                         size_t num_rounds, effsc_hip_transcript* t, const uint64_t* proof,
+                                         size_t proof_words, effsc_hip_hook hook, void* hook_ctx,
+                                         uint64_t* challenges, uint64_t* final_claim, size_t* err_round,
+                                         int* err_kind) {
+  if (!claimed_sum || !t || !final_claim || (num_rounds && !challenges) || (proof_words && !proof))
+    return fail(EFFSC_HIP_EINVAL, "null argument");
+  if (err_kind) *err_kind = 0;
+  if (ext == 1)
+    return verify_t<Fp>(0, claimed_sum, degree, num_rounds, t, proof, proof_words, hook, hook_ctx, challenges,
+                        final_claim, err_round, err_kind);
+  if (ext == 3)
+    return verify_t<Fp3>(gl::canon(w), claimed_sum, degree, num_rounds, t, proof, proof_words, hook, hook_ctx,
+                         challenges, final_claim, err_round, err_kind);
+  return fail(EFFSC_HIP_EINVAL, "ext must be 1 or 3");
+}
+
+// ------------------------------------------------------------------ utilities
+extern "C" int effsc_hip_device_alloc(int device, size_t words, uint64_t** out) {
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  HIPCHK(hipSetDevice(device));
+  return alloc_dev(out, words);
+}
+
+extern "C" int effsc_hip_device_free(int device, uint64_t* ptr) {
+  HIPCHK(hipSetDevice(device));
+  if (ptr) HIPCHK(hipFree(ptr));
+  return EFFSC_HIP_OK;
+}
+
+extern "C" int effsc_hip_copy(int device, uint64_t* dst, const uint64_t* src, size_t words) {
+  HIPCHK(hipSetDevice(device));
+  if (words) HIPCHK(hipMemcpy(dst, src, words * 8, hipMemcpyDefault));
+  return EFFSC_HIP_OK;
+}
+
+extern "C" int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint64_t seed, uint64_t first) {
+  if (!dst && n) return fail(EFFSC_HIP_EINVAL, "null destination");
+  HIPCHK(hipSetDevice(device));
+  if (!n) return EFFSC_HIP_OK;
+  uint64_t z = seed;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  size_t b = std::min<size_t>((n + kBlock - 1) / kBlock, 16384);
+  hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)b), dim3(kBlock), 0, nullptr, dst, n, z, first);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return EFFSC_HIP_OK;
+}

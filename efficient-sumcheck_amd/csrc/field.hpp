@@ -1,0 +1,186 @@
+// Goldilocks field p = 2^64 - 2^32 + 1 and its cubic extension, canonical form,
+// shared by the gfx950 kernels and the host protocol code.
+//
+// Every stored value is canonical (in [0, p)).  The reference keeps arkworks
+// Montgomery words (src/simd_fields/goldilocks/avx512.rs:1-35); words are
+// converted at the C-ABI boundary when the caller asks for that
+// representation, so the arithmetic here never sees Montgomery form.
+//
+// Multiplication: a 64x64 -> 128 product from four 32x32 -> 64 partial
+// products (v_mad_u64_u32 on gfx950), then the Goldilocks reduction
+// 2^64 = 2^32 - 1, 2^96 = -1 (mod p): x = lo + 2^64 (hl + 2^32 hh) is
+// congruent to lo - hh + hl (2^32 - 1).  No Montgomery step, no division.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GLHD __host__ __device__ __forceinline__
+
+namespace gl {
+
+constexpr uint64_t P = 0xFFFFFFFF00000001ull;
+constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+// 2^-64 mod p = -2^32 mod p: multiplying an arkworks Montgomery word by this
+// constant yields the canonical value.
+constexpr uint64_t MONT_TO_CANON = 0xFFFFFFFE00000001ull;
+// 2^64 mod p: canonical -> Montgomery word.
+constexpr uint64_t CANON_TO_MONT = EPS;
+
+GLHD uint64_t add(uint64_t a, uint64_t b) {
+  uint64_t s = a + b;
+  uint64_t r = (s < a) ? s + EPS : s;  // wrapped: a + b - 2^64 + eps = a + b - p
+  return (r >= P) ? r - P : r;
+}
+
+GLHD uint64_t sub(uint64_t a, uint64_t b) {
+  uint64_t d = a - b;
+  return (a < b) ? d + P : d;  // wrapping add of p
+}
+
+GLHD uint64_t neg(uint64_t a) { return a ? P - a : 0; }
+
+GLHD void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  uint64_t p00 = (uint64_t)a0 * b0;
+  uint64_t p01 = (uint64_t)a0 * b1;
+  uint64_t p10 = (uint64_t)a1 * b0;
+  uint64_t p11 = (uint64_t)a1 * b1;
+  // (2^32-1)^2 + 2 (2^32-1) = 2^64 - 1: no overflow
+  uint64_t mid = p01 + (p00 >> 32) + (uint32_t)p10;
+  lo = (mid << 32) | (uint32_t)p00;
+  hi = p11 + (mid >> 32) + (p10 >> 32);
+}
+
+// Reduce lo + 2^64 hi (any 128-bit value) to [0, p).
+GLHD uint64_t reduce128(uint64_t lo, uint64_t hi) {
+  uint64_t hh = hi >> 32;
+  uint64_t hl = hi & EPS;
+  uint64_t t0 = lo - hh;
+  if (lo < hh) t0 -= EPS;           // borrow: lo - hh + p
+  uint64_t t1 = (hl << 32) - hl;    // hl * (2^32 - 1) < 2^64
+  uint64_t r = t0 + t1;
+  if (r < t1) r += EPS;             // carry: cannot overflow again
+  return (r >= P) ? r - P : r;
+}
+
+GLHD uint64_t mul(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  return reduce128(lo, hi);
+}
+
+// Any u64 word to its canonical residue.
+GLHD uint64_t canon(uint64_t a) { return a >= P ? a - P : a; }
+
+GLHD uint64_t pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r = mul(r, a);
+    a = mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+GLHD uint64_t inv(uint64_t a) { return pow(a, P - 2); }
+
+// ---------------------------------------------------------------------------
+// Field element types.  Fp: base field.  Fp3: F_p[X]/(X^3 - w), w runtime.
+// ---------------------------------------------------------------------------
+struct Fp {
+  uint64_t v;
+  static constexpr int D = 1;
+};
+
+struct Fp3 {
+  uint64_t c0, c1, c2;
+  static constexpr int D = 3;
+};
+
+GLHD Fp add(Fp a, Fp b) { return {add(a.v, b.v)}; }
+GLHD Fp sub(Fp a, Fp b) { return {sub(a.v, b.v)}; }
+GLHD Fp mul(Fp a, Fp b, uint64_t) { return {mul(a.v, b.v)}; }
+GLHD Fp3 add(Fp3 a, Fp3 b) { return {add(a.c0, b.c0), add(a.c1, b.c1), add(a.c2, b.c2)}; }
+GLHD Fp3 sub(Fp3 a, Fp3 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1), sub(a.c2, b.c2)}; }
+
+// Karatsuba-style product mod X^3 - w: 6 base products + 2 products by w.
+GLHD Fp3 mul(Fp3 a, Fp3 b, uint64_t w) {
+  uint64_t v0 = mul(a.c0, b.c0);
+  uint64_t v1 = mul(a.c1, b.c1);
+  uint64_t v2 = mul(a.c2, b.c2);
+  uint64_t m12 = mul(add(a.c1, a.c2), add(b.c1, b.c2));  // a1b1 + a1b2 + a2b1 + a2b2
+  uint64_t m01 = mul(add(a.c0, a.c1), add(b.c0, b.c1));  // a0b0 + a0b1 + a1b0 + a1b1
+  uint64_t m02 = mul(add(a.c0, a.c2), add(b.c0, b.c2));  // a0b0 + a0b2 + a2b0 + a2b2
+  uint64_t x12 = sub(sub(m12, v1), v2);                  // a1b2 + a2b1
+  uint64_t x01 = sub(sub(m01, v0), v1);                  // a0b1 + a1b0
+  uint64_t x02 = sub(sub(m02, v0), v2);                  // a0b2 + a2b0
+  return {add(v0, mul(w, x12)), add(x01, mul(w, v2)), add(x02, v1)};
+}
+
+// Mixed products: extension element times base element.
+GLHD Fp3 mul(Fp3 a, Fp b, uint64_t) { return {mul(a.c0, b.v), mul(a.c1, b.v), mul(a.c2, b.v)}; }
+GLHD Fp3 mul(Fp a, Fp3 b, uint64_t) { return {mul(a.v, b.c0), mul(a.v, b.c1), mul(a.v, b.c2)}; }
+
+// Lift base -> extension.
+template <class F>
+GLHD F lift(Fp x);
+template <>
+GLHD Fp lift<Fp>(Fp x) { return x; }
+template <>
+GLHD Fp3 lift<Fp3>(Fp x) { return {x.v, 0, 0}; }
+
+// Lift into a (possibly) larger field.
+GLHD void lift_into(Fp x, Fp& o) { o = x; }
+GLHD void lift_into(Fp x, Fp3& o) { o = {x.v, 0, 0}; }
+GLHD void lift_into(Fp3 x, Fp3& o) { o = x; }
+template <class FO, class FI>
+GLHD FO lift_to(FI x) {
+  FO o;
+  lift_into(x, o);
+  return o;
+}
+
+template <class F>
+GLHD F zero();
+template <>
+GLHD Fp zero<Fp>() { return {0}; }
+template <>
+GLHD Fp3 zero<Fp3>() { return {0, 0, 0}; }
+
+template <class F>
+GLHD F one();
+template <>
+GLHD Fp one<Fp>() { return {1}; }
+template <>
+GLHD Fp3 one<Fp3>() { return {1, 0, 0}; }
+
+GLHD bool is_zero(Fp a) { return a.v == 0; }
+GLHD bool is_zero(Fp3 a) { return (a.c0 | a.c1 | a.c2) == 0; }
+
+GLHD Fp inv(Fp a, uint64_t) { return {a.v ? inv(a.v) : 0}; }
+// Inverse through the adjugate of the multiplication matrix.
+GLHD Fp3 inv(Fp3 a, uint64_t w) {
+  uint64_t t0 = sub(mul(a.c0, a.c0), mul(w, mul(a.c1, a.c2)));
+  uint64_t t1 = sub(mul(w, mul(a.c2, a.c2)), mul(a.c0, a.c1));
+  uint64_t t2 = sub(mul(a.c1, a.c1), mul(a.c0, a.c2));
+  uint64_t n = add(mul(a.c0, t0), mul(w, add(mul(a.c2, t1), mul(a.c1, t2))));
+  uint64_t ni = n ? inv(n) : 0;
+  return {mul(t0, ni), mul(t1, ni), mul(t2, ni)};
+}
+
+// Component access (element -> words).
+GLHD void store_words(Fp a, uint64_t* w) { w[0] = a.v; }
+GLHD void store_words(Fp3 a, uint64_t* w) {
+  w[0] = a.c0;
+  w[1] = a.c1;
+  w[2] = a.c2;
+}
+template <class F>
+GLHD F load_words(const uint64_t* w);
+template <>
+GLHD Fp load_words<Fp>(const uint64_t* w) { return {canon(w[0])}; }
+template <>
+GLHD Fp3 load_words<Fp3>(const uint64_t* w) { return {canon(w[0]), canon(w[1]), canon(w[2])}; }
+
+}  // namespace gl

@@ -1,0 +1,369 @@
+// gfx950 kernels for the half-split (MSB) sumcheck round.
+//
+// One family covers the multilinear (K = 1), inner-product (K = 2) and
+// product-of-K coefficient provers: the round polynomial of
+// prod_i f_i(x) over a pair (lo_i, hi_i) of every table is evaluated at the
+// EvalsInfty points {0, inf, 2, ..., K-1} (src/sumcheck_prover.rs:40-59):
+//   h(0)   = prod lo_i
+//   h(inf) = prod (hi_i - lo_i)          (leading coefficient)
+//   h(t)   = prod (lo_i + t (hi_i - lo_i)), t = 2..K-1
+// For K = 1 that is [sum lo] (MultilinearProver, src/provers/multilinear.rs:58-67)
+// and for K = 2 it is [q(0), q(inf)] (InnerProductProver,
+// src/provers/inner_product.rs:67-112).
+//
+// Kernels:
+//   k_eval   - round polynomial of the current table, no fold (round 0, or
+//              after a separate fold); pairs (k, k+half), implicit-zero high
+//              half for k >= paired (src/provers/inner_product.rs:87-110).
+//   k_fused  - fold with the previous challenge and evaluate in one pass over
+//              the quadruple (k, k+L/4, k+L/2, k+3L/4)
+//              (src/inner_product_sumcheck.rs:199-293,
+//               src/multilinear_sumcheck.rs:189-254).
+//   k_fold   - fold only, with the (1 - w) rule for the unpaired tail
+//              (src/inner_product_sumcheck.rs:143-176).
+// Field addition is exact, so the order of the cross-block reduction does not
+// change any output bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "field.hpp"
+
+namespace effsc_hip {
+
+using gl::Fp;
+using gl::Fp3;
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kMaxK = 4;             // tables per product
+constexpr int kMaxWaves = kBlock / 64;
+
+// ---------------------------------------------------------------------------
+// Table views.  Base tables are one u64 plane; Fp3 tables are three planes
+// (structure of arrays) `stride` words apart, so every load is coalesced.
+// ---------------------------------------------------------------------------
+template <class F>
+struct View;
+
+template <>
+struct View<Fp> {
+  uint64_t* p;
+  size_t stride;  // unused
+  __device__ __forceinline__ Fp ld(size_t i) const { return {p[i]}; }
+  __device__ __forceinline__ void st(size_t i, Fp x) const { p[i] = x.v; }
+  __device__ __forceinline__ void ld2(size_t i, Fp& a, Fp& b) const {
+    ulonglong2 q = *reinterpret_cast<const ulonglong2*>(p + i);
+    a.v = q.x;
+    b.v = q.y;
+  }
+  __device__ __forceinline__ void st2(size_t i, Fp a, Fp b) const {
+    *reinterpret_cast<ulonglong2*>(p + i) = make_ulonglong2(a.v, b.v);
+  }
+};
+
+template <>
+struct View<Fp3> {
+  uint64_t* p;
+  size_t stride;
+  __device__ __forceinline__ Fp3 ld(size_t i) const { return {p[i], p[stride + i], p[2 * stride + i]}; }
+  __device__ __forceinline__ void st(size_t i, Fp3 x) const {
+    p[i] = x.c0;
+    p[stride + i] = x.c1;
+    p[2 * stride + i] = x.c2;
+  }
+  __device__ __forceinline__ void ld2(size_t i, Fp3& a, Fp3& b) const {
+    ulonglong2 q0 = *reinterpret_cast<const ulonglong2*>(p + i);
+    ulonglong2 q1 = *reinterpret_cast<const ulonglong2*>(p + stride + i);
+    ulonglong2 q2 = *reinterpret_cast<const ulonglong2*>(p + 2 * stride + i);
+    a = {q0.x, q1.x, q2.x};
+    b = {q0.y, q1.y, q2.y};
+  }
+  __device__ __forceinline__ void st2(size_t i, Fp3 a, Fp3 b) const {
+    *reinterpret_cast<ulonglong2*>(p + i) = make_ulonglong2(a.c0, b.c0);
+    *reinterpret_cast<ulonglong2*>(p + stride + i) = make_ulonglong2(a.c1, b.c1);
+    *reinterpret_cast<ulonglong2*>(p + 2 * stride + i) = make_ulonglong2(a.c2, b.c2);
+  }
+};
+
+template <class F, int K>
+struct Views {
+  View<F> t[K];
+};
+
+// Reduction scratch owned by one prover: per-block partials, a ticket counter
+// (returned to zero by the last block) and the final K values.
+struct Reduce {
+  uint64_t* partials;  // gridDim * K * D words
+  unsigned* counter;
+  uint64_t* result;    // K * D words
+};
+
+// ---------------------------------------------------------------------------
+// Product-of-K pair evaluation (accumulates into acc[0..K)).
+// ---------------------------------------------------------------------------
+template <class F, int K>
+__device__ __forceinline__ void eval_pair(const F* lo, const F* hi, F* acc, uint64_t w) {
+  if constexpr (K == 1) {
+    acc[0] = gl::add(acc[0], lo[0]);
+  } else {
+    F d[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) d[i] = gl::sub(hi[i], lo[i]);
+    F p0 = lo[0], pinf = d[0];
+#pragma unroll
+    for (int i = 1; i < K; i++) {
+      p0 = gl::mul(p0, lo[i], w);
+      pinf = gl::mul(pinf, d[i], w);
+    }
+    acc[0] = gl::add(acc[0], p0);
+    acc[1] = gl::add(acc[1], pinf);
+    if constexpr (K >= 3) {
+      // x_t = lo + t d, stepped from x_1 = hi
+      F x[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) x[i] = hi[i];
+#pragma unroll
+      for (int t = 2; t < K; t++) {
+#pragma unroll
+        for (int i = 0; i < K; i++) x[i] = gl::add(x[i], d[i]);
+        F pt = x[0];
+#pragma unroll
+        for (int i = 1; i < K; i++) pt = gl::mul(pt, x[i], w);
+        acc[t] = gl::add(acc[t], pt);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block reduction of K field elements.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  int lo = __shfl_xor((int)(uint32_t)v, m, 64);
+  int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ Fp shfl_xor(Fp a, int m) { return {shfl_xor64(a.v, m)}; }
+__device__ __forceinline__ Fp3 shfl_xor(Fp3 a, int m) {
+  return {shfl_xor64(a.c0, m), shfl_xor64(a.c1, m), shfl_xor64(a.c2, m)};
+}
+
+// Reduces acc over the block; the result is valid in thread 0.
+template <class F, int K>
+__device__ __forceinline__ void block_reduce(F* acc) {
+  __shared__ uint64_t lds[kMaxWaves][kMaxK * 3];
+#pragma unroll
+  for (int i = 0; i < K; i++)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc[i] = gl::add(acc[i], shfl_xor(acc[i], m));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < K; i++) gl::store_words(acc[i], &lds[wave][i * F::D]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int wv = 1; wv < nw; wv++)
+#pragma unroll
+      for (int i = 0; i < K; i++) acc[i] = gl::add(acc[i], gl::load_words<F>(&lds[wv][i * F::D]));
+  }
+  __syncthreads();
+}
+
+// Writes this block's partial; the last block to arrive sums all partials,
+// writes the K results and re-arms the counter.
+template <class F, int K>
+__device__ __forceinline__ void finish(F* acc, Reduce r) {
+  block_reduce<F, K>(acc);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < K; i++) gl::store_words(acc[i], r.partials + ((size_t)blockIdx.x * K + i) * F::D);
+    __threadfence();
+    unsigned ticket = atomicAdd(r.counter, 1u);
+    last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  F s[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) s[i] = gl::zero<F>();
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      const uint64_t* src = r.partials + ((size_t)b * K + i) * F::D;
+      uint64_t wds[3];
+#pragma unroll
+      for (int c = 0; c < F::D; c++) wds[c] = __atomic_load_n(src + c, __ATOMIC_RELAXED);
+      s[i] = gl::add(s[i], gl::load_words<F>(wds));
+    }
+  }
+  block_reduce<F, K>(s);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < K; i++) gl::store_words(s[i], r.result + i * F::D);
+    *r.counter = 0;
+    __threadfence();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fold helper: lo + c (hi - lo), mixed input/output fields.
+// ---------------------------------------------------------------------------
+template <class FO, class FI>
+__device__ __forceinline__ FO fold1(FI lo, FI hi, FO c, uint64_t w) {
+  return gl::add(gl::mul(c, gl::sub(hi, lo), w), gl::lift_to<FO>(lo));
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t gtid() { return (size_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Round polynomial of K tables of length L without folding.  Pairs
+// (k, k + half) for k < paired; (k, 0) for paired <= k < half.
+// VEC = 2 requires paired == half and even half.
+template <class F, int K, int VEC>
+__global__ __launch_bounds__(kBlock) void k_eval(Views<F, K> t, size_t half, size_t paired, uint64_t w,
+                                                 Reduce r) {
+  F acc[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) acc[i] = gl::zero<F>();
+  const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
+  if constexpr (VEC == 2) {
+    for (size_t k = gtid() * 2; k < half; k += step) {
+      F lo0[K], lo1[K], hi0[K], hi1[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        t.t[i].ld2(k, lo0[i], lo1[i]);
+        t.t[i].ld2(k + half, hi0[i], hi1[i]);
+      }
+      eval_pair<F, K>(lo0, hi0, acc, w);
+      eval_pair<F, K>(lo1, hi1, acc, w);
+    }
+  } else {
+    for (size_t k = gtid(); k < half; k += step) {
+      F lo[K], hi[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        lo[i] = t.t[i].ld(k);
+        hi[i] = k < paired ? t.t[i].ld(k + half) : gl::zero<F>();
+      }
+      eval_pair<F, K>(lo, hi, acc, w);
+    }
+  }
+  finish<F, K>(acc, r);
+}
+
+// Fold the pow2 length-L tables `in` with c into `out` (length L/2) and
+// evaluate the next round polynomial over (k, k + L/4).  `in` and `out` may
+// alias: positions k and k+q are read and written by the same thread only,
+// positions k+2q and k+3q are only read.  q = L/4.  VEC = 2 requires even q.
+template <class FI, class FO, int K, int VEC>
+__global__ __launch_bounds__(kBlock) void k_fused(Views<FI, K> in, Views<FO, K> out, FO c, size_t q, uint64_t w,
+                                                  Reduce r) {
+  FO acc[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) acc[i] = gl::zero<FO>();
+  const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
+  if constexpr (VEC == 2) {
+    for (size_t k = gtid() * 2; k < q; k += step) {
+      FI x0a[K], x0b[K], x1a[K], x1b[K], x2a[K], x2b[K], x3a[K], x3b[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        in.t[i].ld2(k, x0a[i], x0b[i]);
+        in.t[i].ld2(k + q, x1a[i], x1b[i]);
+        in.t[i].ld2(k + 2 * q, x2a[i], x2b[i]);
+        in.t[i].ld2(k + 3 * q, x3a[i], x3b[i]);
+      }
+      FO loa[K], lob[K], hia[K], hib[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        loa[i] = fold1<FO>(x0a[i], x2a[i], c, w);
+        lob[i] = fold1<FO>(x0b[i], x2b[i], c, w);
+        hia[i] = fold1<FO>(x1a[i], x3a[i], c, w);
+        hib[i] = fold1<FO>(x1b[i], x3b[i], c, w);
+        out.t[i].st2(k, loa[i], lob[i]);
+        out.t[i].st2(k + q, hia[i], hib[i]);
+      }
+      eval_pair<FO, K>(loa, hia, acc, w);
+      eval_pair<FO, K>(lob, hib, acc, w);
+    }
+  } else {
+    for (size_t k = gtid(); k < q; k += step) {
+      FO lo[K], hi[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        FI x0 = in.t[i].ld(k), x1 = in.t[i].ld(k + q), x2 = in.t[i].ld(k + 2 * q), x3 = in.t[i].ld(k + 3 * q);
+        lo[i] = fold1<FO>(x0, x2, c, w);
+        hi[i] = fold1<FO>(x1, x3, c, w);
+        out.t[i].st(k, lo[i]);
+        out.t[i].st(k + q, hi[i]);
+      }
+      eval_pair<FO, K>(lo, hi, acc, w);
+    }
+  }
+  finish<FO, K>(acc, r);
+}
+
+// Fold only: out[k] = in[k] + c (in[k+half] - in[k]) for k < paired and
+// in[k] (1 - c) for paired <= k < half.  May run in place (thread k reads k
+// and k+half and writes k).
+template <class FI, class FO, int K>
+__global__ __launch_bounds__(kBlock) void k_fold(Views<FI, K> in, Views<FO, K> out, FO c, size_t half,
+                                                 size_t paired, uint64_t w) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < half; k += step) {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      FI lo = in.t[i].ld(k);
+      FI hi = k < paired ? in.t[i].ld(k + half) : gl::zero<FI>();
+      out.t[i].st(k, fold1<FO>(lo, hi, c, w));
+    }
+  }
+}
+
+// AoS caller words -> canonical SoA planes (optionally from Montgomery form).
+__global__ __launch_bounds__(kBlock) void k_import(uint64_t* dst, size_t stride, const uint64_t* src, size_t n,
+                                                   int d, int mont) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    for (int c = 0; c < d; c++) {
+      uint64_t x = src[i * d + c];
+      x = mont ? gl::mul(x, gl::MONT_TO_CANON) : gl::canon(x);
+      dst[c * stride + i] = x;
+    }
+  }
+}
+
+// Canonical SoA planes -> AoS caller words (optionally to Montgomery form).
+__global__ __launch_bounds__(kBlock) void k_export(uint64_t* dst, const uint64_t* src, size_t stride, size_t n,
+                                                   int d, int mont) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    for (int c = 0; c < d; c++) {
+      uint64_t x = src[c * stride + i];
+      dst[i * d + c] = mont ? gl::mul(x, gl::CANON_TO_MONT) : x;
+    }
+  }
+}
+
+// Counter-based uniform values in [0, p): first candidate below p of
+// mix(mix(seed) + i G + a K) (splitmix64 finaliser).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(kBlock) void k_fill_uniform(uint64_t* dst, size_t n, uint64_t key, uint64_t first) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    uint64_t base = key + (first + i) * 0x9E3779B97F4A7C15ull;
+    uint64_t c = mix64(base);
+    for (uint64_t a = 1; c >= gl::P; a++) c = mix64(base + a * 0xD1B54A32D192ED03ull);
+    dst[i] = c;
+  }
+}
+
+}  // namespace effsc_hip

@@ -1,0 +1,118 @@
+// Keccak-f[1600] (FIPS 202) and the SHAKE256-based Fiat-Shamir transcript,
+// usable from host and device code.
+//
+// Transcript byte stream (the reference's transcript is the external
+// spongefish crate, src/transcript/spongefish.rs:17-44, whose wire encoding is
+// not part of the reference; this library fixes its own):
+//   "effsc-hip/fs/v1" || u64le(len(domsep)) || domsep
+//   send(x)      : 0x01 || u64le(canonical component) for each component
+//   challenge(d) : SHAKE256(stream || 0x02) -> d x 16 bytes, each read as a
+//                  little-endian u128 and reduced mod p; then 0x02 is
+//                  appended to the stream.
+// The sponge is kept incrementally, so each operation costs O(1) permutations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "field.hpp"
+
+namespace effsc_hip {
+
+#define SKHD __host__ __device__ __forceinline__
+
+SKHD uint64_t rotl64(uint64_t x, int r) { return r ? (x << r) | (x >> (64 - r)) : x; }
+
+// Round constants and rotation offsets of Keccak-f[1600] (FIPS 202, 3.2).
+SKHD uint64_t keccak_rc(int i) {
+  const uint64_t rc[24] = {
+      0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
+      0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
+      0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
+      0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
+      0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
+      0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+  return rc[i];
+}
+
+__host__ __device__ inline void keccak_f1600(uint64_t* s) {
+  const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+  for (int round = 0; round < 24; round++) {
+    uint64_t c[5], b[25];
+    for (int x = 0; x < 5; x++) c[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
+    for (int x = 0; x < 5; x++) {
+      uint64_t d = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
+      for (int y = 0; y < 25; y += 5) s[x + y] ^= d;
+    }
+    // rho + pi: lane (x, y) moves to (y, 2x + 3y)
+    for (int x = 0; x < 5; x++)
+      for (int y = 0; y < 5; y++) b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(s[x + 5 * y], rot[x + 5 * y]);
+    for (int y = 0; y < 25; y += 5)
+      for (int x = 0; x < 5; x++) s[x + y] = b[x + y] ^ (~b[(x + 1) % 5 + y] & b[(x + 2) % 5 + y]);
+    s[0] ^= keccak_rc(round);
+  }
+}
+
+constexpr int kShakeRate = 136;
+
+struct Sponge {
+  uint64_t st[25];
+  uint32_t pos;
+
+  SKHD void init() {
+    for (int i = 0; i < 25; i++) st[i] = 0;
+    pos = 0;
+  }
+  SKHD void absorb_byte(uint8_t b) {
+    st[pos >> 3] ^= (uint64_t)b << (8 * (pos & 7));
+    if (++pos == kShakeRate) {
+      keccak_f1600(st);
+      pos = 0;
+    }
+  }
+  SKHD void absorb(const uint8_t* m, size_t n) {
+    for (size_t i = 0; i < n; i++) absorb_byte(m[i]);
+  }
+  SKHD void absorb_u64(uint64_t x) {
+    for (int j = 0; j < 8; j++) absorb_byte((uint8_t)(x >> (8 * j)));
+  }
+  // Pads a copy with the SHAKE suffix and squeezes up to one rate block.
+  SKHD void squeeze_copy(uint8_t* out, int outlen) const {
+    Sponge c = *this;
+    c.st[c.pos >> 3] ^= (uint64_t)0x1F << (8 * (c.pos & 7));
+    c.st[(kShakeRate - 1) >> 3] ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) & 7));
+    keccak_f1600(c.st);
+    for (int i = 0; i < outlen; i++) out[i] = (uint8_t)(c.st[i >> 3] >> (8 * (i & 7)));
+  }
+};
+
+// u128 (little-endian bytes) mod p
+SKHD uint64_t u128_mod_p(const uint8_t* b) {
+  uint64_t lo = 0, hi = 0;
+  for (int j = 0; j < 8; j++) lo |= (uint64_t)b[j] << (8 * j);
+  for (int j = 0; j < 8; j++) hi |= (uint64_t)b[8 + j] << (8 * j);
+  return gl::reduce128(lo, hi);
+}
+
+struct FsTranscript {
+  Sponge sp;
+
+  SKHD void init(const uint8_t* domsep, size_t len) {
+    const char tag[] = "effsc-hip/fs/v1";
+    sp.init();
+    sp.absorb(reinterpret_cast<const uint8_t*>(tag), sizeof(tag) - 1);
+    sp.absorb_u64((uint64_t)len);
+    sp.absorb(domsep, len);
+  }
+  SKHD void send(int d, const uint64_t* v) {
+    sp.absorb_byte(0x01);
+    for (int i = 0; i < d; i++) sp.absorb_u64(gl::canon(v[i]));
+  }
+  SKHD void challenge(int d, uint64_t* out) {
+    sp.absorb_byte(0x02);  // the stream grows by 0x02 ...
+    uint8_t buf[48];
+    sp.squeeze_copy(buf, 16 * d);  // ... and the squeeze reads stream || 0x02
+    for (int i = 0; i < d; i++) out[i] = u128_mod_p(buf + 16 * i);
+  }
+};
+
+}  // namespace effsc_hip

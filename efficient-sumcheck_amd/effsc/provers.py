@@ -1,0 +1,262 @@
+"""GPU-backed provers behind the reference's SumcheckProver trait
+(src/sumcheck_prover.rs:36-71): degree(), round(challenge|None), finalize(r),
+final_value(), plus each prover's post-state accessors.
+
+Every method forwards to the C ABI (include/effsc_hip.h); the tables live in
+device memory for the prover's lifetime.
+
+Inputs: numpy uint64 arrays of words (AoS, `input_ext` words per element) or
+sequences of elements (ints, or 3-tuples for ext = 3).  With borrow=True the
+inputs are device addresses (ints) of canonical planes, see
+EFFSC_HIP_BORROW_INPUT.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .field import CANONICAL, MONT64, P, W_FROM_RAW7, Field
+
+BORROW_INPUT = 0x1
+
+
+def _words(table, input_ext):
+    if isinstance(table, np.ndarray):
+        return np.ascontiguousarray(table, dtype=np.uint64)
+    out = []
+    for x in table:
+        if isinstance(x, (tuple, list)):
+            out.extend(int(c) % P for c in x)
+        else:
+            out.append(int(x) % P)
+    return np.array(out, dtype=np.uint64)
+
+
+def _config(ext, input_ext, repr, w, device, flags):
+    cfg = N.Config()
+    N.lib().effsc_hip_config_default(C.byref(cfg))
+    cfg.ext = ext
+    cfg.input_ext = input_ext
+    cfg.repr = repr
+    cfg.ext_w = w
+    cfg.device = device
+    cfg.flags = flags
+    return cfg
+
+
+class NativeProver:
+    """Common forwarding to the C ABI."""
+
+    def __init__(self, ext, repr):
+        self.h = None
+        self.ext = ext
+        self.repr = repr
+        self.field = Field(ext)
+
+    def _finish(self, handle):
+        self.h = handle
+        self._deg = self._size(N.lib().effsc_hip_degree)
+
+    def _size(self, fn, *args):
+        out = C.c_size_t(0)
+        N.check(fn(self.h, *args, C.byref(out)))
+        return out.value
+
+    def _elem(self, words):
+        return int(words[0]) if self.ext == 1 else tuple(int(x) for x in words[:3])
+
+    def _chal_words(self, c):
+        if self.ext == 1:
+            return np.array([int(c)], dtype=np.uint64)
+        return np.array([int(x) for x in c], dtype=np.uint64)
+
+    # ---- trait
+    def degree(self):
+        return self._deg
+
+    def round(self, challenge=None):
+        nw = max(self._deg, 1)
+        out = np.zeros(nw * self.ext, dtype=np.uint64)
+        if challenge is None:
+            N.check(N.lib().effsc_hip_round(self.h, None, out.ctypes.data_as(N.u64p)))
+        else:
+            cw = self._chal_words(challenge)
+            N.check(N.lib().effsc_hip_round(self.h, cw.ctypes.data_as(N.u64p), out.ctypes.data_as(N.u64p)))
+        return [self._elem(out[i * self.ext:(i + 1) * self.ext]) for i in range(nw)]
+
+    def finalize(self, last_challenge):
+        cw = self._chal_words(last_challenge)
+        N.check(N.lib().effsc_hip_finalize(self.h, cw.ctypes.data_as(N.u64p)))
+
+    def final_value(self):
+        out = np.zeros(self.ext, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_final_value(self.h, out.ctypes.data_as(N.u64p)))
+        return self._elem(out)
+
+    # ---- post-state
+    def table_words(self, i):
+        n = self._size(N.lib().effsc_hip_table_len, i)
+        out = np.zeros(max(n * self.ext, 1), dtype=np.uint64)
+        got = C.c_size_t(0)
+        N.check(N.lib().effsc_hip_read_table(self.h, i, out.ctypes.data_as(N.u64p), n * self.ext, C.byref(got)))
+        return out[:n * self.ext]
+
+    def table(self, i):
+        w = self.table_words(i)
+        return [self._elem(w[j * self.ext:(j + 1) * self.ext]) for j in range(len(w) // self.ext)]
+
+    def reset(self):
+        N.check(N.lib().effsc_hip_reset(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().effsc_hip_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class MultilinearProver(NativeProver):
+    """MultilinearProver (src/provers/multilinear.rs:25-80), degree 1."""
+
+    def __init__(self, evals, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0,
+                 borrow=False, n=None):
+        super().__init__(ext, repr)
+        input_ext = ext if input_ext is None else input_ext
+        cfg = _config(ext, input_ext, repr, w, device, BORROW_INPUT if borrow else 0)
+        h = C.c_void_p()
+        if borrow:
+            N.check(N.lib().effsc_hip_multilinear_new(C.byref(cfg), C.cast(C.c_void_p(evals), N.u64p), n,
+                                                       C.byref(h)))
+        else:
+            e = _words(evals, input_ext)
+            N.check(N.lib().effsc_hip_multilinear_new(C.byref(cfg), e.ctypes.data_as(N.u64p), len(e) // input_ext,
+                                                       C.byref(h)))
+        self._finish(h)
+
+    def evals(self):
+        return self.table(0)
+
+    def num_variables(self):
+        n = self._size(N.lib().effsc_hip_table_len, 0)
+        return 0 if n == 0 else (n - 1).bit_length()
+
+
+class InnerProductProver(NativeProver):
+    """InnerProductProver (src/provers/inner_product.rs:26-127), degree 2."""
+
+    def __init__(self, a, b, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False,
+                 n=None):
+        super().__init__(ext, repr)
+        input_ext = ext if input_ext is None else input_ext
+        cfg = _config(ext, input_ext, repr, w, device, BORROW_INPUT if borrow else 0)
+        h = C.c_void_p()
+        if borrow:
+            N.check(N.lib().effsc_hip_inner_product_new(C.byref(cfg), C.cast(C.c_void_p(a), N.u64p),
+                                                         C.cast(C.c_void_p(b), N.u64p), n, C.byref(h)))
+        else:
+            A, B = _words(a, input_ext), _words(b, input_ext)
+            if len(A) != len(B):
+                raise ValueError("a and b must have equal length")  # src/provers/inner_product.rs:34
+            N.check(N.lib().effsc_hip_inner_product_new(C.byref(cfg), A.ctypes.data_as(N.u64p),
+                                                         B.ctypes.data_as(N.u64p), len(A) // input_ext, C.byref(h)))
+        self._finish(h)
+
+    def evaluations(self):
+        return self.table(0), self.table(1)
+
+    def final_evaluations(self):
+        a, b = self.evaluations()
+        if len(a) == 1:
+            return a[0], b[0]
+        return self.field.zero(), self.field.zero()
+
+
+class CoefficientProver(NativeProver):
+    """CoefficientProver with the built-in product-of-k evaluator
+    (src/provers/coefficient.rs:18-330; the k = 3 evaluator is the
+    reference's TripleProductEval, tests/adversarial_verifier.rs:444-465):
+    proves sum_x prod_i f_i(x), degree k (1 <= k <= 4)."""
+
+    def __init__(self, pairwise, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0,
+                 borrow=False, n=None):
+        super().__init__(ext, repr)
+        input_ext = ext if input_ext is None else input_ext
+        k = len(pairwise)
+        cfg = _config(ext, input_ext, repr, w, device, BORROW_INPUT if borrow else 0)
+        h = C.c_void_p()
+        if borrow:
+            ptrs = (N.u64p * k)(*[C.cast(C.c_void_p(p), N.u64p) for p in pairwise])
+            N.check(N.lib().effsc_hip_product_new(C.byref(cfg), ptrs, k, n, C.byref(h)))
+        else:
+            arrs = [_words(t, input_ext) for t in pairwise]
+            if len({len(a) for a in arrs}) != 1:
+                raise ValueError("pairwise tables must have equal length")
+            ptrs = (N.u64p * k)(*[a.ctypes.data_as(N.u64p) for a in arrs])
+            N.check(N.lib().effsc_hip_product_new(C.byref(cfg), ptrs, k, len(arrs[0]) // input_ext, C.byref(h)))
+        self._finish(h)
+
+    def pairwise(self):
+        return [self.table(i) for i in range(self._size(N.lib().effsc_hip_table_count))]
+
+
+ProductProver = CoefficientProver
+
+
+class EqFactoredProver(NativeProver):
+    """EqFactoredProver (src/provers/eq_factored.rs:67-289): sum_x eq(w,x) p(x)."""
+
+    def __init__(self, w_point, p_evals, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0):
+        super().__init__(ext, repr)
+        cfg = _config(ext, ext, repr, w, device, 0)
+        W = _words(w_point, ext)
+        Pv = _words(p_evals, ext)
+        h = C.c_void_p()
+        N.check(N.lib().effsc_hip_eq_factored_new(C.byref(cfg), W.ctypes.data_as(N.u64p), len(W) // ext,
+                                                   Pv.ctypes.data_as(N.u64p), len(Pv) // ext, C.byref(h)))
+        self._finish(h)
+
+    def final_factors(self):
+        p = self.table(0)
+        if len(p) == 1:
+            l, r = self.table(1)[0], self.table(2)[0]
+            return p[0], self.field.mul(l, r)
+        return self.field.zero(), self.field.zero()
+
+
+def fold(values, weight, device=0):
+    """fold(&mut Vec<F>, weight) (src/fold.rs:14) for base-field tables:
+    returns the folded list (length next_pow2(n)/2), computed on the device."""
+    v = _words(values, 1)
+    n = len(v)
+    if n <= 1:
+        return [int(x) for x in v]
+    ptr = N.u64p()
+    N.check(N.lib().effsc_hip_device_alloc(device, n, C.byref(ptr)))
+    try:
+        N.check(N.lib().effsc_hip_copy(device, ptr, v.ctypes.data_as(N.u64p), n))
+        m = fold_device(C.cast(ptr, C.c_void_p).value, n, weight, device)
+        out = np.zeros(m, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_copy(device, out.ctypes.data_as(N.u64p), ptr, m))
+    finally:
+        N.lib().effsc_hip_device_free(device, ptr)
+    return [int(x) for x in out]
+
+
+def fold_device(ptr, n, weight, device=0):
+    """In-place fold of n canonical words at device address `ptr`."""
+    out = C.c_size_t(0)
+    N.check(N.lib().effsc_hip_fold(device, C.cast(C.c_void_p(ptr), N.u64p), n, int(weight) % P, C.byref(out)))
+    return out.value

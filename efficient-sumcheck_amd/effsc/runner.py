@@ -1,0 +1,74 @@
+"""runner::sumcheck (src/runner.rs:29-69).
+
+A native prover driven with a native transcript runs the whole loop in the
+library's C++ runner (effsc_hip_sumcheck); any other combination runs the
+same loop here.  The hook is called after the round's values are sent and
+before the challenge is read; `None` is the reference's noop_hook
+(src/lib.rs:28-30).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .proof import SumcheckProof
+from .provers import NativeProver
+from .transcript import NativeTranscript
+
+
+def noop_hook(round, transcript):
+    return None
+
+
+def sumcheck(prover, num_rounds, transcript, hook=None) -> SumcheckProof:
+    if hook is noop_hook:
+        hook = None
+    if isinstance(prover, NativeProver) and isinstance(transcript, NativeTranscript):
+        if transcript.ext != prover.ext:
+            raise ValueError("transcript and prover fields differ")
+        if prover.repr == 0:
+            return _native_sumcheck(prover, num_rounds, transcript, hook)
+    round_polys, challenges = [], []
+    prev = None
+    for rnd in range(num_rounds):
+        evals = prover.round(prev)
+        for v in evals:
+            transcript.send(v)
+        round_polys.append(evals)
+        if hook is not None:
+            hook(rnd, transcript)
+        r = transcript.challenge()
+        challenges.append(r)
+        prev = r
+    if prev is not None:
+        prover.finalize(prev)
+    return SumcheckProof(round_polys, challenges, prover.final_value())
+
+
+def _native_sumcheck(prover, num_rounds, transcript, hook):
+    E = prover.ext
+    nw = max(prover.degree(), 1)
+    polys = np.zeros(max(num_rounds, 1) * nw * E, dtype=np.uint64)
+    chals = np.zeros(max(num_rounds, 1) * E, dtype=np.uint64)
+    fin = np.zeros(E, dtype=np.uint64)
+    errors = []
+    if hook is None:
+        cb = C.cast(None, N.HOOK)
+    else:
+        def _cb(ctx, rnd, t):
+            try:
+                hook(rnd, transcript)
+            except BaseException as e:  # surfaced after the native call returns
+                errors.append(e)
+            return 0
+        cb = N.HOOK(_cb)
+    N.check(N.lib().effsc_hip_sumcheck(prover.h, num_rounds, transcript.h, cb, None, polys.ctypes.data_as(N.u64p),
+                                       chals.ctypes.data_as(N.u64p), fin.ctypes.data_as(N.u64p)))
+    if errors:
+        raise errors[0]
+    el = prover._elem
+    rp = [[el(polys[(r * nw + i) * E:(r * nw + i + 1) * E]) for i in range(nw)] for r in range(num_rounds)]
+    ch = [el(chals[r * E:(r + 1) * E]) for r in range(num_rounds)]
+    return SumcheckProof(rp, ch, el(fin))

@@ -1,0 +1,210 @@
+/*
+ * effsc_hip.h — C ABI of the MI355X (gfx950) Goldilocks sumcheck prover.
+ *
+ * Drop-in boundary for the reference crate `effsc` 0.0.2
+ * (compsec-epfl/efficient-sumcheck).  The reference's boundary is the Rust
+ * trait `SumcheckProver<F>` (src/sumcheck_prover.rs:36-71) consumed by
+ * `runner::sumcheck` (src/runner.rs:29-69); a foreign binding implements that
+ * trait by forwarding each method to the entry point named beside it below.
+ *
+ * Conventions
+ *  - Field elements are u64 words: `ext` words per element (ext = 1 for
+ *    F_p, 3 for F_p[X]/(X^3 - w)), components in order c0, c1, c2 (the
+ *    arkworks layout the reference reinterprets through _as_u64_slice,
+ *    src/field.rs:118-145).
+ *  - `repr` selects the word encoding at this boundary: canonical values in
+ *    [0, p), or arkworks Montgomery words (R = 2^64), which is what the
+ *    reference's F64 stores (src/tests/fields.rs:26-30).  Device tables are
+ *    always canonical.
+ *  - Every function returns an effsc_hip_status; no C++ exception crosses the
+ *    boundary.  On failure effsc_hip_last_error() describes it (thread-local).
+ *    EINVAL is where the reference panics (e.g. src/provers/inner_product.rs:34,
+ *    src/provers/eq_factored.rs:93-99).
+ *  - A prover handle is not thread-safe; distinct handles may be driven from
+ *    distinct threads.  Each handle owns one HIP stream on its device.
+ *  - Input tables are copied to the device at construction (host or device
+ *    source pointers are both accepted) unless EFFSC_HIP_BORROW_INPUT is set.
+ *    Folding never writes the input tables: the first fold writes a separate
+ *    half-size work table, so effsc_hip_reset() restarts a proof for free.
+ */
+#ifndef EFFSC_HIP_H
+#define EFFSC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EFFSC_HIP_OK = 0,
+  EFFSC_HIP_EINVAL = 1,   /* bad argument (the reference panics here) */
+  EFFSC_HIP_ENOMEM = 2,   /* device or pinned host allocation failed */
+  EFFSC_HIP_EDEVICE = 3,  /* HIP runtime error */
+  EFFSC_HIP_ESTATE = 4,   /* call not valid in the prover's current state */
+  EFFSC_HIP_EVERIFY = 5   /* verifier: transcript error (proof too short) or hook error */
+} effsc_hip_status;
+
+typedef enum { EFFSC_HIP_CANONICAL = 0, EFFSC_HIP_MONT64 = 1 } effsc_hip_repr;
+
+/* Input flag: `tables` are device pointers to canonical words in the device
+ * layout (base: one plane; ext3: three planes of `n` words each), owned by the
+ * caller and kept alive and unmodified until effsc_hip_free. */
+#define EFFSC_HIP_BORROW_INPUT 0x1u
+
+/* The reference's extension non-residue F64::from_raw(7), canonical value
+ * 7 * 2^-64 mod p (src/tests/fields.rs:49,80). */
+#define EFFSC_HIP_W_FROM_RAW7 0xFFFFFFF800000001ull
+
+typedef struct {
+  int ext;          /* field of tables, challenges and round polynomials: 1 or 3 */
+  int input_ext;    /* field of the input words: 1 (lifted when ext = 3) or ext */
+  int repr;         /* effsc_hip_repr of every word crossing this ABI */
+  uint64_t ext_w;   /* canonical non-residue w of X^3 - w (ignored when ext = 1) */
+  int device;       /* HIP device ordinal */
+  unsigned flags;   /* EFFSC_HIP_BORROW_INPUT */
+} effsc_hip_config;
+
+typedef struct effsc_hip_prover effsc_hip_prover;
+
+/* Fills `cfg` with ext = input_ext = 1, canonical words, w = from_raw(7),
+ * device 0, no flags. */
+void effsc_hip_config_default(effsc_hip_config* cfg);
+
+/* ---------------------------------------------------------------- provers */
+
+/* MultilinearProver::new(evals) — src/provers/multilinear.rs:31-35.
+ * Degree 1; any n >= 0 (non-powers of two are implicitly zero-padded). */
+int effsc_hip_multilinear_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n,
+                              effsc_hip_prover** out);
+
+/* InnerProductProver::new(a, b) — src/provers/inner_product.rs:33-36.
+ * Degree 2; a and b have equal length n. */
+int effsc_hip_inner_product_new(const effsc_hip_config* cfg, const uint64_t* a, const uint64_t* b, size_t n,
+                                effsc_hip_prover** out);
+
+/* CoefficientProver::new(&evaluator, vec![], pairwise) with the built-in
+ * product-of-k evaluator, sum_x prod_i f_i(x) (degree k, 1 <= k <= 4) —
+ * src/provers/coefficient.rs:28-40; the k = 3 evaluator is the reference's
+ * TripleProductEval (tests/adversarial_verifier.rs:444-465).  All k tables
+ * have length n, a power of two. */
+int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
+                          effsc_hip_prover** out);
+
+/* EqFactoredProver::new(w, p) — src/provers/eq_factored.rs:90-114.
+ * Degree 2; `point` holds v elements, p holds n <= 2^v elements (zero-padded). */
+int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point, size_t v, const uint64_t* p,
+                              size_t n, effsc_hip_prover** out);
+
+/* SumcheckProver::degree — src/sumcheck_prover.rs:37-38. */
+int effsc_hip_degree(const effsc_hip_prover* h, size_t* out);
+
+/* Extension degree of the prover's field (words per element). */
+int effsc_hip_ext(const effsc_hip_prover* h, size_t* out);
+
+/* SumcheckProver::round — src/sumcheck_prover.rs:40-59.  `challenge` is NULL
+ * for Option::None, else `ext` words; `out` receives max(degree, 1) * ext
+ * words in the EvalsInfty wire format. */
+int effsc_hip_round(effsc_hip_prover* h, const uint64_t* challenge, uint64_t* out);
+
+/* SumcheckProver::finalize — src/sumcheck_prover.rs:61-66. */
+int effsc_hip_finalize(effsc_hip_prover* h, const uint64_t* challenge);
+
+/* SumcheckProver::final_value — src/sumcheck_prover.rs:68-70 (`ext` words). */
+int effsc_hip_final_value(effsc_hip_prover* h, uint64_t* out);
+
+/* Post-state accessors: evals() (src/provers/multilinear.rs:45-47),
+ * evaluations() (src/provers/inner_product.rs:39-41), pairwise()
+ * (src/provers/coefficient.rs:46-48).  EqFactored exposes p, eq_l, eq_r as
+ * tables 0, 1, 2 (final_factors, src/provers/eq_factored.rs:117-123). */
+int effsc_hip_table_count(const effsc_hip_prover* h, size_t* out);
+int effsc_hip_table_len(const effsc_hip_prover* h, size_t table, size_t* out);
+/* Copies table `table` (len * ext words, AoS, in cfg.repr) into dst; fails
+ * with EINVAL if cap_words is too small. */
+int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_t cap_words, size_t* len);
+
+/* Restarts the prover from its (unmodified) input tables.  No reference
+ * counterpart: the reference consumes its tables. */
+int effsc_hip_reset(effsc_hip_prover* h);
+
+/* Drop. */
+void effsc_hip_free(effsc_hip_prover* h);
+
+/* Kernel timing, no reference counterpart.  effsc_hip_profile(h, 1) clears
+ * the counters and brackets every later launch with HIP events on the
+ * prover's stream; effsc_hip_profile_read returns, per kernel class
+ * (0 fused fold+evaluate, 1 evaluate, 2 fold), the launch count, the summed
+ * event time in ms and the summed algorithmic bytes (each input element read
+ * once, each output element written once). */
+int effsc_hip_profile(effsc_hip_prover* h, int enable);
+int effsc_hip_profile_read(const effsc_hip_prover* h, uint64_t* launches, double* ms, double* bytes);
+
+/* hipDeviceSynchronize on `device`. */
+int effsc_hip_synchronize(int device);
+
+/* Thread-local description of the last failure ("" if none). */
+const char* effsc_hip_last_error(void);
+
+/* ------------------------------------------------------------ free fold */
+
+/* fold(&mut Vec<F>, weight) — src/fold.rs:14, src/multilinear_sumcheck.rs:132-178:
+ * in-place half-split fold of n elements (device pointer, canonical, base
+ * field) with implicit zero padding; *new_len = next_pow2(n) / 2 (or n if
+ * n <= 1). */
+int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size_t* new_len);
+
+/* --------------------------------------------------------- transcripts */
+
+typedef struct effsc_hip_transcript effsc_hip_transcript;
+
+/* Fiat-Shamir transcript (SHAKE256 duplex over a fixed byte encoding; the
+ * reference binds spongefish here, src/transcript/spongefish.rs:17-44). */
+effsc_hip_transcript* effsc_hip_transcript_new(const uint8_t* domain_separator, size_t len);
+/* TestTranscript (src/transcript/sanity.rs:30-61): sends are no-ops and each
+ * challenge is from_u64(splitmix64 next) lifted into the extension. */
+effsc_hip_transcript* effsc_hip_transcript_new_test(uint64_t seed);
+/* ProverTranscript::send / VerifierTranscript::receive absorption
+ * (src/transcript/transcript.rs:11-43); `value` is ext canonical words. */
+int effsc_hip_transcript_send(effsc_hip_transcript* t, int ext, const uint64_t* value);
+int effsc_hip_transcript_challenge(effsc_hip_transcript* t, int ext, uint64_t* out);
+void effsc_hip_transcript_free(effsc_hip_transcript* t);
+
+/* ------------------------------------------------- runner and verifier */
+
+/* Per-round hook: called after the round's values are sent and before the
+ * challenge is read; a non-zero return from a verifier hook is
+ * SumcheckError::HookError (src/verifier.rs:86-87).  The prover runner
+ * ignores the return value, as the reference's FnMut hook returns nothing. */
+typedef int (*effsc_hip_hook)(void* ctx, size_t round, effsc_hip_transcript* t);
+
+/* runner::sumcheck — src/runner.rs:29-69.  round_polys: num_rounds *
+ * max(degree,1) * ext words; challenges: num_rounds * ext; final_value: ext.
+ * Words are canonical.  hook may be NULL (noop_hook, src/lib.rs:28-30). */
+int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, effsc_hip_hook hook,
+                       void* hook_ctx, uint64_t* round_polys, uint64_t* challenges, uint64_t* final_value);
+
+/* verifier::sumcheck_verify — src/verifier.rs:61-139.  The prover messages
+ * are read from `proof` (proof_words canonical words).  On success returns
+ * OK with challenges (num_rounds * ext) and final_claim (ext); the oracle
+ * check final_claim == g(r) stays with the caller, as in the reference.
+ * Returns EVERIFY with *err_round set for SumcheckError::TranscriptError or
+ * ::HookError (*err_kind = 1 transcript, 2 hook). */
+int effsc_hip_sumcheck_verify(int ext, uint64_t w, const uint64_t* claimed_sum, size_t degree, size_t num_rounds,
+                              effsc_hip_transcript* t, const uint64_t* proof, size_t proof_words,
+                              effsc_hip_hook hook, void* hook_ctx, uint64_t* challenges, uint64_t* final_claim,
+                              size_t* err_round, int* err_kind);
+
+/* ------------------------------------------------------------ utilities */
+
+/* Device buffers for callers that build tables on the device. */
+int effsc_hip_device_alloc(int device, size_t words, uint64_t** out);
+int effsc_hip_device_free(int device, uint64_t* ptr);
+int effsc_hip_copy(int device, uint64_t* dst, const uint64_t* src, size_t words);
+/* Counter-based uniform values in [0, p): dst[i] = U(seed, first + i). */
+int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint64_t seed, uint64_t first);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EFFSC_HIP_H */

@@ -1,0 +1,222 @@
+"""Pure-Python brute-force sumcheck, written from the mathematics.
+
+TEST INFRASTRUCTURE ONLY.  It shares no code with the C oracle and folds
+nothing: every round polynomial is evaluated from the definition
+
+    g_j(t) = sum_{b in {0,1}^(v-j-1)} F(r_0, ..., r_{j-1}, t, b)
+
+where F is the summand written with multilinear extensions in the reference's
+MSB (half-split) variable order (src/multilinear_sumcheck.rs:1-8: round j binds
+the top remaining index bit), and the Fiat-Shamir transcript is recomputed
+from its whole byte stream with hashlib's SHAKE256.  Intended for v <= 6.
+
+Wire format (EvalsInfty, src/sumcheck_prover.rs:40-59): degree 1 sends [g(0)];
+degree d >= 2 sends [g(0), lead(g), g(2), ..., g(d-1)].
+"""
+from __future__ import annotations
+
+import hashlib
+from math import factorial
+
+P = 0xFFFFFFFF00000001
+W_FROM_RAW7 = 0xFFFFFFF800000001
+MASK = (1 << 64) - 1
+
+
+# ----------------------------------------------------------------- field
+def inv(a):
+    return pow(a % P, P - 2, P)
+
+
+class Ext:
+    """F_p[X]/(X^d - w); elements are tuples of d ints."""
+
+    def __init__(self, d=1, w=W_FROM_RAW7):
+        self.d = d
+        self.w = w % P
+
+    def zero(self):
+        return (0,) * self.d
+
+    def one(self):
+        return (1,) + (0,) * (self.d - 1)
+
+    def lift(self, x):
+        return (x % P,) + (0,) * (self.d - 1)
+
+    def add(self, a, b):
+        return tuple((x + y) % P for x, y in zip(a, b))
+
+    def sub(self, a, b):
+        return tuple((x - y) % P for x, y in zip(a, b))
+
+    def mul(self, a, b):
+        d = self.d
+        prod = [0] * (2 * d - 1)
+        for i in range(d):
+            for j in range(d):
+                prod[i + j] += a[i] * b[j]
+        for k in range(2 * d - 2, d - 1, -1):  # X^k = w X^(k-d)
+            prod[k - d] += self.w * prod[k]
+        return tuple(x % P for x in prod[:d])
+
+    def scale(self, a, s):
+        return tuple(x * s % P for x in a)
+
+
+# ----------------------------------------------------------------- inputs
+def _mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+def uniform(seed, index):
+    """Counter-based uniform element of [0, p): first candidate below p."""
+    base = (_mix64(seed) + index * 0x9E3779B97F4A7C15) & MASK
+    a = 0
+    while True:
+        c = _mix64((base + a * 0xD1B54A32D192ED03) & MASK)
+        if c < P:
+            return c
+        a += 1
+
+
+def uniform_table(n, seed):
+    return [uniform(seed, i) for i in range(n)]
+
+
+# ----------------------------------------------------------------- transcripts
+FS_TAG = b"effsc-hip/fs/v1"
+
+
+class ShakeTranscript:
+    """Byte-stream definition of the Fiat-Shamir transcript."""
+
+    def __init__(self, domsep: bytes):
+        self.stream = bytearray(FS_TAG + len(domsep).to_bytes(8, "little") + domsep)
+
+    def send(self, value):
+        self.stream += b"\x01" + b"".join((x % P).to_bytes(8, "little") for x in value)
+
+    def challenge(self, d):
+        out = hashlib.shake_256(bytes(self.stream) + b"\x02").digest(16 * d)
+        self.stream += b"\x02"
+        return tuple(int.from_bytes(out[16 * i:16 * i + 16], "little") % P for i in range(d))
+
+
+class RngTranscript:
+    """Test transcript: challenge = from_u64(splitmix64 next), lifted."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK
+
+    def send(self, value):
+        pass
+
+    def challenge(self, d):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK
+        return (_mix64(self.state) % P,) + (0,) * (d - 1)
+
+
+# ----------------------------------------------------------------- MLE
+def mle(F: Ext, table, point):
+    """sum_x table[x] * prod_j (z_j if bit_j(x) else 1 - z_j), MSB-first bits."""
+    v = len(point)
+    assert len(table) == 1 << v
+    acc = F.zero()
+    for x, val in enumerate(table):
+        term = val
+        for j, z in enumerate(point):
+            bit = (x >> (v - 1 - j)) & 1
+            term = F.mul(term, z if bit else F.sub(F.one(), z))
+        acc = F.add(acc, term)
+    return acc
+
+
+def eq_point(F: Ext, w, z):
+    acc = F.one()
+    for wj, zj in zip(w, z):
+        t = F.add(F.mul(wj, zj), F.mul(F.sub(F.one(), wj), F.sub(F.one(), zj)))
+        acc = F.mul(acc, t)
+    return acc
+
+
+def _bits(b, m, F):
+    return [F.lift((b >> (m - 1 - i)) & 1) for i in range(m)]
+
+
+def _lead_coeff(F, evals, d):
+    acc = F.zero()
+    for i, g in enumerate(evals):
+        c = inv(factorial(i) * factorial(d - i))
+        if (d - i) % 2:
+            c = (P - c) % P
+        acc = F.add(acc, F.scale(g, c))
+    return acc
+
+
+def prove(F: Ext, summand, v, degree, transcript, num_rounds=None):
+    """Generic brute-force prover for a summand F(z) given as a callable on a
+    full point in F^v.  Returns (round_polys, challenges, final_value) with
+    every element a tuple of d ints."""
+    num_rounds = v if num_rounds is None else num_rounds
+    chals, polys = [], []
+    for j in range(num_rounds):
+        m = v - j - 1
+        evals = []
+        for t in range(degree + 1):
+            acc = F.zero()
+            for b in range(1 << m):
+                acc = F.add(acc, summand(chals + [F.lift(t)] + _bits(b, m, F)))
+            evals.append(acc)
+        if degree <= 1:
+            wire = [evals[0]]
+        else:
+            wire = [evals[0], _lead_coeff(F, evals, degree)] + [evals[i] for i in range(2, degree)]
+        for x in wire:
+            transcript.send(x)
+        polys.append(wire)
+        chals.append(transcript.challenge(F.d))
+    final = summand(chals) if num_rounds == v else None
+    return polys, chals, final
+
+
+def _pad(table, v, F, lift):
+    t = [F.lift(x) if lift else tuple(x) for x in table]
+    return t + [F.zero()] * ((1 << v) - len(t))
+
+
+def prove_multilinear(F, evals, v, transcript, lift=True):
+    T = _pad(evals, v, F, lift)
+    return prove(F, lambda z: mle(F, T, z), v, 1, transcript)
+
+
+def prove_product(F, tables, v, transcript, lift=True):
+    Ts = [_pad(t, v, F, lift) for t in tables]
+
+    def summand(z):
+        acc = F.one()
+        for T in Ts:
+            acc = F.mul(acc, mle(F, T, z))
+        return acc
+
+    return prove(F, summand, v, len(tables), transcript)
+
+
+def prove_eq(F, w, p_evals, transcript, lift=True):
+    v = len(w)
+    T = _pad(p_evals, v, F, lift)
+    W = [F.lift(x) if lift else tuple(x) for x in w]
+    return prove(F, lambda z: F.mul(eq_point(F, W, z), mle(F, T, z)), v, 2, transcript)
+
+
+def claimed_sum_product(F, tables, lift=True):
+    Ts = [[F.lift(x) if lift else tuple(x) for x in t] for t in tables]
+    acc = F.zero()
+    for i in range(len(Ts[0])):
+        term = F.one()
+        for T in Ts:
+            term = F.mul(term, T[i])
+        acc = F.add(acc, term)
+    return acc

@@ -1,0 +1,271 @@
+"""Parity of the HIP provers with the oracle and the golden vectors.
+
+Every test runs through the C ABI (libeffsc_hip.so) on the GPU and compares
+round polynomials, challenges, final values and post-state bit for bit.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import effsc
+from effsc import (CoefficientProver, EqFactoredProver, InnerProductProver, MultilinearProver, TestTranscript,
+                   Transcript, sumcheck, sumcheck_verify)
+from effsc import _native as N
+from effsc.field import MONT64, to_mont
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def el(words, ext):
+    return int(words[0]) if ext == 1 else tuple(int(x) for x in words)
+
+
+def as_oracle(proof, ext):
+    rp = [[[x] if ext == 1 else list(x) for x in r] for r in proof.round_polys]
+    ch = [[c] if ext == 1 else list(c) for c in proof.challenges]
+    fin = [proof.final_value] if ext == 1 else list(proof.final_value)
+    return rp, ch, fin
+
+
+def make_pair(kind, ext, input_ext, tabs, wpt=None):
+    """(gpu prover, oracle prover) over the same canonical input words."""
+    if kind == "ml":
+        g = MultilinearProver(tabs[0], ext=ext, input_ext=input_ext)
+        o = O.Prover.multilinear(tabs[0], ext=ext) if input_ext == ext else O.Prover.product([tabs[0]], ext=ext)
+    elif kind == "ip":
+        g = InnerProductProver(tabs[0], tabs[1], ext=ext, input_ext=input_ext)
+        if input_ext == ext:
+            o = O.Prover.inner_product(tabs[0], tabs[1], ext=ext)
+        else:
+            o = O.Prover.product(tabs[:2], ext=ext)
+    elif kind.startswith("p"):
+        k = int(kind[1:])
+        g = CoefficientProver(tabs[:k], ext=ext, input_ext=input_ext)
+        o = O.Prover.product(tabs[:k], ext=ext, tables_in_ext=(input_ext == ext and ext > 1))
+    else:
+        g = EqFactoredProver(wpt, tabs[0], ext=ext)
+        o = O.Prover.eq_factored(wpt, tabs[0], ext=ext)
+    return g, o
+
+
+def check_proof(kind, ext, input_ext, n, v, seed, transcript="shake", rounds=None):
+    rounds = v if rounds is None else rounds
+    k = {"ml": 1, "ip": 2, "eq": 1}.get(kind) or int(kind[1:])
+    tabs = [O.uniform(n * input_ext, seed + i) for i in range(k)]
+    wpt = O.uniform(v * ext, seed + 100) if kind == "eq" else None
+    say(f"case kind={kind} ext={ext} input_ext={input_ext} n={n} v={v} rounds={rounds} tr={transcript}")
+    g, o = make_pair(kind, ext, input_ext, tabs, wpt)
+    if transcript == "shake":
+        tg, to = Transcript(b"effsc-hip/test", ext=ext), O.Transcript.shake(b"effsc-hip/test")
+    else:
+        tg, to = TestTranscript(seed, ext=ext), O.Transcript.rng(seed)
+    pg = sumcheck(g, rounds, tg)
+    po = o.sumcheck(rounds, to)
+    assert as_oracle(pg, ext) == list(po)
+    # post-state
+    for i in range(len(tabs) if kind != "eq" else 3):
+        assert np.array_equal(g.table_words(i), o.table(i)), f"table {i}"
+    return g, pg
+
+
+# ------------------------------------------------------------------ golden
+def test_golden_proofs(golden):
+    for case in golden["proofs"]:
+        ext, n, v, kind = case["ext"], case["n"], case["v"], case["kind"]
+        seeds = case["seeds"]
+        say("golden", kind, ext, n, v)
+        if kind == "ml":
+            tabs = [O.uniform(n, seeds[0])]
+            g = MultilinearProver(tabs[0], ext=ext, input_ext=1)
+        elif kind == "ip":
+            tabs = [O.uniform(n, s) for s in seeds[:2]]
+            g = InnerProductProver(tabs[0], tabs[1], ext=ext, input_ext=1)
+        elif kind.startswith("product"):
+            k = int(kind[-1])
+            g = CoefficientProver([O.uniform(n, s) for s in seeds[:k]], ext=ext, input_ext=1)
+        else:
+            g = EqFactoredProver(O.uniform(v * ext, seeds[1]), O.uniform(n, seeds[0]), ext=ext)
+        t = (Transcript(case["domsep"].encode(), ext=ext) if case["transcript"] == "shake"
+             else TestTranscript(7, ext=ext))
+        pr = sumcheck(g, v, t)
+        rp, ch, fin = as_oracle(pr, ext)
+        assert rp == case["round_polys"]
+        assert ch == case["challenges"]
+        assert fin == case["final"]
+
+
+# ------------------------------------------------------------- oracle parity
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 4, 5, 8, 11, 14])
+def test_pow2_parity(kind, ext, input_ext, v):
+    check_proof(kind, ext, input_ext, 1 << v, v, 1000 + v)
+
+
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("n", [1, 3, 5, 13, 33, 100, 1000, 4097])
+def test_non_pow2_parity(kind, n):
+    v = (n - 1).bit_length()
+    check_proof(kind, 1, 1, n, v, 77 + n)
+    check_proof(kind, 3, 3, n, v, 78 + n, transcript="rng")
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3)])
+@pytest.mark.parametrize("v", [1, 2, 3, 6, 12])
+def test_product_parity(k, ext, input_ext, v):
+    check_proof(f"p{k}", ext, input_ext, 1 << v, v, 500 + 10 * k + v)
+
+
+@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 5, 8, 13])
+def test_eq_factored_parity(ext, v):
+    check_proof("eq", ext, ext, 1 << v, v, 900 + v)
+
+
+def test_eq_factored_short_input_is_zero_padded():
+    check_proof("eq", 1, 1, 37, 7, 4242)
+
+
+@pytest.mark.parametrize("kind", ["ml", "ip", "p3"])
+def test_large_parity(kind):
+    v = 20
+    check_proof(kind, 1, 1, 1 << v, v, 31337)
+
+
+def test_large_ext3_product_parity():
+    check_proof("p3", 3, 1, 1 << 16, 16, 4711)
+
+
+# --------------------------------------------------------- runner semantics
+def test_partial_then_continue_and_per_round_abi():
+    # tests/canonical_api.rs:146-178 and 258-290
+    a, b = O.uniform(256, 1), O.uniform(256, 2)
+    full = sumcheck(InnerProductProver(a, b), 8, Transcript(b"p"))
+    g = InnerProductProver(a, b)
+    t = Transcript(b"p")
+    first = sumcheck(g, 3, t)
+    second = sumcheck(g, 5, t)
+    assert first.round_polys + second.round_polys == full.round_polys
+    assert first.challenges + second.challenges == full.challenges
+    assert second.final_value == full.final_value
+    # per-round C-ABI calls driven from Python (Python runner) == native runner
+    class PyT:
+        def __init__(self):
+            self.t = Transcript(b"p")
+
+        def send(self, v):
+            self.t.send(v)
+
+        def challenge(self):
+            return self.t.challenge()
+
+    py = sumcheck(InnerProductProver(a, b), 8, PyT())
+    assert py.round_polys == full.round_polys and py.final_value == full.final_value
+
+
+def test_hook_order_and_zero_rounds():
+    # tests/canonical_api.rs:180-197; tests/inner_product_sumcheck.rs:142-157
+    calls = []
+    g = MultilinearProver(O.uniform(32, 5))
+    sumcheck(g, 5, Transcript(b"h"), lambda r, t: calls.append(r))
+    assert calls == list(range(5))
+    g = InnerProductProver(O.uniform(16, 1), O.uniform(16, 2))
+    pr = sumcheck(g, 0, Transcript(b"z"))
+    assert pr.round_polys == [] and pr.challenges == []
+    assert len(g.evaluations()[0]) == 16
+
+
+def test_reset_reproduces_proof():
+    a, b = O.uniform(1 << 12, 3), O.uniform(1 << 12, 4)
+    g = InnerProductProver(a, b)
+    p1 = sumcheck(g, 12, Transcript(b"r"))
+    g.reset()
+    p2 = sumcheck(g, 12, Transcript(b"r"))
+    assert p1.round_polys == p2.round_polys and p1.final_value == p2.final_value
+
+
+def test_montgomery_boundary():
+    # arkworks words in, arkworks words out (src/field.rs:99-145)
+    a, b = O.uniform(1 << 10, 5), O.uniform(1 << 10, 6)
+    am = np.array([to_mont(int(x)) for x in a], dtype=np.uint64)
+    bm = np.array([to_mont(int(x)) for x in b], dtype=np.uint64)
+    gm = InnerProductProver(am, bm, repr=MONT64)
+    gc = InnerProductProver(a, b)
+    r0m, r0c = gm.round(None), gc.round(None)
+    assert r0m == [to_mont(x) for x in r0c]
+    cm = to_mont(123456789)
+    assert gm.round(cm) == [to_mont(x) for x in gc.round(123456789)]
+    assert [int(x) for x in gm.table_words(0)] == [to_mont(int(x)) for x in gc.table_words(0)]
+
+
+def test_borrowed_device_input_and_fill_uniform():
+    n, v = 1 << 14, 14
+    ptrs = []
+    for seed in (8, 9):
+        p = N.u64p()
+        N.check(N.lib().effsc_hip_device_alloc(0, n, C.byref(p)))
+        N.check(N.lib().effsc_hip_fill_uniform(0, p, n, seed, 0))
+        ptrs.append(p)
+    try:
+        host = np.zeros(n, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_copy(0, host.ctypes.data_as(N.u64p), ptrs[0], n))
+        assert np.array_equal(host, O.uniform(n, 8))
+        addr = [C.cast(p, C.c_void_p).value for p in ptrs]
+        g = InnerProductProver(addr[0], addr[1], borrow=True, n=n)
+        pg = sumcheck(g, v, Transcript(b"b"))
+        po = O.Prover.inner_product(O.uniform(n, 8), O.uniform(n, 9)).sumcheck(v, O.Transcript.shake(b"b"))
+        assert as_oracle(pg, 1) == list(po)
+        g.close()
+        # input untouched by the proof
+        N.check(N.lib().effsc_hip_copy(0, host.ctypes.data_as(N.u64p), ptrs[0], n))
+        assert np.array_equal(host, O.uniform(n, 8))
+    finally:
+        for p in ptrs:
+            N.lib().effsc_hip_device_free(0, p)
+
+
+def test_free_fold_matches_reference_semantics():
+    for n in (2, 3, 7, 8, 1000, 1 << 15):
+        vals = [int(x) for x in O.uniform(n, n)]
+        w = 987654321987
+        got = effsc.fold(vals, w)
+        pr = O.Prover.multilinear(vals)
+        pr.finalize([w])
+        assert got == [int(x) for x in pr.table(0)]
+
+
+def test_gpu_proofs_accepted_by_verifier():
+    for kind, k in (("ml", 1), ("ip", 2), ("p3", 3)):
+        v = 10
+        tabs = [O.uniform(1 << v, 60 + i) for i in range(k)]
+        claim = 0
+        for i in range(1 << v):
+            x = 1
+            for t in tabs:
+                x = x * int(t[i]) % P
+            claim = (claim + x) % P
+        g = (MultilinearProver(tabs[0]) if k == 1 else InnerProductProver(*tabs) if k == 2
+             else CoefficientProver(tabs))
+        proof = sumcheck(g, v, Transcript(b"verify"))
+        res = sumcheck_verify(claim, k, v, Transcript(b"verify").verifier(proof))
+        assert res.final_claim == proof.final_value
+        assert res.challenges == proof.challenges
+
+
+def test_errors_are_reported_not_raised_natively():
+    with pytest.raises(ValueError):
+        InnerProductProver([1, 2, 3], [1, 2])
+    with pytest.raises(N.NativeError):
+        CoefficientProver([[1, 2, 3]])  # non-power-of-two product table
+    with pytest.raises(N.NativeError):
+        EqFactoredProver([1, 2], [1, 2, 3, 4, 5])  # more than 2^v evaluations

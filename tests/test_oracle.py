@@ -1,0 +1,253 @@
+"""Pins the C oracle: golden vectors from Python big integers and hashlib,
+the reference's constants and known-answer tests, and the brute force."""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+import brute as B
+import oracle as O
+
+P = O.P
+
+
+def test_field_constants_match_reference():
+    # avx512.rs:19-35 and tests/fields.rs:61-64
+    assert O.lib().or_to_mont(1) == 0xFFFFFFFF
+    assert O.lib().or_to_mont(P - 1) == 0xFFFFFFFE00000002
+    assert O.lib().or_from_mont(0xFFFFFFFF) == 1
+    # from_raw(7) has canonical value 7 * 2^-64 = -7 * 2^32 mod p
+    assert O.lib().or_from_mont(7) == O.W_FROM_RAW7 == (P - 7 * 2**32) % P
+    # INV = -p^-1 mod 2^64
+    assert (P * 0xFFFFFFFEFFFFFFFF) % 2**64 == 2**64 - 1
+
+
+def test_mont_mul_edge_cases():
+    # avx512.rs:689-701: 0*(-1) = 0, 1*(-1) = -1, (-1)(-1) = 1 in Montgomery form
+    one, neg = O.lib().or_to_mont(1), O.lib().or_to_mont(P - 1)
+    assert O.lib().or_mont_mul(0, neg) == 0
+    assert O.lib().or_mont_mul(one, neg) == neg
+    assert O.lib().or_mont_mul(neg, neg) == one
+
+
+def test_field_golden(golden):
+    L = O.lib()
+    for a, b, c in golden["field"]["mul"]:
+        assert L.or_gl_mul(a, b) == c
+        assert L.or_gl_mul_i128(a, b) == c
+    for a, b, c in golden["field"]["add"]:
+        assert L.or_gl_add(a, b) == c
+    for a, b, c in golden["field"]["sub"]:
+        assert L.or_gl_sub(a, b) == c
+    for a, c in golden["field"]["inv"]:
+        assert L.or_gl_inv(a) == c
+
+
+def test_field_random_vs_bigint():
+    rng = random.Random(1)
+    L = O.lib()
+    for _ in range(20000):
+        a, b = rng.randrange(P), rng.randrange(P)
+        assert L.or_gl_mul(a, b) == a * b % P
+        m = L.or_mont_mul(a, b)
+        assert m == a * b * pow(2**64, P - 2, P) % P
+
+
+def test_ext_golden(golden):
+    for key, cases in golden["field"]["ext"].items():
+        d = int(key[1])
+        w = int(key.split("_w")[1], 16)
+        for a, b, c in cases:
+            assert O.ext_mul(d, w, a, b) == c
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("w", [7, O.W_FROM_RAW7])
+def test_ext_inverse_and_nonresidue(d, w):
+    rng = random.Random(d * 31 + w % 97)
+    one = [1] + [0] * (d - 1)
+    for _ in range(50):
+        a = [rng.randrange(P) for _ in range(d)]
+        assert O.ext_mul(d, w, a, O.ext_inv(d, w, a)) == one
+    # X^d = w generates a field: w is not a d-th power (Euler criterion)
+    assert pow(w, (P - 1) // d, P) != 1
+
+
+def test_uniform_generator_matches_python():
+    assert [int(x) for x in O.uniform(64, 5)] == B.uniform_table(64, 5)
+    assert all(int(x) < P for x in O.uniform(1000, 9))
+
+
+def test_shake256_golden(golden):
+    for msg, out in golden["shake256"]:
+        assert O.shake256(bytes.fromhex(msg), 200).hex() == out
+    assert O.shake256(b"x" * 1000, 400) == hashlib.shake_256(b"x" * 1000).digest(400)
+
+
+def test_keccak_permutation_of_zero_state():
+    # FIPS 202 / KeccakCodePackage: Keccak-f[1600] applied to the all-zero state
+    st = (O.C.c_uint64 * 25)() if hasattr(O, "C") else None
+    import ctypes
+    st = (ctypes.c_uint64 * 25)()
+    O.lib().or_keccak_f1600(st)
+    assert st[0] == 0xF1258F7940E1DDE7 and st[1] == 0x84D5CCF933C0478A
+
+
+def test_transcript_golden(golden):
+    for case in golden["transcript"]:
+        if "rng_seed" in case:
+            t = O.Transcript.rng(case["rng_seed"])
+            assert [t.challenge(1) for _ in case["challenges"]] == case["challenges"]
+            continue
+        t = O.Transcript.shake(bytes.fromhex(case["domsep"]))
+        for op, val in case["ops"]:
+            if op == "send":
+                t.send(val)
+            else:
+                assert t.challenge(case["ext"]) == val
+
+
+def _oracle_proof(case):
+    ext, n, v = case["ext"], case["n"], case["v"]
+    seeds = case["seeds"]
+    kind = case["kind"]
+    tabs = [O.uniform(n, s) for s in seeds]
+    if kind == "ml":
+        if ext == 1:
+            pr = O.Prover.multilinear(tabs[0])
+        else:
+            pr = O.Prover.product([tabs[0]], ext=ext)
+    elif kind == "ip":
+        pr = O.Prover.inner_product(tabs[0], tabs[1], ext=ext)
+    elif kind.startswith("product"):
+        k = int(kind[-1])
+        pr = O.Prover.product(tabs[:k], ext=ext)
+    else:
+        wpt = O.uniform(v * ext, seeds[1])
+        pr = O.Prover.eq_factored(wpt, tabs[0], ext=ext)
+    t = (O.Transcript.shake(case["domsep"].encode()) if case["transcript"] == "shake" else O.Transcript.rng(7))
+    return pr.sumcheck(v, t)
+
+
+def test_oracle_proofs_match_brute_force_golden(golden):
+    for case in golden["proofs"]:
+        rp, ch, fin = _oracle_proof(case)
+        assert rp == case["round_polys"], case["kind"]
+        assert ch == case["challenges"]
+        assert fin == case["final"]
+
+
+@pytest.mark.parametrize("v", [1, 3, 6])
+@pytest.mark.parametrize("ext", [1, 3])
+def test_eq_factored_equals_inner_product_with_eq_table(v, ext):
+    # src/provers/eq_factored.rs:411-446: EqFactored == IP(p, build_eq_table(w))
+    n = 1 << v
+    p = O.uniform(n * ext, 3)
+    wpt = O.uniform(v * ext, 4)
+    F = B.Ext(ext)
+    W = [tuple(int(x) for x in wpt[i * ext:(i + 1) * ext]) for i in range(v)]
+    eq = []
+    for x in range(n):
+        eq.extend(B.eq_point(F, W, B._bits(x, v, F)))
+    a = O.Prover.eq_factored(wpt, p, ext=ext).sumcheck(v, O.Transcript.shake(b"eq"))
+    b = O.Prover.inner_product(p, np.array(eq, dtype=np.uint64), ext=ext).sumcheck(v, O.Transcript.shake(b"eq"))
+    assert a == b
+
+
+def test_eq_table_known_answer():
+    # src/hypercube/eq_evals.rs:70-79: LSB eq([2,3]) = [2, -4, -3, 6]; the MSB
+    # table of build_eq_table swaps the middle entries.
+    pr = O.Prover.eq_factored([2, 3], [0, 0, 0, 0])
+    # v = 2 -> v_l = 1: eq_l = eq([2]) = [-1, 2], eq_r = eq([3]) = [-2, 3]
+    assert [int(x) for x in pr.table(1)] == [P - 1, 2]
+    assert [int(x) for x in pr.table(2)] == [P - 2, 3]
+    full = [(l * r) % P for l in (P - 1, 2) for r in (P - 2, 3)]
+    assert full == [2, P - 3, P - 4, 6]
+
+
+def test_coefficient_degree1_equals_multilinear():
+    # src/provers/coefficient.rs:378-414
+    t = O.uniform(64, 8)
+    a = O.Prover.product([t]).sumcheck(6, O.Transcript.shake(b"c"))
+    b = O.Prover.multilinear(t).sumcheck(6, O.Transcript.shake(b"c"))
+    assert a == b
+
+
+def test_round0_closed_forms():
+    # tests/multilinear_sumcheck.rs:140-157 and tests/inner_product_sumcheck.rs:159-186
+    n = 32
+    a, b = O.uniform(n, 1), O.uniform(n, 2)
+    A, Bv = [int(x) for x in a], [int(x) for x in b]
+    (s0,), = O.Prover.multilinear(a).round(None)
+    assert s0 == sum(A[:n // 2]) % P
+    q = O.Prover.inner_product(a, b).round(None)
+    assert q[0] == [sum(x * y for x, y in zip(A[:16], Bv[:16])) % P]
+    assert q[1] == [sum((A[16 + i] - A[i]) * (Bv[16 + i] - Bv[i]) for i in range(16)) % P]
+
+
+def test_verifier_accepts_honest_and_catches_corruption():
+    # tests/adversarial_verifier.rs:110-230, 492-530
+    for kind, k in (("ml", 1), ("ip", 2), ("p3", 3)):
+        v, n = 6, 64
+        tabs = [O.uniform(n, 40 + i) for i in range(k)]
+        claimed = [0]
+        for i in range(n):
+            x = 1
+            for t in tabs:
+                x = x * int(t[i]) % P
+            claimed[0] = (claimed[0] + x) % P
+        pr = O.Prover.multilinear(tabs[0]) if k == 1 else (
+            O.Prover.inner_product(*tabs) if k == 2 else O.Prover.product(tabs))
+        rp, ch, fin = pr.sumcheck(v, O.Transcript.shake(b"adv"))
+        words = [w for r in rp for e in r for w in e]
+        st, vch, vfin = O.verify(claimed, k, v, O.Transcript.shake(b"adv"), words)
+        assert st == 0 and vch == ch and vfin == fin
+        bad = list(words)
+        bad[len(bad) // 2] = (bad[len(bad) // 2] + 1) % P
+        st, _, vfin = O.verify(claimed, k, v, O.Transcript.shake(b"adv"), bad)
+        assert st == 0 and vfin != fin  # caller's oracle check catches it
+        st, _, vfin = O.verify([(claimed[0] + 1) % P], k, v, O.Transcript.shake(b"adv"), words)
+        assert vfin != fin
+        st, _, _ = O.verify(claimed, k, v, O.Transcript.shake(b"adv"), words[:-1])
+        assert st == v  # TranscriptError in the last round
+
+
+def test_final_value_is_mle_at_challenges():
+    # tests/canonical_api.rs:22-34,139
+    v = 5
+    a, b = O.uniform(32, 5), O.uniform(32, 6)
+    F = B.Ext(1)
+    pr = O.Prover.inner_product(a, b)
+    rp, ch, fin = pr.sumcheck(v, O.Transcript.rng(3))
+    pt = [tuple(c) for c in ch]
+    fa = B.mle(F, [(int(x),) for x in a], pt)
+    fb = B.mle(F, [(int(x),) for x in b], pt)
+    assert fin == [fa[0] * fb[0] % P]
+    assert [int(pr.table(0)[0])] == list(fa)
+
+
+def test_partial_then_continue_equals_full():
+    # tests/canonical_api.rs:146-178
+    a, b = O.uniform(256, 1), O.uniform(256, 2)
+    full = O.Prover.inner_product(a, b).sumcheck(8, O.Transcript.shake(b"p"))
+    pr = O.Prover.inner_product(a, b)
+    t = O.Transcript.shake(b"p")
+    first = pr.sumcheck(3, t)
+    second = pr.sumcheck(5, t)
+    assert first[0] + second[0] == full[0]
+    assert first[1] + second[1] == full[1]
+    assert second[2] == full[2]
+
+
+def test_plonky3_style_counter_closed_form():
+    # tests/plonky3_roundtrip.rs:158-181, MSB analogue: evals x+1, challenges
+    # 2,3,4,5 give f(r) = 1 + 8*2 + 4*3 + 2*4 + 5 = 42
+    evals = list(range(1, 17))
+    pr = O.Prover.multilinear(evals)
+    out = [pr.round(None)]
+    for c in (2, 3, 4):
+        out.append(pr.round([c]))
+    pr.finalize([5])
+    assert out[0] == [[36]]
+    assert pr.final_value() == [42]
