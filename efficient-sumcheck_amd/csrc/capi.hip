@@ -11,6 +11,7 @@
 
 #include "effsc_hip.h"
 #include "kernels.hpp"
+#include "prove.hpp"
 #include "shake.hpp"
 
 using namespace effsc_hip;
@@ -85,19 +86,35 @@ struct effsc_hip_prover {
   uint64_t* eq_l = nullptr;  // ext planes, stride eq_l_stride
   uint64_t* eq_r = nullptr;
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
-  uint64_t* eq_inner = nullptr;  // per-row partial sums for the left phase
+  uint64_t* eq_inner = nullptr;  // unused
+  // Device-resident proof loop (effsc_hip_sumcheck without a hook)
+  bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
+  unsigned last_grid = 0;
+  DevFs* d_fs = nullptr;
+  uint64_t* d_polys = nullptr;  // kMaxRounds * kMaxK * 3 words
+  uint64_t* d_chals = nullptr;  // kMaxRounds * 3 words
+  uint64_t* d_cur = nullptr;    // 3 words: current challenge
+  uint64_t* d_fin = nullptr;    // 3 words
+  uint64_t* h_proof = nullptr;  // pinned staging: polys | chals | fin | fs
   // Kernel timing (effsc_hip_profile): HIP events around each launch on the
   // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold.
   bool prof_on = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int prof_pending = -1;
-  double prof_pending_bytes = 0;
+  std::vector<hipEvent_t> ev;             // pairs (2i, 2i+1)
+  std::vector<std::pair<int, double>> pend;  // (slot, bytes) of recorded pairs
   uint64_t prof_launches[3] = {0, 0, 0};
   double prof_ms[3] = {0, 0, 0};
   double prof_bytes[3] = {0, 0, 0};
 };
 
+struct effsc_hip_transcript {
+  int kind;  // 0: Fiat-Shamir, 1: test RNG
+  FsTranscript fs;
+  uint64_t rng;
+};
+
 namespace {
+
+constexpr size_t kMaxRounds = 64;
 
 int alloc_dev(uint64_t** p, size_t words) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(words, 1) * 8);
@@ -111,26 +128,35 @@ int alloc_dev(uint64_t** p, size_t words) {
 // Brackets a kernel launch with events when profiling; the elapsed time is
 // collected at the next stream synchronisation (prof_collect).
 int prof_begin(effsc_hip_prover* h) {
-  if (h->prof_on) HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (!h->prof_on) return EFFSC_HIP_OK;
+  size_t i = h->pend.size();
+  while (h->ev.size() < 2 * (i + 1)) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    h->ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(h->ev[2 * i], h->stream));
   return EFFSC_HIP_OK;
 }
 int prof_end(effsc_hip_prover* h, int slot, double bytes) {
   if (!h->prof_on) return EFFSC_HIP_OK;
-  HIPCHK(hipEventRecord(h->ev1, h->stream));
-  h->prof_pending = slot;
-  h->prof_pending_bytes = bytes;
+  size_t i = h->pend.size();
+  HIPCHK(hipEventRecord(h->ev[2 * i + 1], h->stream));
+  h->pend.push_back({slot, bytes});
   return EFFSC_HIP_OK;
 }
 int prof_collect(effsc_hip_prover* h) {
-  if (!h->prof_on || h->prof_pending < 0) return EFFSC_HIP_OK;
-  HIPCHK(hipEventSynchronize(h->ev1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  int s = h->prof_pending;
-  h->prof_launches[s]++;
-  h->prof_ms[s] += ms;
-  h->prof_bytes[s] += h->prof_pending_bytes;
-  h->prof_pending = -1;
+  if (!h->prof_on) return EFFSC_HIP_OK;
+  for (size_t i = 0; i < h->pend.size(); i++) {
+    HIPCHK(hipEventSynchronize(h->ev[2 * i + 1]));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
+    int s = h->pend[i].first;
+    h->prof_launches[s]++;
+    h->prof_ms[s] += ms;
+    h->prof_bytes[s] += h->pend[i].second;
+  }
+  h->pend.clear();
   return EFFSC_HIP_OK;
 }
 
@@ -152,6 +178,15 @@ F from_words(const uint64_t* w) {
 }
 
 // -------------------------------------------------------------- launchers
+template <class F, int K>
+int reduce(effsc_hip_prover* h, unsigned nblocks) {
+  h->last_grid = nblocks;
+  if (h->dev_mode) return EFFSC_HIP_OK;  // the caller launches k_reduce_fs
+  hipLaunchKernelGGL((k_reduce<F, K>), dim3(1), dim3(kBlock), 0, h->stream, h->red, nblocks);
+  HIPCHK(hipGetLastError());
+  return EFFSC_HIP_OK;
+}
+
 template <class FI, class FO, int K>
 struct Ops {
   static int eval(effsc_hip_prover* h, uint64_t* const* planes, size_t stride, size_t half, size_t paired) {
@@ -168,36 +203,68 @@ struct Ops {
     HIPCHK(hipGetLastError());
     // algorithmic bytes: every element of the K tables read once
     TRY(prof_end(h, 1, 8.0 * K * FI::D * (double)(half + paired)));
-    return EFFSC_HIP_OK;
+    return reduce<FI, K>(h, vec ? blocks_for(h, half / 2) : blocks_for(h, half));
   }
-  static int fused(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t q) {
+  // chal: host words of the challenge, or nullptr with cptr the device words
+  static int fused(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t q,
+                   const uint64_t* cptr = nullptr) {
     Views<FI, K> in = views<FI, K>(src, src_stride);
     Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
-    FO c = from_words<FO>(chal);
+    FO c = chal ? from_words<FO>(chal) : gl::zero<FO>();
     bool vec = (q % 2 == 0) && (src_stride % 2 == 0) && (h->work_stride % 2 == 0);
     TRY(prof_begin(h));
     if (vec) {
       unsigned g = blocks_for(h, q / 2);
-      hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, q, h->w, h->red);
+      hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w, h->red);
     } else {
       unsigned g = blocks_for(h, q);
-      hipLaunchKernelGGL((k_fused<FI, FO, K, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, q, h->w, h->red);
+      hipLaunchKernelGGL((k_fused<FI, FO, K, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w, h->red);
     }
     HIPCHK(hipGetLastError());
     // algorithmic bytes: read 4q elements, write 2q elements, per table
     TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D)));
-    return EFFSC_HIP_OK;
+    return reduce<FO, K>(h, vec ? blocks_for(h, q / 2) : blocks_for(h, q));
   }
   static int fold(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t half,
-                  size_t paired) {
+                  size_t paired, const uint64_t* cptr = nullptr) {
     Views<FI, K> in = views<FI, K>(src, src_stride);
     Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
-    FO c = from_words<FO>(chal);
+    FO c = chal ? from_words<FO>(chal) : gl::zero<FO>();
     unsigned g = blocks_for(h, half);
     TRY(prof_begin(h));
-    hipLaunchKernelGGL((k_fold<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, half, paired, h->w);
+    hipLaunchKernelGGL((k_fold<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, half, paired, h->w);
     HIPCHK(hipGetLastError());
     TRY(prof_end(h, 2, 8.0 * K * ((double)(half + paired) * FI::D + (double)half * FO::D)));
+    return EFFSC_HIP_OK;
+  }
+};
+
+// Tail threshold: once the tables have at most this many elements, one
+// workgroup runs every remaining round (k_tail).
+size_t tail_max() {
+  if (const char* e = std::getenv("EFFSC_HIP_TAIL")) {
+    long long x = std::atoll(e);
+    if (x >= 2) return (size_t)x;
+  }
+  return (size_t)1 << 14;
+}
+
+template <class FI, class F, int K>
+struct DevOps {
+  static int reduce_fs(effsc_hip_prover* h, unsigned g, size_t round) {
+    hipLaunchKernelGGL((k_reduce_fs<FI, K>), dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs, h->ext,
+                       h->d_polys + round * K * h->ext, h->d_chals + round * h->ext, h->d_cur);
+    HIPCHK(hipGetLastError());
+    return EFFSC_HIP_OK;
+  }
+  static int tail(effsc_hip_prover* h, uint64_t* const* src, size_t sst, bool from_input, size_t L, int rounds,
+                  bool have, size_t first_round) {
+    Views<FI, K> in = views<FI, K>(src, sst);
+    Views<F, K> work = views<F, K>(h->work, h->work_stride);
+    hipLaunchKernelGGL((k_tail<FI, F, K>), dim3(1), dim3(kTailBlock), 0, h->stream, in, work, from_input ? 1 : 0, L,
+                       rounds, have ? 1 : 0, h->d_fs, h->ext, h->d_polys + first_round * K * h->ext,
+                       h->d_chals + first_round * h->ext, h->d_cur, h->d_fin, h->w);
+    HIPCHK(hipGetLastError());
     return EFFSC_HIP_OK;
   }
 };
@@ -213,12 +280,12 @@ int with_k(int K, Fn&& fn) {
   return fail(EFFSC_HIP_EINVAL, "unsupported table count");
 }
 
-// Calls fn(Ops<FI, FO, K>{}) for the runtime field degrees and table count.
-template <class Fn>
+// Calls fn(O<FI, FO, K>{}) for the runtime field degrees and table count.
+template <template <class, class, int> class O = Ops, class Fn>
 int dispatch(int fi, int fo, int K, Fn&& fn) {
-  if (fi == 1 && fo == 1) return with_k<Ops, Fp, Fp>(K, fn);
-  if (fi == 1 && fo == 3) return with_k<Ops, Fp, Fp3>(K, fn);
-  if (fi == 3 && fo == 3) return with_k<Ops, Fp3, Fp3>(K, fn);
+  if (fi == 1 && fo == 1) return with_k<O, Fp, Fp>(K, fn);
+  if (fi == 1 && fo == 3) return with_k<O, Fp, Fp3>(K, fn);
+  if (fi == 3 && fo == 3) return with_k<O, Fp3, Fp3>(K, fn);
   return fail(EFFSC_HIP_EINVAL, "unsupported field combination");
 }
 
@@ -260,12 +327,13 @@ int read_elements(effsc_hip_prover* h, size_t idx, uint64_t* out) {
   return EFFSC_HIP_OK;
 }
 
-int fetch_result(effsc_hip_prover* h, int fi, uint64_t* out_vals) {
-  // out_vals: K values of the prover's field (ext words each)
-  HIPCHK(hipMemcpyAsync(h->h_res, h->red.result, (size_t)h->K * fi * 8, hipMemcpyDeviceToHost, h->stream));
+// Copies `count` reduced values (fi words each on the device) into out_vals,
+// lifted to the prover's field (ext words each).
+int fetch_result(effsc_hip_prover* h, int fi, uint64_t* out_vals, int count) {
+  HIPCHK(hipMemcpyAsync(h->h_res, h->red.result, (size_t)count * fi * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   TRY(prof_collect(h));
-  for (int i = 0; i < h->K; i++)
+  for (int i = 0; i < count; i++)
     for (int c = 0; c < h->ext; c++) out_vals[i * h->ext + c] = c < fi ? h->h_res[i * fi + c] : 0;
   return EFFSC_HIP_OK;
 }
@@ -298,7 +366,7 @@ int evaluate(effsc_hip_prover* h, bool already, uint64_t* vals) {
   size_t L = h->len;
   const int E = h->ext;
   std::fill(vals, vals + (size_t)h->K * E, 0);
-  if (already) return fetch_result(h, h->ext, vals);
+  if (already) return fetch_result(h, h->ext, vals, h->K);
   if (h->kind != K_PRODUCT && L <= 1) {
     if (L == 0) return EFFSC_HIP_OK;
     // n == 1: ML -> [v0]; IP -> [a0 b0, 0] (src/provers/inner_product.rs:74-81)
@@ -321,7 +389,7 @@ int evaluate(effsc_hip_prover* h, bool already, uint64_t* vals) {
   uint64_t* const* src = src_planes(h);
   size_t sst = src_stride(h);
   TRY(dispatch(fi, fi, h->K, [&](auto op) { return decltype(op)::eval(h, src, sst, half, paired); }));
-  return fetch_result(h, fi, vals);
+  return fetch_result(h, fi, vals, h->K);
 }
 
 // EvalsInfty wire from the K product values (h(0), h(inf), h(2), ...): the
@@ -402,14 +470,15 @@ void destroy(effsc_hip_prover* h) {
     if (h->work[i]) (void)hipFree(h->work[i]);
   }
   if (h->red.partials) (void)hipFree(h->red.partials);
-  if (h->red.counter) (void)hipFree(h->red.counter);
   if (h->red.result) (void)hipFree(h->red.result);
   if (h->eq_l) (void)hipFree(h->eq_l);
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_inner) (void)hipFree(h->eq_inner);
   if (h->h_res) (void)hipHostFree(h->h_res);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
+  if (h->d_fs) (void)hipFree(h->d_fs);
+  if (h->h_proof) (void)hipHostFree(h->h_proof);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -421,10 +490,6 @@ int setup_common(effsc_hip_prover* h, size_t cap) {
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->grid_max = grid_max_for(h->dev);
   TRY(alloc_dev(&h->red.partials, (size_t)h->grid_max * kMaxK * 3));
-  uint64_t* ctr = nullptr;
-  TRY(alloc_dev(&ctr, 1));
-  h->red.counter = reinterpret_cast<unsigned*>(ctr);
-  HIPCHK(hipMemsetAsync(h->red.counter, 0, 8, h->stream));
   TRY(alloc_dev(&h->red.result, kMaxK * 3));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_res), kMaxK * 3 * 8, hipHostMallocDefault));
   h->work_stride = round_up(std::max<size_t>(cap, 1), 64);
@@ -570,7 +635,7 @@ int eq_dispatch_left(effsc_hip_prover* h) {
                        h->w, h->red);
   }
   HIPCHK(hipGetLastError());
-  return EFFSC_HIP_OK;
+  return h->ext == 1 ? reduce<Fp, 2>(h, g) : reduce<Fp3, 2>(h, g);
 }
 
 // Folds one eq half-table in place (ip::fold semantics on a pow2 table).
@@ -581,12 +646,12 @@ int eq_fold_table(effsc_hip_prover* h, uint64_t* planes, size_t stride, size_t* 
   uint64_t* pl[kMaxK] = {planes};
   if (h->ext == 1) {
     Views<Fp, 1> v = views<Fp, 1>(pl, stride);
-    hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp>(c), half, half,
-                       h->w);
+    hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp>(c), nullptr,
+                       half, half, h->w);
   } else {
     Views<Fp3, 1> v = views<Fp3, 1>(pl, stride);
-    hipLaunchKernelGGL((k_fold<Fp3, Fp3, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp3>(c), half,
-                       half, h->w);
+    hipLaunchKernelGGL((k_fold<Fp3, Fp3, 1>), dim3(g), dim3(kBlock), 0, h->stream, v, v, from_words<Fp3>(c),
+                       nullptr, half, half, h->w);
   }
   HIPCHK(hipGetLastError());
   *len = half;
@@ -626,7 +691,7 @@ int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   if (h->rounds_elapsed < h->v_l) {
     TRY(eq_dispatch_left(h));
     uint64_t vals[kMaxK * 3];
-    TRY(fetch_result(h, E, vals));
+    TRY(fetch_result(h, E, vals, 2));
     std::copy(vals, vals + 2 * E, out);
   } else {
     // round_poly_right (src/provers/eq_factored.rs:178-209): the inner-product
@@ -664,8 +729,9 @@ int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
         hipLaunchKernelGGL((k_eval<Fp3, 2, 1>), dim3(g), dim3(kBlock), 0, h->stream, t, half, half, h->w, h->red);
       }
       HIPCHK(hipGetLastError());
+      TRY((E == 1 ? reduce<Fp, 2>(h, g) : reduce<Fp3, 2>(h, g)));
       uint64_t vals[kMaxK * 3];
-      TRY(fetch_result(h, E, vals));
+      TRY(fetch_result(h, E, vals, 2));
       if (E == 1) {
         out[0] = gl::mul(vals[0], scalar[0]);
         out[1] = gl::mul(vals[1], scalar[0]);
@@ -724,6 +790,132 @@ int build_eq(effsc_hip_prover* h, const uint64_t* wpt_canon_dev, size_t off, siz
     hipLaunchKernelGGL(k_eq_table<Fp3>, dim3(g), dim3(kBlock), 0, h->stream, *planes, *stride,
                        wpt_canon_dev + off * h->ext, vv, h->w);
   HIPCHK(hipGetLastError());
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Device-resident proof loop (effsc_hip_sumcheck with no hook)
+// ===========================================================================
+namespace {
+
+bool device_loop_applies(const effsc_hip_prover* h, size_t num_rounds) {
+  if (h->kind == K_EQ || num_rounds == 0 || num_rounds > kMaxRounds) return false;
+  if (!is_pow2(h->len)) return false;
+  size_t logl = 0;
+  while (((size_t)1 << logl) < h->len) logl++;
+  return num_rounds <= logl;  // every round evaluates a table of length >= 2
+}
+
+int ensure_dev_bufs(effsc_hip_prover* h) {
+  if (h->d_fs) return EFFSC_HIP_OK;
+  uint64_t* fs = nullptr;
+  TRY(alloc_dev(&fs, (sizeof(DevFs) + 7) / 8));
+  h->d_fs = reinterpret_cast<DevFs*>(fs);
+  TRY(alloc_dev(&h->d_polys, kMaxRounds * kMaxK * 3));
+  TRY(alloc_dev(&h->d_chals, kMaxRounds * 3));
+  TRY(alloc_dev(&h->d_cur, 4));
+  TRY(alloc_dev(&h->d_fin, 4));
+  size_t words = kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4 + (sizeof(DevFs) + 7) / 8;
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), words * 8, hipHostMallocDefault));
+  return EFFSC_HIP_OK;
+}
+
+struct DevModeGuard {
+  effsc_hip_prover* h;
+  explicit DevModeGuard(effsc_hip_prover* p) : h(p) { h->dev_mode = true; }
+  ~DevModeGuard() { h->dev_mode = false; }
+};
+
+// Runs rounds [0, num_rounds), finalize and final_value on the device with
+// the transcript state copied in and out of `t`.  Words out are canonical.
+int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, uint64_t* round_polys,
+                 uint64_t* challenges, uint64_t* final_value) {
+  TRY(ensure_dev_bufs(h));
+  DevModeGuard guard(h);
+  const int E = h->ext, K = h->K;
+  // transcript state -> device
+  DevFs* hs = reinterpret_cast<DevFs*>(h->h_proof + kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4);
+  std::memset(hs, 0, sizeof(DevFs));
+  hs->kind = (uint32_t)t->kind;
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) hs->st[i] = t->fs.sp.st[i];
+    hs->pos = t->fs.sp.pos;
+  }
+  hs->rng = t->rng;
+  HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+
+  size_t L = h->len, r = 0;
+  bool have = false, tail_used = false;
+  while (r < num_rounds) {
+    if (L <= tail_max()) break;
+    int fi = src_field(h);
+    uint64_t* const* src = src_planes(h);
+    size_t sst = src_stride(h);
+    int fr;  // field of the reduced values
+    if (have) {
+      TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fused(h, src, sst, nullptr, L / 4, h->d_cur); }));
+      L /= 2;
+      h->in_work = true;
+      fr = E;
+    } else {
+      TRY(dispatch(fi, fi, K, [&](auto op) { return decltype(op)::eval(h, src, sst, L / 2, L / 2); }));
+      fr = fi;
+    }
+    unsigned g = h->last_grid;
+    TRY(dispatch<DevOps>(fr, fr, K, [&](auto op) { return decltype(op)::reduce_fs(h, g, r); }));
+    r++;
+    have = true;
+  }
+  if (r < num_rounds || L <= tail_max()) {
+    // remaining rounds, finalize and final value in one workgroup
+    int fi = src_field(h);
+    bool from_input = !h->in_work;
+    TRY(dispatch<DevOps>(fi, E, K, [&](auto op) {
+      return decltype(op)::tail(h, src_planes(h), src_stride(h), from_input, L, (int)(num_rounds - r), have, r);
+    }));
+    // folds inside the tail: one per round that has a challenge, then the
+    // final fold with the last challenge
+    size_t rounds_left = num_rounds - r;
+    size_t folds = rounds_left == 0 ? 0 : (have ? rounds_left : rounds_left - 1);
+    size_t Lt = L >> folds;
+    if ((have || rounds_left > 0) && Lt >= 2) Lt /= 2;
+    h->len = Lt;
+    h->in_work = true;
+    tail_used = true;
+  } else {
+    // partial run ending on a large table: finalize with the multi-block fold
+    size_t half = L / 2;
+    int fi = src_field(h);
+    uint64_t* const* src = src_planes(h);
+    size_t sst = src_stride(h);
+    TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, half, half, h->d_cur); }));
+    h->len = half;
+    h->in_work = true;
+  }
+  // proof and transcript state -> host
+  uint64_t* hp = h->h_proof;
+  uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
+  uint64_t* hf = hc + kMaxRounds * 3;
+  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * K * E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
+  if (tail_used) HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
+  std::copy(hp, hp + num_rounds * K * E, round_polys);
+  std::copy(hc, hc + num_rounds * E, challenges);
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) t->fs.sp.st[i] = hs->st[i];
+    t->fs.sp.pos = hs->pos;
+  }
+  t->rng = hs->rng;
+  if (tail_used) {
+    std::copy(hf, hf + E, final_value);
+  } else {
+    TRY(final_value_impl(h, final_value));
+  }
   return EFFSC_HIP_OK;
 }
 
@@ -944,12 +1136,8 @@ void effsc_hip_free(effsc_hip_prover* h) { destroy(h); }
 int effsc_hip_profile(effsc_hip_prover* h, int enable) {
   if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
   HIPCHK(hipSetDevice(h->dev));
-  if (enable && !h->ev0) {
-    HIPCHK(hipEventCreate(&h->ev0));
-    HIPCHK(hipEventCreate(&h->ev1));
-  }
   h->prof_on = enable != 0;
-  h->prof_pending = -1;
+  h->pend.clear();
   for (int i = 0; i < 3; i++) {
     h->prof_launches[i] = 0;
     h->prof_ms[i] = 0;
@@ -986,7 +1174,7 @@ int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size
   Views<Fp, 1> v = views<Fp, 1>(pl, n);
   size_t b = std::min<size_t>((half + kBlock - 1) / kBlock, 8192);
   hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3((unsigned)std::max<size_t>(b, 1)), dim3(kBlock), 0, nullptr, v, v,
-                     Fp{gl::canon(weight)}, half, paired, (uint64_t)0);
+                     Fp{gl::canon(weight)}, nullptr, half, paired, (uint64_t)0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   if (new_len) *new_len = half;
@@ -994,12 +1182,6 @@ int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size
 }
 
 // ----------------------------------------------------------------- transcripts
-struct effsc_hip_transcript {
-  int kind;  // 0: Fiat-Shamir, 1: test RNG
-  FsTranscript fs;
-  uint64_t rng;
-};
-
 effsc_hip_transcript* effsc_hip_transcript_new(const uint8_t* domsep, size_t len) {
   auto* t = new effsc_hip_transcript();
   t->kind = 0;
@@ -1046,6 +1228,8 @@ int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcr
   if (!h || !t || !final_value || (num_rounds && (!round_polys || !challenges)))
     return fail(EFFSC_HIP_EINVAL, "null argument");
   HIPCHK(hipSetDevice(h->dev));
+  if (!hook && device_loop_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
+    return prove_device(h, num_rounds, t, round_polys, challenges, final_value);
   const int E = h->ext;
   const size_t nw = (size_t)std::max(h->deg, 1);
   uint64_t prev[3];

@@ -35,12 +35,14 @@ using gl::Fp3;
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kMaxK = 4;             // tables per product
-constexpr int kMaxWaves = kBlock / 64;
+constexpr int kMaxWaves = 16;              // block_reduce supports up to 1024 threads
 
 // ---------------------------------------------------------------------------
 // Table views.  Base tables are one u64 plane; Fp3 tables are three planes
 // (structure of arrays) `stride` words apart, so every load is coalesced.
 // ---------------------------------------------------------------------------
+typedef uint64_t v2u64 __attribute__((ext_vector_type(2)));
+
 template <class F>
 struct View;
 
@@ -57,6 +59,12 @@ struct View<Fp> {
   }
   __device__ __forceinline__ void st2(size_t i, Fp a, Fp b) const {
     *reinterpret_cast<ulonglong2*>(p + i) = make_ulonglong2(a.v, b.v);
+  }
+  // streaming read: no cache allocation (data read once)
+  __device__ __forceinline__ void ld2_nt(size_t i, Fp& a, Fp& b) const {
+    v2u64 q = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + i));
+    a.v = q.x;
+    b.v = q.y;
   }
 };
 
@@ -82,6 +90,13 @@ struct View<Fp3> {
     *reinterpret_cast<ulonglong2*>(p + stride + i) = make_ulonglong2(a.c1, b.c1);
     *reinterpret_cast<ulonglong2*>(p + 2 * stride + i) = make_ulonglong2(a.c2, b.c2);
   }
+  __device__ __forceinline__ void ld2_nt(size_t i, Fp3& a, Fp3& b) const {
+    v2u64 q0 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + i));
+    v2u64 q1 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + stride + i));
+    v2u64 q2 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + 2 * stride + i));
+    a = {q0.x, q1.x, q2.x};
+    b = {q0.y, q1.y, q2.y};
+  }
 };
 
 template <class F, int K>
@@ -89,11 +104,10 @@ struct Views {
   View<F> t[K];
 };
 
-// Reduction scratch owned by one prover: per-block partials, a ticket counter
-// (returned to zero by the last block) and the final K values.
+// Reduction scratch owned by one prover: per-block partials and the final K
+// values.
 struct Reduce {
   uint64_t* partials;  // gridDim * K * D words
-  unsigned* counter;
   uint64_t* result;    // K * D words
 };
 
@@ -170,41 +184,34 @@ __device__ __forceinline__ void block_reduce(F* acc) {
   __syncthreads();
 }
 
-// Writes this block's partial; the last block to arrive sums all partials,
-// writes the K results and re-arms the counter.
+// Writes this block's partial (K values) to r.partials[blockIdx.x].  The
+// cross-block sum is a separate one-workgroup kernel (k_reduce) launched next
+// on the same stream: a last-block-arrives scheme would need a device-scope
+// release fence per workgroup, which on gfx950 writes back the XCD's L2 and
+// costs far more than one extra dispatch.
 template <class F, int K>
 __device__ __forceinline__ void finish(F* acc, Reduce r) {
   block_reduce<F, K>(acc);
-  __shared__ bool last;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < K; i++) gl::store_words(acc[i], r.partials + ((size_t)blockIdx.x * K + i) * F::D);
-    __threadfence();
-    unsigned ticket = atomicAdd(r.counter, 1u);
-    last = (ticket == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// Sums `nblocks` partials of K values each into r.result (one workgroup).
+template <class F, int K>
+__global__ __launch_bounds__(kBlock) void k_reduce(Reduce r, unsigned nblocks) {
   F s[K];
 #pragma unroll
   for (int i = 0; i < K; i++) s[i] = gl::zero<F>();
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-      const uint64_t* src = r.partials + ((size_t)b * K + i) * F::D;
-      uint64_t wds[3];
-#pragma unroll
-      for (int c = 0; c < F::D; c++) wds[c] = __atomic_load_n(src + c, __ATOMIC_RELAXED);
-      s[i] = gl::add(s[i], gl::load_words<F>(wds));
-    }
+    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * K + i) * F::D));
   }
   block_reduce<F, K>(s);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < K; i++) gl::store_words(s[i], r.result + i * F::D);
-    *r.counter = 0;
-    __threadfence();
   }
 }
 
@@ -236,8 +243,8 @@ __global__ __launch_bounds__(kBlock) void k_eval(Views<F, K> t, size_t half, siz
       F lo0[K], lo1[K], hi0[K], hi1[K];
 #pragma unroll
       for (int i = 0; i < K; i++) {
-        t.t[i].ld2(k, lo0[i], lo1[i]);
-        t.t[i].ld2(k + half, hi0[i], hi1[i]);
+        t.t[i].ld2_nt(k, lo0[i], lo1[i]);
+        t.t[i].ld2_nt(k + half, hi0[i], hi1[i]);
       }
       eval_pair<F, K>(lo0, hi0, acc, w);
       eval_pair<F, K>(lo1, hi1, acc, w);
@@ -261,8 +268,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(Views<F, K> t, size_t half, siz
 // alias: positions k and k+q are read and written by the same thread only,
 // positions k+2q and k+3q are only read.  q = L/4.  VEC = 2 requires even q.
 template <class FI, class FO, int K, int VEC>
-__global__ __launch_bounds__(kBlock) void k_fused(Views<FI, K> in, Views<FO, K> out, FO c, size_t q, uint64_t w,
-                                                  Reduce r) {
+__global__ __launch_bounds__(kBlock) void k_fused(Views<FI, K> in, Views<FO, K> out, FO c, const uint64_t* cptr,
+                                                  size_t q, uint64_t w, Reduce r) {
+  if (cptr) c = gl::load_words<FO>(cptr);
   FO acc[K];
 #pragma unroll
   for (int i = 0; i < K; i++) acc[i] = gl::zero<FO>();
@@ -311,8 +319,9 @@ __global__ __launch_bounds__(kBlock) void k_fused(Views<FI, K> in, Views<FO, K> 
 // in[k] (1 - c) for paired <= k < half.  May run in place (thread k reads k
 // and k+half and writes k).
 template <class FI, class FO, int K>
-__global__ __launch_bounds__(kBlock) void k_fold(Views<FI, K> in, Views<FO, K> out, FO c, size_t half,
-                                                 size_t paired, uint64_t w) {
+__global__ __launch_bounds__(kBlock) void k_fold(Views<FI, K> in, Views<FO, K> out, FO c, const uint64_t* cptr,
+                                                 size_t half, size_t paired, uint64_t w) {
+  if (cptr) c = gl::load_words<FO>(cptr);
   const size_t step = (size_t)gridDim.x * blockDim.x;
   for (size_t k = gtid(); k < half; k += step) {
 #pragma unroll

@@ -1,0 +1,233 @@
+// Device-resident proof loop: the Fiat-Shamir transcript lives in device
+// memory, so a whole sumcheck (src/runner.rs:29-69 with the no-op hook) is
+// enqueued without a host round trip per round.
+//
+// Per round while the tables are large:
+//   k_fused / k_eval          fold with the challenge read from device memory
+//                             and evaluate; one partial per workgroup
+//   k_reduce_fs               one workgroup: sum the partials, write the round
+//                             polynomial, absorb it into the transcript and
+//                             squeeze the next challenge into device memory
+// Once the tables are small, k_tail runs every remaining round, the final fold
+// and the final value in a single workgroup (tables stay in L2).
+#pragma once
+#include "kernels.hpp"
+#include "shake.hpp"
+
+namespace effsc_hip {
+
+// Transcript state in device memory (host FsTranscript / test RNG mirror).
+struct DevFs {
+  uint64_t st[25];
+  uint32_t pos;
+  uint32_t kind;  // 0: SHAKE256 Fiat-Shamir, 1: test RNG (splitmix64)
+  uint64_t rng;
+};
+
+// Register-resident sponge: every state index is static after unrolling, so
+// the 25 lanes stay in VGPRs (a byte at a runtime position is xor-ed through a
+// predicated sweep over the 17 rate lanes).
+struct RegFs {
+  uint64_t st[25];
+  uint32_t pos, kind;
+  uint64_t rng;
+
+  __device__ __forceinline__ void load(const DevFs* t) {
+#pragma unroll
+    for (int i = 0; i < 25; i++) st[i] = t->st[i];
+    pos = t->pos;
+    kind = t->kind;
+    rng = t->rng;
+  }
+  __device__ __forceinline__ void store(DevFs* t) const {
+#pragma unroll
+    for (int i = 0; i < 25; i++) t->st[i] = st[i];
+    t->pos = pos;
+    t->rng = rng;
+  }
+  __device__ __forceinline__ void xor_byte(uint32_t p, uint64_t b) {
+#pragma unroll
+    for (int i = 0; i < kShakeRate / 8; i++)
+      if ((uint32_t)i == (p >> 3)) st[i] ^= b << (8 * (p & 7));
+  }
+  __device__ __forceinline__ void absorb_byte(uint64_t b) {
+    xor_byte(pos, b);
+    if (++pos == kShakeRate) {
+      keccak_f1600(st);
+      pos = 0;
+    }
+  }
+  __device__ __forceinline__ void send(int ext, const uint64_t* w) {
+    if (kind != 0) return;
+    absorb_byte(0x01);
+    for (int i = 0; i < ext; i++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) absorb_byte((w[i] >> (8 * j)) & 0xFF);
+  }
+  __device__ __forceinline__ void challenge(int ext, uint64_t* out) {
+    if (kind == 1) {
+      rng += 0x9E3779B97F4A7C15ull;
+      out[0] = gl::canon(mix64(rng));
+      out[1] = out[2] = 0;
+      return;
+    }
+    absorb_byte(0x02);
+    uint64_t c[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) c[i] = st[i];
+    // SHAKE padding: 0x1F at the current position, 0x80 in the last rate byte
+#pragma unroll
+    for (int i = 0; i < kShakeRate / 8; i++)
+      if ((uint32_t)i == (pos >> 3)) c[i] ^= (uint64_t)0x1F << (8 * (pos & 7));
+    c[(kShakeRate - 1) / 8] ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) % 8));
+    keccak_f1600(c);
+    // component i = little-endian u128 of squeezed bytes 16i..16i+15
+    out[0] = gl::reduce128(c[0], c[1]);
+    out[1] = gl::reduce128(c[2], c[3]);
+    out[2] = gl::reduce128(c[4], c[5]);
+    for (int i = ext; i < 3; i++) out[i] = 0;
+  }
+};
+
+// Thread 0 only: writes the K round values (lifted to ext words) to `poly`,
+// absorbs them, squeezes the challenge into chal_out and chal_cur.
+template <class F, int K>
+__device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64_t* poly, uint64_t* chal_out,
+                                           uint64_t* chal_cur) {
+  RegFs fs;
+  fs.load(t);
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t wd[3] = {0, 0, 0};
+    gl::store_words(s[i], wd);
+    for (int c = 0; c < ext; c++) poly[i * ext + c] = wd[c];
+    fs.send(ext, wd);
+  }
+  uint64_t ch[3];
+  fs.challenge(ext, ch);
+  for (int c = 0; c < ext; c++) {
+    chal_out[c] = ch[c];
+    chal_cur[c] = ch[c];
+  }
+  fs.store(t);
+}
+
+// Cross-block sum of one round plus the transcript step.
+template <class F, int K>
+__global__ __launch_bounds__(kBlock) void k_reduce_fs(Reduce r, unsigned nblocks, DevFs* t, int ext, uint64_t* poly,
+                                                      uint64_t* chal_out, uint64_t* chal_cur) {
+  F s[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) s[i] = gl::zero<F>();
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * K + i) * F::D));
+  }
+  block_reduce<F, K>(s);
+  if (threadIdx.x == 0) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
+}
+
+constexpr int kTailBlock = 1024;
+
+// All remaining rounds in one workgroup.  On entry the K tables have pow2
+// length L (>= 2); they are in `in` (field FI) when from_input, else in
+// `work` (field F).  Round r (0 <= r < rounds) folds with the current
+// challenge when have_chal (fused with the evaluation, L >= 4), evaluates
+// pairs (k, k + L/2) and runs the transcript step; afterwards the last
+// challenge folds the tables once more and the final value
+// (prod_i table_i[0] when the length reached 1, else 0) goes to fin.
+template <class FI, class F, int K>
+__global__ __launch_bounds__(kTailBlock) void k_tail(Views<FI, K> in, Views<F, K> work, int from_input, size_t L,
+                                                     int rounds, int have_chal, DevFs* t, int ext, uint64_t* polys,
+                                                     uint64_t* chals, uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
+  __shared__ uint64_t chal_sh[3];
+  if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal ? chal_cur[threadIdx.x] : 0;
+  __syncthreads();
+  const unsigned tid = threadIdx.x, nt = blockDim.x;
+  bool src_in = from_input != 0;
+  for (int r = 0; r < rounds; r++) {
+    F acc[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) acc[i] = gl::zero<F>();
+    if (have_chal) {
+      F c = gl::load_words<F>(chal_sh);
+      size_t q = L / 4;
+      // thread k reads positions k, k+q, k+2q, k+3q and alone writes k and
+      // k+q; positions >= 2q are only read, so the fold runs in place
+      for (size_t k = tid; k < q; k += nt) {
+        F lo[K], hi[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          if (src_in) {
+            FI x0 = in.t[i].ld(k), x1 = in.t[i].ld(k + q), x2 = in.t[i].ld(k + 2 * q), x3 = in.t[i].ld(k + 3 * q);
+            lo[i] = fold1<F>(x0, x2, c, w);
+            hi[i] = fold1<F>(x1, x3, c, w);
+          } else {
+            F x0 = work.t[i].ld(k), x1 = work.t[i].ld(k + q), x2 = work.t[i].ld(k + 2 * q), x3 = work.t[i].ld(k + 3 * q);
+            lo[i] = fold1<F>(x0, x2, c, w);
+            hi[i] = fold1<F>(x1, x3, c, w);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          work.t[i].st(k, lo[i]);
+          work.t[i].st(k + q, hi[i]);
+        }
+        eval_pair<F, K>(lo, hi, acc, w);
+      }
+      L /= 2;
+      src_in = false;
+    } else {
+      size_t half = L / 2;
+      for (size_t k = tid; k < half; k += nt) {
+        F lo[K], hi[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          if (src_in) {
+            lo[i] = gl::lift_to<F>(in.t[i].ld(k));
+            hi[i] = gl::lift_to<F>(in.t[i].ld(k + half));
+          } else {
+            lo[i] = work.t[i].ld(k);
+            hi[i] = work.t[i].ld(k + half);
+          }
+        }
+        eval_pair<F, K>(lo, hi, acc, w);
+      }
+    }
+    block_reduce<F, K>(acc);
+    if (tid == 0) emit_round<F, K>(acc, ext, t, polys + (size_t)r * K * ext, chals + (size_t)r * ext, chal_sh);
+    __syncthreads();
+    have_chal = 1;
+  }
+  // finalize: fold with the last challenge (src/runner.rs:57-60)
+  if (have_chal && L >= 2) {
+    F c = gl::load_words<F>(chal_sh);
+    size_t half = L / 2;
+    for (size_t k = tid; k < half; k += nt) {
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        F v;
+        if (src_in)
+          v = fold1<F>(in.t[i].ld(k), in.t[i].ld(k + half), c, w);
+        else
+          v = fold1<F>(work.t[i].ld(k), work.t[i].ld(k + half), c, w);
+        work.t[i].st(k, v);
+      }
+    }
+    L = half;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    F v = gl::zero<F>();
+    if (L == 1) {
+      v = work.t[0].ld(0);
+#pragma unroll
+      for (int i = 1; i < K; i++) v = gl::mul(v, work.t[i].ld(0), w);
+    }
+    uint64_t wd[3] = {0, 0, 0};
+    gl::store_words(v, wd);
+    for (int c = 0; c < ext; c++) fin[c] = wd[c];
+  }
+}
+
+}  // namespace effsc_hip

@@ -38,15 +38,22 @@ __host__ __device__ inline void keccak_f1600(uint64_t* s) {
   const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
   for (int round = 0; round < 24; round++) {
     uint64_t c[5], b[25];
+#pragma unroll
     for (int x = 0; x < 5; x++) c[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
+#pragma unroll
     for (int x = 0; x < 5; x++) {
       uint64_t d = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
+#pragma unroll
       for (int y = 0; y < 25; y += 5) s[x + y] ^= d;
     }
     // rho + pi: lane (x, y) moves to (y, 2x + 3y)
+#pragma unroll
     for (int x = 0; x < 5; x++)
+#pragma unroll
       for (int y = 0; y < 5; y++) b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(s[x + 5 * y], rot[x + 5 * y]);
+#pragma unroll
     for (int y = 0; y < 25; y += 5)
+#pragma unroll
       for (int x = 0; x < 5; x++) s[x + y] = b[x + y] ^ (~b[(x + 1) % 5 + y] & b[(x + 2) % 5 + y]);
     s[0] ^= keccak_rc(round);
   }

@@ -70,7 +70,7 @@ def check_proof(kind, ext, input_ext, n, v, seed, transcript="shake", rounds=Non
         tg, to = TestTranscript(seed, ext=ext), O.Transcript.rng(seed)
     pg = sumcheck(g, rounds, tg)
     po = o.sumcheck(rounds, to)
-    assert as_oracle(pg, ext) == list(po)
+    assert tuple(as_oracle(pg, ext)) == tuple(po)
     # post-state
     for i in range(len(tabs) if kind != "eq" else 3):
         assert np.array_equal(g.table_words(i), o.table(i)), f"table {i}"
@@ -224,7 +224,7 @@ def test_borrowed_device_input_and_fill_uniform():
         g = InnerProductProver(addr[0], addr[1], borrow=True, n=n)
         pg = sumcheck(g, v, Transcript(b"b"))
         po = O.Prover.inner_product(O.uniform(n, 8), O.uniform(n, 9)).sumcheck(v, O.Transcript.shake(b"b"))
-        assert as_oracle(pg, 1) == list(po)
+        assert tuple(as_oracle(pg, 1)) == tuple(po)
         g.close()
         # input untouched by the proof
         N.check(N.lib().effsc_hip_copy(0, host.ctypes.data_as(N.u64p), ptrs[0], n))
@@ -269,3 +269,63 @@ def test_errors_are_reported_not_raised_natively():
         CoefficientProver([[1, 2, 3]])  # non-power-of-two product table
     with pytest.raises(N.NativeError):
         EqFactoredProver([1, 2], [1, 2, 3, 4, 5])  # more than 2^v evaluations
+
+
+# ------------------------------------------- device-resident proof loop
+def _proof_both_loops(make, rounds, transcript_factory, monkeypatch, tail):
+    monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
+    monkeypatch.delenv("EFFSC_HIP_HOST_LOOP", raising=False)
+    dev = sumcheck(make(), rounds, transcript_factory())
+    monkeypatch.setenv("EFFSC_HIP_HOST_LOOP", "1")
+    host = sumcheck(make(), rounds, transcript_factory())
+    monkeypatch.delenv("EFFSC_HIP_HOST_LOOP")
+    return dev, host
+
+
+@pytest.mark.parametrize("tail", [2, 64, 1 << 14, 1 << 22])
+@pytest.mark.parametrize("kind,ext,input_ext", [("ml", 1, 1), ("ip", 1, 1), ("ip", 3, 3), ("p3", 1, 1),
+                                                ("p3", 3, 1), ("p4", 3, 3), ("p2", 3, 1)])
+@pytest.mark.parametrize("v", [1, 2, 5, 11, 16])
+def test_device_loop_matches_oracle(kind, ext, input_ext, v, tail, monkeypatch):
+    k = {"ml": 1, "ip": 2}.get(kind) or int(kind[1:])
+    n = 1 << v
+    tabs = [O.uniform(n * input_ext, 7000 + 13 * v + i) for i in range(k)]
+    for rounds in sorted({v, max(v - 1, 1), 1}):
+        for tr in ("shake", "rng"):
+            say(f"device loop kind={kind} ext={ext}/{input_ext} v={v} rounds={rounds} tail={tail} tr={tr}")
+
+            def make():
+                return make_pair(kind, ext, input_ext, tabs)[0]
+
+            tf = (lambda: Transcript(b"dev", ext=ext)) if tr == "shake" else (lambda: TestTranscript(99, ext=ext))
+            dev, host = _proof_both_loops(make, rounds, tf, monkeypatch, tail)
+            assert dev.round_polys == host.round_polys
+            assert dev.challenges == host.challenges
+            assert dev.final_value == host.final_value
+            o = make_pair(kind, ext, input_ext, tabs)[1]
+            po = o.sumcheck(rounds, O.Transcript.shake(b"dev") if tr == "shake" else O.Transcript.rng(99))
+            assert tuple(as_oracle(dev, ext)) == tuple(po)
+
+
+@pytest.mark.parametrize("tail", [2, 1 << 14])
+def test_device_loop_partial_then_continue(tail, monkeypatch):
+    monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
+    a, b = O.uniform(1 << 16, 1), O.uniform(1 << 16, 2)
+    full = sumcheck(InnerProductProver(a, b), 16, Transcript(b"p"))
+    g = InnerProductProver(a, b)
+    t = Transcript(b"p")
+    parts = [sumcheck(g, r, t) for r in (3, 9, 4)]
+    assert sum((p.round_polys for p in parts), []) == full.round_polys
+    assert sum((p.challenges for p in parts), []) == full.challenges
+    assert parts[-1].final_value == full.final_value
+    # post-state after partial runs equals the oracle's
+    o = O.Prover.inner_product(a, b)
+    ot = O.Transcript.shake(b"p")
+    for r in (3, 9):
+        o.sumcheck(r, ot)
+    g2 = InnerProductProver(a, b)
+    t2 = Transcript(b"p")
+    for r in (3, 9):
+        sumcheck(g2, r, t2)
+    assert np.array_equal(g2.table_words(0), o.table(0))
+    assert np.array_equal(g2.table_words(1), o.table(1))
