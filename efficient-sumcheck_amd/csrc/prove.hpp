@@ -24,20 +24,29 @@ struct DevFs {
   uint64_t rng;
 };
 
-// Register-resident sponge: every state index is static after unrolling, so
-// the 25 lanes stay in VGPRs (a byte at a runtime position is xor-ed through a
-// predicated sweep over the 17 rate lanes).
+// Wave-uniform copy of a 64-bit value held by lane 0.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Register-resident sponge, run by a whole wave on wave-uniform values: every
+// state index is static after unrolling and every value is uniform, so the
+// permutation runs on the scalar ALU (64-bit s_xor/s_andn2/s_lshl) in SGPRs.
+// Items are whole words, so absorbing xors one lane chosen by a predicated
+// (branch-free) sweep over the 17 rate lanes.
 struct RegFs {
   uint64_t st[25];
-  uint32_t pos, kind;
+  uint32_t pos, kind;  // pos in bytes, a multiple of 8
   uint64_t rng;
 
   __device__ __forceinline__ void load(const DevFs* t) {
 #pragma unroll
-    for (int i = 0; i < 25; i++) st[i] = t->st[i];
-    pos = t->pos;
-    kind = t->kind;
-    rng = t->rng;
+    for (int i = 0; i < 25; i++) st[i] = uniform64(t->st[i]);
+    pos = __builtin_amdgcn_readfirstlane(t->pos);
+    kind = __builtin_amdgcn_readfirstlane(t->kind);
+    rng = uniform64(t->rng);
   }
   __device__ __forceinline__ void store(DevFs* t) const {
 #pragma unroll
@@ -45,24 +54,23 @@ struct RegFs {
     t->pos = pos;
     t->rng = rng;
   }
-  __device__ __forceinline__ void xor_byte(uint32_t p, uint64_t b) {
+  __device__ __forceinline__ void absorb_word(uint64_t x) {
+    const uint32_t lane = pos >> 3;
 #pragma unroll
-    for (int i = 0; i < kShakeRate / 8; i++)
-      if ((uint32_t)i == (p >> 3)) st[i] ^= b << (8 * (p & 7));
-  }
-  __device__ __forceinline__ void absorb_byte(uint64_t b) {
-    xor_byte(pos, b);
-    if (++pos == kShakeRate) {
+    for (int i = 0; i < kShakeRate / 8; i++) st[i] ^= x & (0 - (uint64_t)((uint32_t)i == lane));
+    pos += 8;
+    if (pos == kShakeRate) {
       keccak_f1600(st);
       pos = 0;
     }
   }
+  // w holds 3 words; the first ext are sent
   __device__ __forceinline__ void send(int ext, const uint64_t* w) {
     if (kind != 0) return;
-    absorb_byte(0x01);
-    for (int i = 0; i < ext; i++)
+    absorb_word(1);
 #pragma unroll
-      for (int j = 0; j < 8; j++) absorb_byte((w[i] >> (8 * j)) & 0xFF);
+    for (int i = 0; i < 3; i++)
+      if (i < ext) absorb_word(w[i]);
   }
   __device__ __forceinline__ void challenge(int ext, uint64_t* out) {
     if (kind == 1) {
@@ -71,45 +79,55 @@ struct RegFs {
       out[1] = out[2] = 0;
       return;
     }
-    absorb_byte(0x02);
+    absorb_word(2);
     uint64_t c[25];
+    const uint32_t lane = pos >> 3;
 #pragma unroll
     for (int i = 0; i < 25; i++) c[i] = st[i];
-    // SHAKE padding: 0x1F at the current position, 0x80 in the last rate byte
+    // SHAKE padding: 0x1F at the current (word-aligned) position, 0x80 in the
+    // last rate byte
 #pragma unroll
-    for (int i = 0; i < kShakeRate / 8; i++)
-      if ((uint32_t)i == (pos >> 3)) c[i] ^= (uint64_t)0x1F << (8 * (pos & 7));
+    for (int i = 0; i < kShakeRate / 8; i++) c[i] ^= (uint64_t)0x1F & (0 - (uint64_t)((uint32_t)i == lane));
     c[(kShakeRate - 1) / 8] ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) % 8));
     keccak_f1600(c);
     // component i = little-endian u128 of squeezed bytes 16i..16i+15
     out[0] = gl::reduce128(c[0], c[1]);
-    out[1] = gl::reduce128(c[2], c[3]);
-    out[2] = gl::reduce128(c[4], c[5]);
-    for (int i = ext; i < 3; i++) out[i] = 0;
+    out[1] = ext > 1 ? gl::reduce128(c[2], c[3]) : 0;
+    out[2] = ext > 2 ? gl::reduce128(c[4], c[5]) : 0;
   }
 };
 
-// Thread 0 only: writes the K round values (lifted to ext words) to `poly`,
-// absorbs them, squeezes the challenge into chal_out and chal_cur.
+// Called by all 64 lanes of one wave; lane 0 holds the K round values.
+// Writes them (lifted to ext words) to `poly`, absorbs them, squeezes the
+// challenge into chal_out and chal_cur; lane 0 does the stores.
 template <class F, int K>
 __device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64_t* poly, uint64_t* chal_out,
                                            uint64_t* chal_cur) {
+  const bool lane0 = (threadIdx.x & 63) == 0;
   RegFs fs;
   fs.load(t);
 #pragma unroll
   for (int i = 0; i < K; i++) {
     uint64_t wd[3] = {0, 0, 0};
     gl::store_words(s[i], wd);
-    for (int c = 0; c < ext; c++) poly[i * ext + c] = wd[c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) wd[c] = uniform64(wd[c]);
+    if (lane0)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        if (c < ext) poly[i * ext + c] = wd[c];
     fs.send(ext, wd);
   }
   uint64_t ch[3];
   fs.challenge(ext, ch);
-  for (int c = 0; c < ext; c++) {
-    chal_out[c] = ch[c];
-    chal_cur[c] = ch[c];
+  if (lane0) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (c < ext) chal_out[c] = ch[c];
+      chal_cur[c] = ch[c];
+    }
+    fs.store(t);
   }
-  fs.store(t);
 }
 
 // Cross-block sum of one round plus the transcript step.
@@ -124,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_fs(Reduce r, unsigned nblocks
     for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * K + i) * F::D));
   }
   block_reduce<F, K>(s);
-  if (threadIdx.x == 0) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
+  if (threadIdx.x < 64) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
 }
 
 constexpr int kTailBlock = 1024;
@@ -195,7 +213,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(Views<FI, K> in, Views<F, K
       }
     }
     block_reduce<F, K>(acc);
-    if (tid == 0) emit_round<F, K>(acc, ext, t, polys + (size_t)r * K * ext, chals + (size_t)r * ext, chal_sh);
+    if (tid < 64) emit_round<F, K>(acc, ext, t, polys + (size_t)r * K * ext, chals + (size_t)r * ext, chal_sh);
     __syncthreads();
     have_chal = 1;
   }

@@ -1,14 +1,14 @@
 // Keccak-f[1600] (FIPS 202) and the SHAKE256-based Fiat-Shamir transcript,
 // usable from host and device code.
 //
-// Transcript byte stream (the reference's transcript is the external
-// spongefish crate, src/transcript/spongefish.rs:17-44, whose wire encoding is
-// not part of the reference; this library fixes its own):
-//   "effsc-hip/fs/v1" || u64le(len(domsep)) || domsep
-//   send(x)      : 0x01 || u64le(canonical component) for each component
-//   challenge(d) : SHAKE256(stream || 0x02) -> d x 16 bytes, each read as a
-//                  little-endian u128 and reduced mod p; then 0x02 is
-//                  appended to the stream.
+// Transcript stream (the reference's transcript is the external spongefish
+// crate, src/transcript/spongefish.rs:17-44, whose wire encoding is not part
+// of the reference; this library fixes its own).  Every item is a whole number
+// of little-endian u64 words, so the sponge absorbs whole lanes:
+//   "effsc-hip/fs/v2\0" (2 words) || len(domsep) || domsep zero-padded to 8 bytes
+//   send(x)      : word 1, then each canonical component
+//   challenge(d) : word 2 is appended; the first 16 d bytes of
+//                  SHAKE256(stream) give d little-endian u128 values mod p.
 // The sponge is kept incrementally, so each operation costs O(1) permutations.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -103,21 +103,30 @@ SKHD uint64_t u128_mod_p(const uint8_t* b) {
 struct FsTranscript {
   Sponge sp;
 
+  SKHD void absorb_word(uint64_t x) {
+    sp.st[sp.pos >> 3] ^= x;
+    sp.pos += 8;
+    if (sp.pos == kShakeRate) {
+      keccak_f1600(sp.st);
+      sp.pos = 0;
+    }
+  }
   SKHD void init(const uint8_t* domsep, size_t len) {
-    const char tag[] = "effsc-hip/fs/v1";
+    const char tag[16] = "effsc-hip/fs/v2";
     sp.init();
-    sp.absorb(reinterpret_cast<const uint8_t*>(tag), sizeof(tag) - 1);
+    sp.absorb(reinterpret_cast<const uint8_t*>(tag), 16);
     sp.absorb_u64((uint64_t)len);
     sp.absorb(domsep, len);
+    for (size_t i = len; i % 8; i++) sp.absorb_byte(0);
   }
   SKHD void send(int d, const uint64_t* v) {
-    sp.absorb_byte(0x01);
-    for (int i = 0; i < d; i++) sp.absorb_u64(gl::canon(v[i]));
+    absorb_word(1);
+    for (int i = 0; i < d; i++) absorb_word(gl::canon(v[i]));
   }
   SKHD void challenge(int d, uint64_t* out) {
-    sp.absorb_byte(0x02);  // the stream grows by 0x02 ...
+    absorb_word(2);
     uint8_t buf[48];
-    sp.squeeze_copy(buf, 16 * d);  // ... and the squeeze reads stream || 0x02
+    sp.squeeze_copy(buf, 16 * d);
     for (int i = 0; i < d; i++) out[i] = u128_mod_p(buf + 16 * i);
   }
 };

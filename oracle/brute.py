@@ -87,21 +87,26 @@ def uniform_table(n, seed):
 
 
 # ----------------------------------------------------------------- transcripts
-FS_TAG = b"effsc-hip/fs/v1"
+FS_TAG = b"effsc-hip/fs/v2\x00"  # 16 bytes
+
+
+def _pad8(bs):
+    return bs + b"\x00" * (-len(bs) % 8)
 
 
 class ShakeTranscript:
-    """Byte-stream definition of the Fiat-Shamir transcript."""
+    """Byte-stream definition of the Fiat-Shamir transcript; every item is a
+    whole number of little-endian 64-bit words."""
 
     def __init__(self, domsep: bytes):
-        self.stream = bytearray(FS_TAG + len(domsep).to_bytes(8, "little") + domsep)
+        self.stream = bytearray(FS_TAG + len(domsep).to_bytes(8, "little") + _pad8(domsep))
 
     def send(self, value):
-        self.stream += b"\x01" + b"".join((x % P).to_bytes(8, "little") for x in value)
+        self.stream += (1).to_bytes(8, "little") + b"".join((x % P).to_bytes(8, "little") for x in value)
 
     def challenge(self, d):
-        out = hashlib.shake_256(bytes(self.stream) + b"\x02").digest(16 * d)
-        self.stream += b"\x02"
+        self.stream += (2).to_bytes(8, "little")
+        out = hashlib.shake_256(bytes(self.stream)).digest(16 * d)
         return tuple(int.from_bytes(out[16 * i:16 * i + 16], "little") % P for i in range(d))
 
 

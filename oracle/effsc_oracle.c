@@ -279,17 +279,22 @@ struct or_transcript {
     sponge sp;
     uint64_t rng;
 };
-static const char FS_TAG[] = "effsc-hip/fs/v1";
+static const char FS_TAG[16] = "effsc-hip/fs/v2"; /* 15 chars + NUL = 16 bytes */
+
+static void sp_absorb_u64(sponge *s, uint64_t x) {
+    uint8_t b[8];
+    for (int j = 0; j < 8; j++) b[j] = (uint8_t)(x >> (8 * j));
+    sp_absorb(s, b, 8);
+}
 
 or_transcript *or_tr_new_shake(const uint8_t *domsep, size_t len) {
     or_transcript *t = (or_transcript *)calloc(1, sizeof *t);
     t->kind = 0;
     sp_init(&t->sp);
-    sp_absorb(&t->sp, (const uint8_t *)FS_TAG, sizeof(FS_TAG) - 1);
-    uint8_t l8[8];
-    for (int i = 0; i < 8; i++) l8[i] = (uint8_t)((uint64_t)len >> (8 * i));
-    sp_absorb(&t->sp, l8, 8);
+    sp_absorb(&t->sp, (const uint8_t *)FS_TAG, 16);
+    sp_absorb_u64(&t->sp, (uint64_t)len);
     sp_absorb(&t->sp, domsep, len);
+    for (size_t i = len; i % 8; i++) sp_absorb_byte(&t->sp, 0);
     return t;
 }
 or_transcript *or_tr_new_rng(uint64_t seed) {
@@ -304,13 +309,8 @@ static uint64_t splitmix_next(uint64_t *s) {
 }
 void or_tr_send(or_transcript *t, int d, const uint64_t *value) {
     if (t->kind != 0) return;
-    sp_absorb_byte(&t->sp, 0x01);
-    for (int i = 0; i < d; i++) {
-        uint8_t b[8];
-        uint64_t x = value[i] % P;
-        for (int j = 0; j < 8; j++) b[j] = (uint8_t)(x >> (8 * j));
-        sp_absorb(&t->sp, b, 8);
-    }
+    sp_absorb_u64(&t->sp, 1);
+    for (int i = 0; i < d; i++) sp_absorb_u64(&t->sp, value[i] % P);
 }
 void or_tr_challenge(or_transcript *t, int d, uint64_t *out) {
     if (t->kind == 1) {
@@ -318,8 +318,8 @@ void or_tr_challenge(or_transcript *t, int d, uint64_t *out) {
         for (int i = 1; i < d; i++) out[i] = 0;
         return;
     }
+    sp_absorb_u64(&t->sp, 2);
     sponge c = t->sp;
-    sp_absorb_byte(&c, 0x02);
     uint8_t buf[48];
     sp_finish(&c, buf, (size_t)16 * d);
     for (int i = 0; i < d; i++) {
@@ -328,7 +328,6 @@ void or_tr_challenge(or_transcript *t, int d, uint64_t *out) {
         for (int j = 0; j < 8; j++) hi |= (uint64_t)buf[16 * i + 8 + j] << (8 * j);
         out[i] = (uint64_t)((((u128)hi << 64) | lo) % P);
     }
-    sp_absorb_byte(&t->sp, 0x02);
 }
 void or_tr_free(or_transcript *t) { free(t); }
 
