@@ -11,6 +11,7 @@
 
 #include "effsc_hip.h"
 #include "kernels.hpp"
+#include "pass2.hpp"
 #include "prove.hpp"
 #include "shake.hpp"
 
@@ -97,13 +98,23 @@ struct effsc_hip_prover {
   uint64_t* d_fin = nullptr;    // 3 words
   uint64_t* h_proof = nullptr;  // pinned staging: polys | chals | fin | fs
   // Kernel timing (effsc_hip_profile): HIP events around each launch on the
-  // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold.
+  // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold,
+  // 3 two-round pass.
   bool prof_on = false;
   std::vector<hipEvent_t> ev;             // pairs (2i, 2i+1)
   std::vector<std::pair<int, double>> pend;  // (slot, bytes) of recorded pairs
-  uint64_t prof_launches[3] = {0, 0, 0};
-  double prof_ms[3] = {0, 0, 0};
-  double prof_bytes[3] = {0, 0, 0};
+  uint64_t prof_launches[4] = {0, 0, 0, 0};
+  double prof_ms[4] = {0, 0, 0, 0};
+  double prof_bytes[4] = {0, 0, 0, 0};
+  // Two-round passes (base-field ML / IP): the logical table is the
+  // materialised one (len) folded by the npend challenges in fpend; when
+  // have_next, the next round's values are the polynomials in next_coef
+  // (pass layout of pass2.hpp) evaluated at the next challenge.
+  bool twostep = false;
+  int npend = 0;
+  uint64_t fpend[2] = {0, 0};
+  bool have_next = false;
+  uint64_t next_coef[kMaxK + kMaxK * (kMaxK + 1)] = {};
 };
 
 struct effsc_hip_transcript {
@@ -258,12 +269,12 @@ struct DevOps {
     return EFFSC_HIP_OK;
   }
   static int tail(effsc_hip_prover* h, uint64_t* const* src, size_t sst, bool from_input, size_t L, int rounds,
-                  bool have, size_t first_round) {
+                  bool have, size_t first_round, const uint64_t* cur) {
     Views<FI, K> in = views<FI, K>(src, sst);
     Views<F, K> work = views<F, K>(h->work, h->work_stride);
     hipLaunchKernelGGL((k_tail<FI, F, K>), dim3(1), dim3(kTailBlock), 0, h->stream, in, work, from_input ? 1 : 0, L,
                        rounds, have ? 1 : 0, h->d_fs, h->ext, h->d_polys + first_round * K * h->ext,
-                       h->d_chals + first_round * h->ext, h->d_cur, h->d_fin, h->w);
+                       h->d_chals + first_round * h->ext, const_cast<uint64_t*>(cur), h->d_fin, h->w);
     HIPCHK(hipGetLastError());
     return EFFSC_HIP_OK;
   }
@@ -338,6 +349,18 @@ int fetch_result(effsc_hip_prover* h, int fi, uint64_t* out_vals, int count) {
   return EFFSC_HIP_OK;
 }
 
+int pass_value_count(const effsc_hip_prover* h) { return h->K + h->K * (h->K + 1); }
+
+// Reduced pass values (pass2.hpp layout) -> host.
+int fetch_pass(effsc_hip_prover* h, uint64_t* vals) {
+  int n = pass_value_count(h);
+  HIPCHK(hipMemcpyAsync(h->h_res, h->red.result, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
+  std::copy(h->h_res, h->h_res + n, vals);
+  return EFFSC_HIP_OK;
+}
+
 // Fold the current tables with challenge c (ext words).  If `fuse` and the
 // length allows, also evaluates the next round polynomial into red.result
 // and sets *evaluated.
@@ -403,8 +426,104 @@ int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
 int eq_finalize(effsc_hip_prover* h, const uint64_t* chal);
 int eq_final_value(effsc_hip_prover* h, uint64_t* out);
 
+// ---------------------------------------------------------- two-round passes
+template <int K, int P>
+int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
+  uint64_t* const* src = src_planes(h);
+  size_t sst = src_stride(h);
+  Views<Fp, K> in = views<Fp, K>(src, sst);
+  Views<Fp, K> out = views<Fp, K>(h->work, h->work_stride);
+  const size_t L = 4 * q;
+  bool vec = (q % 2 == 0) && (sst % 2 == 0) && (h->work_stride % 2 == 0);
+  unsigned g = blocks_for(h, vec ? q / 2 : q);
+  TRY(prof_begin(h));
+  if (vec)
+    hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
+                       q, h->red);
+  else
+    hipLaunchKernelGGL((k_pass<K, P, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
+                       q, h->red);
+  HIPCHK(hipGetLastError());
+  double bytes = 8.0 * K * ((double)(L << P) + (P > 0 ? (double)L : 0.0));
+  TRY(prof_end(h, 3, bytes));
+  h->last_grid = g;
+  return EFFSC_HIP_OK;
+}
+
+// One pass over the current tables with p pending folds (from fpend, or from
+// device memory at cptr); leaves the reduced pass values in red.result.
+int launch_pass(effsc_hip_prover* h, int p, size_t q, const uint64_t* cptr) {
+  auto go = [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    switch (p) {
+      case 0: return launch_pass_kp<K, 0>(h, q, cptr);
+      case 1: return launch_pass_kp<K, 1>(h, q, cptr);
+      case 2: return launch_pass_kp<K, 2>(h, q, cptr);
+    }
+    return fail(EFFSC_HIP_EINVAL, "bad pending fold count");
+  };
+  if (h->K == 1) return go(std::integral_constant<int, 1>{});
+  if (h->K == 2) return go(std::integral_constant<int, 2>{});
+  return fail(EFFSC_HIP_EINVAL, "two-round pass needs K <= 2");
+}
+
+// Applies the pending folds to the materialised tables.
+int materialize(effsc_hip_prover* h) {
+  for (int i = 0; i < h->npend; i++) {
+    bool ev = false;
+    uint64_t c = h->fpend[i];
+    TRY(fold_step(h, &c, false, &ev));
+  }
+  h->npend = 0;
+  return EFFSC_HIP_OK;
+}
+
+size_t logical_len(const effsc_hip_prover* h) { return h->len >> h->npend; }
+
+int round_twostep(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
+  const int K = h->K;
+  if (chal && h->have_next) {
+    // round s+1 from the coefficients of the previous pass
+    for (int i = 0; i < K; i++)
+      out[i] = K == 1 ? pass_eval_next<1>(h->next_coef, i, chal[0]) : pass_eval_next<2>(h->next_coef, i, chal[0]);
+    h->fpend[h->npend++] = chal[0];
+    h->have_next = false;
+    return EFFSC_HIP_OK;
+  }
+  h->have_next = false;
+  if (chal) {
+    if (h->npend == 2) TRY(materialize(h));
+    h->fpend[h->npend++] = chal[0];
+  }
+  size_t M = h->len;
+  int p = h->npend;
+  if (is_pow2(M) && (M >> p) >= 4) {
+    size_t L = M >> p;
+    TRY(launch_pass(h, p, L / 4, nullptr));
+    TRY((K == 1 ? reduce<Fp, pass_values<1>()>(h, h->last_grid) : reduce<Fp, pass_values<2>()>(h, h->last_grid)));
+    uint64_t vals[kMaxK + kMaxK * (kMaxK + 1)];
+    TRY(fetch_pass(h, vals));
+    if (p > 0) {
+      h->len = L;
+      h->in_work = true;
+      h->npend = 0;
+    }
+    std::copy(vals, vals + pass_value_count(h), h->next_coef);
+    h->have_next = true;
+    std::copy(vals, vals + K, out);
+    return EFFSC_HIP_OK;
+  }
+  // small or non-power-of-two tables: single-round path
+  TRY(materialize(h));
+  uint64_t vals[kMaxK * 3];
+  TRY(evaluate(h, false, vals));
+  to_wire(h, vals, out);
+  return EFFSC_HIP_OK;
+}
+
 int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   if (h->kind == K_EQ) return eq_round(h, chal, out);
+  if (h->twostep) return round_twostep(h, chal, out);
   bool evaluated = false;
   if (chal) TRY(fold_step(h, chal, true, &evaluated));
   uint64_t vals[kMaxK * 3];
@@ -415,12 +534,15 @@ int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
 
 int finalize_impl(effsc_hip_prover* h, const uint64_t* chal) {
   if (h->kind == K_EQ) return eq_finalize(h, chal);
+  h->have_next = false;
+  TRY(materialize(h));
   bool evaluated = false;
   return fold_step(h, chal, false, &evaluated);
 }
 
 int final_value_impl(effsc_hip_prover* h, uint64_t* out) {
   if (h->kind == K_EQ) return eq_final_value(h, out);
+  TRY(materialize(h));
   const int E = h->ext;
   std::fill(out, out + E, 0);
   if (h->len != 1) return EFFSC_HIP_OK;
@@ -557,6 +679,7 @@ int new_product_like(const effsc_hip_config* cfg, int kind, int K, int deg, cons
   h->w = cfg->ext == 3 ? gl::canon(cfg->ext_w) : 0;
   h->n = n;
   h->len = n;
+  h->twostep = (kind == K_ML || kind == K_IP) && h->ext == 1 && h->in_ext == 1 && !std::getenv("EFFSC_HIP_ONE_ROUND");
   int rc = setup_common(h, next_pow2(std::max<size_t>(n, 1)) >> 1);
   if (rc == EFFSC_HIP_OK) rc = import_tables(h, cfg, tables, n);
   if (rc != EFFSC_HIP_OK) {
@@ -846,52 +969,94 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   hs->rng = t->rng;
   HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
 
-  size_t L = h->len, r = 0;
-  bool have = false, tail_used = false;
-  while (r < num_rounds) {
-    if (L <= tail_max()) break;
-    int fi = src_field(h);
+  // host-side pending folds are applied first; from here on the pending
+  // challenges live on the device (d_cur[0..p))
+  TRY(materialize(h));
+  h->have_next = false;
+  size_t r = 0;
+  int p = 0;
+  bool tail_used = false;
+  if (h->twostep) {
+    // two rounds per pass while the tables are large
+    while (num_rounds - r >= 2) {
+      size_t L = h->len >> p;
+      if (L <= tail_max() || L < 4) break;
+      TRY(launch_pass(h, p, L / 4, p ? h->d_cur : nullptr));
+      if (p > 0) {
+        h->len = L;
+        h->in_work = true;
+      }
+      unsigned g = h->last_grid;
+      if (K == 1)
+        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs,
+                           h->d_polys + r, h->d_chals + r, h->d_cur);
+      else
+        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs,
+                           h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
+      HIPCHK(hipGetLastError());
+      r += 2;
+      p = 2;
+    }
+  } else {
+    while (r < num_rounds) {
+      size_t M = h->len;
+      if (M <= tail_max()) break;
+      int fi = src_field(h);
+      uint64_t* const* src = src_planes(h);
+      size_t sst = src_stride(h);
+      int fr;  // field of the reduced values
+      if (p == 1) {
+        TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fused(h, src, sst, nullptr, M / 4, h->d_cur); }));
+        h->len = M / 2;
+        h->in_work = true;
+        fr = E;
+      } else {
+        TRY(dispatch(fi, fi, K, [&](auto op) { return decltype(op)::eval(h, src, sst, M / 2, M / 2); }));
+        fr = fi;
+      }
+      unsigned g = h->last_grid;
+      TRY(dispatch<DevOps>(fr, fr, K, [&](auto op) { return decltype(op)::reduce_fs(h, g, r); }));
+      r++;
+      p = 1;
+    }
+  }
+  const uint64_t* cur = h->d_cur;
+  if (p == 2) {
+    // apply the older of two pending challenges with a multi-block fold
+    size_t M = h->len;
     uint64_t* const* src = src_planes(h);
     size_t sst = src_stride(h);
-    int fr;  // field of the reduced values
-    if (have) {
-      TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fused(h, src, sst, nullptr, L / 4, h->d_cur); }));
-      L /= 2;
-      h->in_work = true;
-      fr = E;
-    } else {
-      TRY(dispatch(fi, fi, K, [&](auto op) { return decltype(op)::eval(h, src, sst, L / 2, L / 2); }));
-      fr = fi;
-    }
-    unsigned g = h->last_grid;
-    TRY(dispatch<DevOps>(fr, fr, K, [&](auto op) { return decltype(op)::reduce_fs(h, g, r); }));
-    r++;
-    have = true;
+    TRY(dispatch(1, 1, K, [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, M / 2, M / 2, h->d_cur); }));
+    h->len = M / 2;
+    h->in_work = true;
+    p = 1;
+    cur = h->d_cur + 1;
   }
-  if (r < num_rounds || L <= tail_max()) {
+  size_t rounds_left = num_rounds - r;
+  size_t M = h->len;
+  const bool have = p == 1;
+  if (rounds_left > 0 || M <= tail_max()) {
     // remaining rounds, finalize and final value in one workgroup
     int fi = src_field(h);
     bool from_input = !h->in_work;
     TRY(dispatch<DevOps>(fi, E, K, [&](auto op) {
-      return decltype(op)::tail(h, src_planes(h), src_stride(h), from_input, L, (int)(num_rounds - r), have, r);
+      return decltype(op)::tail(h, src_planes(h), src_stride(h), from_input, M, (int)rounds_left, have, r, cur);
     }));
     // folds inside the tail: one per round that has a challenge, then the
     // final fold with the last challenge
-    size_t rounds_left = num_rounds - r;
     size_t folds = rounds_left == 0 ? 0 : (have ? rounds_left : rounds_left - 1);
-    size_t Lt = L >> folds;
+    size_t Lt = M >> folds;
     if ((have || rounds_left > 0) && Lt >= 2) Lt /= 2;
     h->len = Lt;
     h->in_work = true;
     tail_used = true;
-  } else {
+  } else if (have) {
     // partial run ending on a large table: finalize with the multi-block fold
-    size_t half = L / 2;
     int fi = src_field(h);
     uint64_t* const* src = src_planes(h);
     size_t sst = src_stride(h);
-    TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, half, half, h->d_cur); }));
-    h->len = half;
+    TRY(dispatch(fi, E, K, [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, M / 2, M / 2, cur); }));
+    h->len = M / 2;
     h->in_work = true;
   }
   // proof and transcript state -> host
@@ -1086,7 +1251,7 @@ int effsc_hip_table_len(const effsc_hip_prover* h, size_t table, size_t* out) {
     return EFFSC_HIP_OK;
   }
   if (table >= (size_t)h->K) return fail(EFFSC_HIP_EINVAL, "table index out of range");
-  *out = h->len;
+  *out = logical_len(h);
   return EFFSC_HIP_OK;
 }
 
@@ -1099,6 +1264,7 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
   if (!dst) return cap_words == 0 ? EFFSC_HIP_OK : fail(EFFSC_HIP_EINVAL, "null destination");
   if (cap_words < L * E) return fail(EFFSC_HIP_EINVAL, "destination too small");
   HIPCHK(hipSetDevice(h->dev));
+  TRY(materialize(h));
   const uint64_t* planes;
   size_t stride;
   int d;
@@ -1128,6 +1294,8 @@ int effsc_hip_reset(effsc_hip_prover* h) {
   if (h->kind == K_EQ) return fail(EFFSC_HIP_ESTATE, "eq-factored prover folds its eq tables in place; rebuild it");
   h->len = h->n;
   h->in_work = false;
+  h->npend = 0;
+  h->have_next = false;
   return EFFSC_HIP_OK;
 }
 
@@ -1138,7 +1306,7 @@ int effsc_hip_profile(effsc_hip_prover* h, int enable) {
   HIPCHK(hipSetDevice(h->dev));
   h->prof_on = enable != 0;
   h->pend.clear();
-  for (int i = 0; i < 3; i++) {
+  for (int i = 0; i < 4; i++) {
     h->prof_launches[i] = 0;
     h->prof_ms[i] = 0;
     h->prof_bytes[i] = 0;
@@ -1148,7 +1316,7 @@ int effsc_hip_profile(effsc_hip_prover* h, int enable) {
 
 int effsc_hip_profile_read(const effsc_hip_prover* h, uint64_t* launches, double* ms, double* bytes) {
   if (!h || !launches || !ms || !bytes) return fail(EFFSC_HIP_EINVAL, "null argument");
-  for (int i = 0; i < 3; i++) {
+  for (int i = 0; i < 4; i++) {
     launches[i] = h->prof_launches[i];
     ms[i] = h->prof_ms[i];
     bytes[i] = h->prof_bytes[i];

@@ -1,0 +1,209 @@
+// Two-round passes for the base-field multilinear (K = 1) and inner-product
+// (K = 2) provers.
+//
+// The round-(s+1) polynomial is, per pair, a product of K functions that are
+// linear in the round-s challenge x, so its EvalsInfty values are polynomials
+// of degree K in x.  One pass over the level-s table therefore yields round s's
+// values and the K+1 coefficients of each round-(s+1) value; once the
+// transcript produces x the host (or k_reduce_fs2) evaluates them.  The next
+// pass then folds twice on the fly (with r_s and r_{s+1}) and materialises
+// only every other level.  For the v = 28 inner product that is 11.4 GB of
+// HBM traffic instead of the 17.2 GB of one fold per round
+// (src/inner_product_sumcheck.rs:199-293 reads and writes every level), and
+// round 1 becomes a pure read pass.  All outputs are the same field elements.
+//
+// Geometry of one pass: the materialised table has length M; p in {0,1,2}
+// pending challenges fold it to the level-L table, L = M / 2^p, without
+// storing intermediate levels.  Unit u in [0, L/4) owns the level-L elements
+// E_j = T_L[u + j L/4], j = 0..3; each needs level-M elements at
+// u + (j + 4 t) L/4, t < 2^p.  Round s pairs (E_0, E_2) and (E_1, E_3);
+// round s+1 pairs (E_0 + x (E_2 - E_0), E_1 + x (E_3 - E_1)).
+// Position u + m L/4 is read and written by unit u only, so the pass may run
+// in place.
+#pragma once
+#include "kernels.hpp"
+
+namespace effsc_hip {
+
+// Unreduced sum of products: value = lo + 2^64 hi + 2^128 top (< 2^160).
+struct Acc128 {
+  uint64_t lo, hi;
+  uint32_t top;
+  __device__ __forceinline__ void zero() {
+    lo = hi = 0;
+    top = 0;
+  }
+  __device__ __forceinline__ void add128(uint64_t plo, uint64_t phi) {
+    uint64_t s = lo + plo;
+    uint64_t c1 = s < plo;
+    lo = s;
+    uint64_t h = hi + phi;
+    uint32_t c2 = h < phi;
+    uint64_t h2 = h + c1;
+    c2 += h2 < h;
+    hi = h2;
+    top += c2;
+  }
+  __device__ __forceinline__ void mac(uint64_t x, uint64_t y) {
+    uint64_t plo, phi;
+    gl::mul128(x, y, plo, phi);
+    add128(plo, phi);
+  }
+  __device__ __forceinline__ void add(uint64_t x) { add128(x, 0); }
+  // canonical residue: 2^128 = -2^32 (mod p)
+  __device__ __forceinline__ Fp reduce() const {
+    uint64_t r = gl::reduce128(lo, hi);
+    return {gl::add(r, gl::mul((uint64_t)top, 0xFFFFFFFE00000001ull))};
+  }
+};
+
+// Number of values a pass produces: K for round s, K (K+1) coefficients for
+// round s+1 (value i, coefficient j at K + i (K+1) + j).
+template <int K>
+constexpr int pass_values() {
+  return K + K * (K + 1);
+}
+
+template <int K>
+struct PassAcc {
+  Acc128 a[pass_values<K>()];
+};
+
+// Accumulates one unit (level-L elements E[i][j], table i, j = 0..3).
+template <int K>
+__device__ __forceinline__ void pass_accumulate(const uint64_t (*E)[4], PassAcc<K>& acc) {
+  if constexpr (K == 1) {
+    // round s: h(0) = sum of the low halves
+    acc.a[0].add(E[0][0]);
+    acc.a[0].add(E[0][1]);
+    // round s+1: s0(x) = E0 + x (E2 - E0)
+    acc.a[1].add(E[0][0]);
+    acc.a[2].add(gl::sub(E[0][2], E[0][0]));
+  } else {
+    const uint64_t *A = E[0], *B = E[1];
+    uint64_t dA0 = gl::sub(A[2], A[0]), dA1 = gl::sub(A[3], A[1]);
+    uint64_t dB0 = gl::sub(B[2], B[0]), dB1 = gl::sub(B[3], B[1]);
+    // round s: q(0) = sum lo_a lo_b, q(inf) = sum d_a d_b over both pairs
+    acc.a[0].mac(A[0], B[0]);
+    acc.a[0].mac(A[1], B[1]);
+    acc.a[1].mac(dA0, dB0);
+    acc.a[1].mac(dA1, dB1);
+    // round s+1, q(0)(x) = (A0 + x dA0)(B0 + x dB0)
+    acc.a[2].mac(A[0], B[0]);
+    acc.a[3].mac(A[0], dB0);
+    acc.a[3].mac(dA0, B[0]);
+    acc.a[4].mac(dA0, dB0);
+    // q(inf)(x) = (eA + x fA)(eB + x fB), e = E1 - E0, f = dA1 - dA0
+    uint64_t eA = gl::sub(A[1], A[0]), eB = gl::sub(B[1], B[0]);
+    uint64_t fA = gl::sub(dA1, dA0), fB = gl::sub(dB1, dB0);
+    acc.a[5].mac(eA, eB);
+    acc.a[6].mac(eA, fB);
+    acc.a[6].mac(fA, eB);
+    acc.a[7].mac(fA, fB);
+  }
+}
+
+// Level-L element at index e from the level-M table with P pending folds.
+template <int P>
+__device__ __forceinline__ uint64_t fold_chain(const uint64_t* src, size_t e, size_t L, uint64_t c1, uint64_t c2) {
+  if constexpr (P == 0) {
+    return src[e];
+  } else if constexpr (P == 1) {
+    uint64_t x0 = src[e], x1 = src[e + L];
+    return gl::add(x0, gl::mul(c1, gl::sub(x1, x0)));
+  } else {
+    uint64_t x0 = src[e], x1 = src[e + L], x2 = src[e + 2 * L], x3 = src[e + 3 * L];
+    uint64_t a = gl::add(x0, gl::mul(c1, gl::sub(x2, x0)));
+    uint64_t b = gl::add(x1, gl::mul(c1, gl::sub(x3, x1)));
+    return gl::add(a, gl::mul(c2, gl::sub(b, a)));
+  }
+}
+
+// Vector form: elements e and e+1 (e even, 16-byte aligned planes).
+template <int P, bool NT>
+__device__ __forceinline__ void fold_chain2(const uint64_t* src, size_t e, size_t L, uint64_t c1, uint64_t c2,
+                                            uint64_t& o0, uint64_t& o1) {
+  auto ld = [&](size_t i) -> v2u64 {
+    if constexpr (NT)
+      return __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(src + i));
+    else
+      return *reinterpret_cast<const v2u64*>(src + i);
+  };
+  if constexpr (P == 0) {
+    v2u64 x = ld(e);
+    o0 = x.x;
+    o1 = x.y;
+  } else if constexpr (P == 1) {
+    v2u64 x0 = ld(e), x1 = ld(e + L);
+    o0 = gl::add(x0.x, gl::mul(c1, gl::sub(x1.x, x0.x)));
+    o1 = gl::add(x0.y, gl::mul(c1, gl::sub(x1.y, x0.y)));
+  } else {
+    v2u64 x0 = ld(e), x1 = ld(e + L), x2 = ld(e + 2 * L), x3 = ld(e + 3 * L);
+    uint64_t a0 = gl::add(x0.x, gl::mul(c1, gl::sub(x2.x, x0.x)));
+    uint64_t a1 = gl::add(x0.y, gl::mul(c1, gl::sub(x2.y, x0.y)));
+    uint64_t b0 = gl::add(x1.x, gl::mul(c1, gl::sub(x3.x, x1.x)));
+    uint64_t b1 = gl::add(x1.y, gl::mul(c1, gl::sub(x3.y, x1.y)));
+    o0 = gl::add(a0, gl::mul(c2, gl::sub(b0, a0)));
+    o1 = gl::add(a1, gl::mul(c2, gl::sub(b1, a1)));
+  }
+}
+
+// One pass: P pending folds from the level-M tables `src` to level L = 4 q,
+// written to `dst` when P > 0; accumulates the pass values.  Challenges come
+// from `cptr` (device words c1, c2) when non-null, else from c1, c2.
+template <int K, int P, int VEC>
+__global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> dst, uint64_t c1, uint64_t c2,
+                                                 const uint64_t* cptr, size_t q, Reduce r) {
+  if (cptr) {
+    c1 = cptr[0];
+    c2 = cptr[1];
+  }
+  const size_t L = 4 * q;
+  PassAcc<K> acc;
+#pragma unroll
+  for (int i = 0; i < pass_values<K>(); i++) acc.a[i].zero();
+  const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
+  for (size_t u = gtid() * VEC; u < q; u += step) {
+    uint64_t E0[K][4], E1[K][4];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if constexpr (VEC == 2) {
+          // a pass without folds reads each element exactly once: stream it
+          fold_chain2<P, P == 0>(src.t[i].p, u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+        } else {
+          E0[i][j] = fold_chain<P>(src.t[i].p, u + j * q, L, c1, c2);
+        }
+      }
+    }
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if constexpr (VEC == 2)
+            *reinterpret_cast<ulonglong2*>(dst.t[i].p + u + j * q) = make_ulonglong2(E0[i][j], E1[i][j]);
+          else
+            dst.t[i].p[u + j * q] = E0[i][j];
+        }
+    }
+    pass_accumulate<K>(E0, acc);
+    if constexpr (VEC == 2) pass_accumulate<K>(E1, acc);
+  }
+  Fp vals[pass_values<K>()];
+#pragma unroll
+  for (int i = 0; i < pass_values<K>(); i++) vals[i] = acc.a[i].reduce();
+  finish<Fp, pass_values<K>()>(vals, r);
+}
+
+// Round-(s+1) value i at x from its K+1 coefficients.
+template <int K>
+__host__ __device__ __forceinline__ uint64_t pass_eval_next(const uint64_t* coeffs, int i, uint64_t x) {
+  const uint64_t* c = coeffs + K + i * (K + 1);
+  uint64_t v = c[K];
+  for (int j = K - 1; j >= 0; j--) v = gl::add(gl::mul(v, x), c[j]);
+  return v;
+}
+
+}  // namespace effsc_hip

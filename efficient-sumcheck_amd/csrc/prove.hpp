@@ -12,6 +12,7 @@
 // and the final value in a single workgroup (tables stay in L2).
 #pragma once
 #include "kernels.hpp"
+#include "pass2.hpp"
 #include "shake.hpp"
 
 namespace effsc_hip {
@@ -143,6 +144,54 @@ __global__ __launch_bounds__(kBlock) void k_reduce_fs(Reduce r, unsigned nblocks
   }
   block_reduce<F, K>(s);
   if (threadIdx.x < 64) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
+}
+
+// Two-round pass (pass2.hpp) over base-field tables: sums the pass values,
+// then runs two transcript steps: round s values, challenge x, round s+1
+// values (the pass polynomials at x), challenge y.  cur receives (x, y), the
+// pending folds of the next pass.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
+                                                       uint64_t* chals, uint64_t* cur) {
+  constexpr int NV = pass_values<K>();
+  Fp s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) s[i] = gl::zero<Fp>();
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) s[i] = gl::add(s[i], gl::load_words<Fp>(r.partials + (size_t)b * NV + i));
+  }
+  block_reduce<Fp, NV>(s);
+  if (threadIdx.x >= 64) return;
+  const bool lane0 = threadIdx.x == 0;
+  uint64_t v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) v[i] = uniform64(s[i].v);
+  RegFs fs;
+  fs.load(t);
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t w[3] = {v[i], 0, 0};
+    if (lane0) poly[i] = v[i];
+    fs.send(1, w);
+  }
+  uint64_t x[3];
+  fs.challenge(1, x);
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t w[3] = {pass_eval_next<K>(v, i, x[0]), 0, 0};
+    if (lane0) poly[K + i] = w[0];
+    fs.send(1, w);
+  }
+  uint64_t y[3];
+  fs.challenge(1, y);
+  if (lane0) {
+    chals[0] = x[0];
+    chals[1] = y[0];
+    cur[0] = x[0];
+    cur[1] = y[0];
+    fs.store(t);
+  }
 }
 
 constexpr int kTailBlock = 1024;

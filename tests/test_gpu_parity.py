@@ -329,3 +329,41 @@ def test_device_loop_partial_then_continue(tail, monkeypatch):
         sumcheck(g2, r, t2)
     assert np.array_equal(g2.table_words(0), o.table(0))
     assert np.array_equal(g2.table_words(1), o.table(1))
+
+
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("v", [2, 3, 6, 9])
+def test_per_round_post_state_matches_oracle(kind, v, monkeypatch):
+    # per-round C-ABI calls with post-state reads between rounds: the
+    # two-round pass keeps folds pending, reads materialise them
+    monkeypatch.setenv("EFFSC_HIP_HOST_LOOP", "1")
+    n = 1 << v
+    tabs = [O.uniform(n, 300 + v + i) for i in range(2)]
+    g = MultilinearProver(tabs[0]) if kind == "ml" else InnerProductProver(tabs[0], tabs[1])
+    o = O.Prover.multilinear(tabs[0]) if kind == "ml" else O.Prover.inner_product(tabs[0], tabs[1])
+    prev = None
+    for r in range(v):
+        chal = None if prev is None else prev
+        got = g.round(chal)
+        exp = o.round(None if chal is None else [chal])
+        assert [[x] for x in got] == exp, f"round {r}"
+        if r % 2 == 1:
+            for i in range(1 if kind == "ml" else 2):
+                assert np.array_equal(g.table_words(i), o.table(i)), f"round {r} table {i}"
+        prev = (r * 0x9E3779B97F4A7C15 + 12345) % P
+    g.finalize(prev)
+    o.finalize([prev])
+    assert [g.final_value()] == o.final_value()
+
+
+@pytest.mark.parametrize("v", [4, 13, 18])
+def test_two_round_pass_equals_one_round_kernels(v, monkeypatch):
+    a, b = O.uniform(1 << v, 5), O.uniform(1 << v, 6)
+    monkeypatch.setenv("EFFSC_HIP_ONE_ROUND", "1")
+    one = sumcheck(InnerProductProver(a, b), v, Transcript(b"2r"))
+    one_ml = sumcheck(MultilinearProver(a), v, Transcript(b"2r"))
+    monkeypatch.delenv("EFFSC_HIP_ONE_ROUND")
+    two = sumcheck(InnerProductProver(a, b), v, Transcript(b"2r"))
+    two_ml = sumcheck(MultilinearProver(a), v, Transcript(b"2r"))
+    assert one.round_polys == two.round_polys and one.final_value == two.final_value
+    assert one_ml.round_polys == two_ml.round_polys and one_ml.final_value == two_ml.final_value
