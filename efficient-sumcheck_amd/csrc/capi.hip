@@ -193,7 +193,7 @@ template <class F, int K>
 int reduce(effsc_hip_prover* h, unsigned nblocks) {
   h->last_grid = nblocks;
   if (h->dev_mode) return EFFSC_HIP_OK;  // the caller launches k_reduce_fs
-  hipLaunchKernelGGL((k_reduce<F, K>), dim3(1), dim3(kBlock), 0, h->stream, h->red, nblocks);
+  hipLaunchKernelGGL((k_reduce<F, K>), dim3(1), dim3(1024), 0, h->stream, h->red, nblocks);
   HIPCHK(hipGetLastError());
   return EFFSC_HIP_OK;
 }
@@ -263,7 +263,7 @@ size_t tail_max() {
 template <class FI, class F, int K>
 struct DevOps {
   static int reduce_fs(effsc_hip_prover* h, unsigned g, size_t round) {
-    hipLaunchKernelGGL((k_reduce_fs<FI, K>), dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs, h->ext,
+    hipLaunchKernelGGL((k_reduce_fs<FI, K>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, h->ext,
                        h->d_polys + round * K * h->ext, h->d_chals + round * h->ext, h->d_cur);
     HIPCHK(hipGetLastError());
     return EFFSC_HIP_OK;
@@ -988,10 +988,10 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
       }
       unsigned g = h->last_grid;
       if (K == 1)
-        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
                            h->d_polys + r, h->d_chals + r, h->d_cur);
       else
-        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kBlock), 0, h->stream, h->red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
                            h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
       HIPCHK(hipGetLastError());
       r += 2;

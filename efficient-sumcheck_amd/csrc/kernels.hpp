@@ -200,7 +200,7 @@ __device__ __forceinline__ void finish(F* acc, Reduce r) {
 
 // Sums `nblocks` partials of K values each into r.result (one workgroup).
 template <class F, int K>
-__global__ __launch_bounds__(kBlock) void k_reduce(Reduce r, unsigned nblocks) {
+__global__ __launch_bounds__(1024) void k_reduce(Reduce r, unsigned nblocks) {
   F s[K];
 #pragma unroll
   for (int i = 0; i < K; i++) s[i] = gl::zero<F>();

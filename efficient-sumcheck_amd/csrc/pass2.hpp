@@ -64,42 +64,67 @@ constexpr int pass_values() {
   return K + K * (K + 1);
 }
 
+// Raw accumulators per unit.  K = 1: S0, S1, S2 = sums of E0, E1, E2.
+// K = 2 (products of table a with table b, A = E[0], B = E[1]):
+//   P1 = A0 B0, P2 = A1 B1, P3 = dA0 dB0, P4 = dA1 dB1, P5 = A2 B2,
+//   P6 = eA eB, P7 = (A3 - A2)(B3 - B2), P8 = fA fB
+// with dA0 = A2 - A0, dA1 = A3 - A1, eA = A1 - A0, fA = dA1 - dA0
+// (so eA + fA = A3 - A2).  Then
+//   round s:      q(0) = P1 + P2, q(inf) = P3 + P4
+//   round s+1:    q(0)(x)   = (A0 + x dA0)(B0 + x dB0) = P1 + x (P5 - P1 - P3) + x^2 P3
+//                 q(inf)(x) = (eA + x fA)(eB + x fB)   = P6 + x (P7 - P6 - P8) + x^2 P8
+// (A0 dB0 + dA0 B0 = A2 B2 - A0 B0 - dA0 dB0 since A2 = A0 + dA0; likewise
+// for e, f): 8 products per unit instead of 12.
+template <int K>
+constexpr int pass_raw() {
+  return K == 1 ? 3 : 8;
+}
+
 template <int K>
 struct PassAcc {
-  Acc128 a[pass_values<K>()];
+  Acc128 a[pass_raw<K>()];
 };
 
 // Accumulates one unit (level-L elements E[i][j], table i, j = 0..3).
 template <int K>
 __device__ __forceinline__ void pass_accumulate(const uint64_t (*E)[4], PassAcc<K>& acc) {
   if constexpr (K == 1) {
-    // round s: h(0) = sum of the low halves
     acc.a[0].add(E[0][0]);
-    acc.a[0].add(E[0][1]);
-    // round s+1: s0(x) = E0 + x (E2 - E0)
-    acc.a[1].add(E[0][0]);
-    acc.a[2].add(gl::sub(E[0][2], E[0][0]));
+    acc.a[1].add(E[0][1]);
+    acc.a[2].add(E[0][2]);
   } else {
     const uint64_t *A = E[0], *B = E[1];
     uint64_t dA0 = gl::sub(A[2], A[0]), dA1 = gl::sub(A[3], A[1]);
     uint64_t dB0 = gl::sub(B[2], B[0]), dB1 = gl::sub(B[3], B[1]);
-    // round s: q(0) = sum lo_a lo_b, q(inf) = sum d_a d_b over both pairs
-    acc.a[0].mac(A[0], B[0]);
-    acc.a[0].mac(A[1], B[1]);
-    acc.a[1].mac(dA0, dB0);
-    acc.a[1].mac(dA1, dB1);
-    // round s+1, q(0)(x) = (A0 + x dA0)(B0 + x dB0)
-    acc.a[2].mac(A[0], B[0]);
-    acc.a[3].mac(A[0], dB0);
-    acc.a[3].mac(dA0, B[0]);
-    acc.a[4].mac(dA0, dB0);
-    // q(inf)(x) = (eA + x fA)(eB + x fB), e = E1 - E0, f = dA1 - dA0
     uint64_t eA = gl::sub(A[1], A[0]), eB = gl::sub(B[1], B[0]);
     uint64_t fA = gl::sub(dA1, dA0), fB = gl::sub(dB1, dB0);
+    acc.a[0].mac(A[0], B[0]);
+    acc.a[1].mac(A[1], B[1]);
+    acc.a[2].mac(dA0, dB0);
+    acc.a[3].mac(dA1, dB1);
+    acc.a[4].mac(A[2], B[2]);
     acc.a[5].mac(eA, eB);
-    acc.a[6].mac(eA, fB);
-    acc.a[6].mac(fA, eB);
+    acc.a[6].mac(gl::sub(A[3], A[2]), gl::sub(B[3], B[2]));
     acc.a[7].mac(fA, fB);
+  }
+}
+
+// Raw sums -> pass values (layout of pass_values).
+template <int K>
+__device__ __forceinline__ void pass_values_from_raw(const Fp* s, Fp* v) {
+  if constexpr (K == 1) {
+    v[0] = gl::add(s[0], s[1]);              // round s: sum of low halves
+    v[1] = s[0];                             // round s+1: E0 + x (E2 - E0)
+    v[2] = gl::sub(s[2], s[0]);
+  } else {
+    v[0] = gl::add(s[0], s[1]);
+    v[1] = gl::add(s[2], s[3]);
+    v[2] = s[0];
+    v[3] = gl::sub(gl::sub(s[4], s[0]), s[2]);
+    v[4] = s[2];
+    v[5] = s[5];
+    v[6] = gl::sub(gl::sub(s[6], s[5]), s[7]);
+    v[7] = s[7];
   }
 }
 
@@ -161,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
   const size_t L = 4 * q;
   PassAcc<K> acc;
 #pragma unroll
-  for (int i = 0; i < pass_values<K>(); i++) acc.a[i].zero();
+  for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
   const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
   for (size_t u = gtid() * VEC; u < q; u += step) {
     uint64_t E0[K][4], E1[K][4];
@@ -191,9 +216,10 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
     pass_accumulate<K>(E0, acc);
     if constexpr (VEC == 2) pass_accumulate<K>(E1, acc);
   }
-  Fp vals[pass_values<K>()];
+  Fp raw[pass_raw<K>()], vals[pass_values<K>()];
 #pragma unroll
-  for (int i = 0; i < pass_values<K>(); i++) vals[i] = acc.a[i].reduce();
+  for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
   finish<Fp, pass_values<K>()>(vals, r);
 }
 
