@@ -47,6 +47,18 @@ def algorithmic_bytes(v, tables=2):
     return 8 * tables * words
 
 
+def recorded_traffic(path=os.path.join(ROOT, "profiles", "k_pass_ip_v28_traffic.md")):
+    # HBM bytes per k_pass launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs (scaled by a 1 GiB copy), or None when absent
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(v_sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -65,6 +77,9 @@ def main():
     ap.add_argument("--v", type=int, default=28)
     ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size (default: v)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="one 1 GiB device-to-device copy before the timed region (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one GPU).",
@@ -85,6 +100,13 @@ def main():
         N.check(L.effsc_hip_fill_uniform(dev, p, n, seed, 0))
         ptrs.append(p)
     addr = [C.cast(p, C.c_void_p).value for p in ptrs]
+    if args.calibrate:
+        cw = min(n, 1 << 27)
+        tmp = N.u64p()
+        N.check(L.effsc_hip_device_alloc(dev, cw, C.byref(tmp)))
+        N.check(L.effsc_hip_copy(dev, tmp, ptrs[0], cw))
+        N.check(L.effsc_hip_device_free(dev, tmp))
+        log(f"bench: calibration copy of {cw * 8} bytes done")
     prover = effsc.InnerProductProver(addr[0], addr[1], borrow=True, n=n, device=dev)
     log(f"bench: inner product v={v}, tables resident ({2 * n * 8 / 2**30:.1f} GiB)")
 
@@ -152,7 +174,8 @@ def main():
         # the reference algorithm's bytes (one fold per round, 32 N per table) over the proof time
         "gbps_reference_bytes": alg / T / 1e9,
         "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs[dom] / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
+                     "frac": gbs[dom] / HBM_PEAK_GBS,
+                     "traffic": recorded_traffic() if dom == 3 and v == 28 else None, "kernel": names[dom],
                      "launches_timed": int(launches[dom]), "kernel_ms_total": ms[dom],
                      "per_class_gbs": {names[i].split(" ")[0]: gbs[i] for i in range(4) if launches[i]}},
         "cpu_baseline": cpu,
