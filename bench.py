@@ -69,6 +69,66 @@ def cpu_baseline(v_sample, threads):
     return n / t, t
 
 
+def bench_batch(args):
+    # Batched inner-product proofs: `count` instances, v_i uniform in
+    # [vmin, vmax] from a fixed seed, uniform tables; the batch (one launch per
+    # round for all instances, hipGraph replay) against per-instance proofs
+    # (one device-resident proof per instance, the same inputs and outputs).
+    import random
+
+    import effsc
+    from effsc import _native as N
+    from effsc.batch import BatchProver, DeviceTable
+
+    rng = random.Random(0x5EED0004)
+    vs = [rng.randint(args.vmin, args.vmax) for _ in range(args.count)]
+    a = [DeviceTable.uniform(1 << v, 0x5EED0004 + 2 * i) for i, v in enumerate(vs)]
+    b = [DeviceTable.uniform(1 << v, 0x5EED0004 + 2 * i + 1) for i, v in enumerate(vs)]
+    total = sum(1 << v for v in vs)
+    log(f"bench: batch of {len(vs)} inner-product instances, v in [{args.vmin},{args.vmax}], "
+        f"{2 * total * 8 / 2**30:.1f} GiB of tables")
+    bp = BatchProver(a, b, kind=2, graph=True)
+
+    def ts():
+        return [effsc.Transcript(b"effsc-hip/batch/%d" % i) for i in range(len(vs))]
+
+    for _ in range(args.warmup):
+        ref = bp.prove(ts())
+    N.check(N.lib().effsc_hip_synchronize(0))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = bp.prove(ts())
+    T_batch = (time.perf_counter() - t0) / args.steps
+    assert all(p.round_polys == q.round_polys and p.final_value == q.final_value for p, q in zip(out, ref))
+    # per-instance baseline
+    provers = [effsc.InnerProductProver(C.cast(x.ptr, C.c_void_p).value, C.cast(y.ptr, C.c_void_p).value,
+                                        borrow=True, n=x.n)
+               for x, y in zip(a, b)]
+    def single_all():
+        res = []
+        for i, (p, v) in enumerate(zip(provers, vs)):
+            p.reset()
+            res.append(effsc.sumcheck(p, v, effsc.Transcript(b"effsc-hip/batch/%d" % i)))
+        return res
+    single_all()
+    t1 = time.perf_counter()
+    reps = max(1, args.steps // 2)
+    for _ in range(reps):
+        single = single_all()
+    T_single = (time.perf_counter() - t1) / reps
+    same = all(p.round_polys == [list(r) for r in q.round_polys] and p.final_value == q.final_value
+               for p, q in zip(out, single))
+    print(json.dumps({"metric": "batched proofs/s (512 inner-product instances, v ~ U[12,24])",
+                      "value": len(vs) / T_batch, "unit": "proofs/s", "ms_per_batch": T_batch * 1e3,
+                      "per_instance_launches_proofs_per_s": len(vs) / T_single,
+                      "per_instance_ms": T_single * 1e3, "speedup": T_single / T_batch,
+                      "bit_identical_to_per_instance": same, "instances": len(vs),
+                      "table_bytes": 2 * total * 8}), flush=True)
+    for p in provers:
+        p.close()
+    bp.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +137,11 @@ def main():
     ap.add_argument("--v", type=int, default=28)
     ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size (default: v)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["ip", "batch"], default="ip",
+                    help="ip: the v=28 inner-product proof (default, the headline line); batch: batched proofs")
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--vmin", type=int, default=12)
+    ap.add_argument("--vmax", type=int, default=24)
     ap.add_argument("--calibrate", action="store_true",
                     help="one 1 GiB device-to-device copy before the timed region (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -85,6 +150,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one GPU).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import effsc
     from effsc import _native as N

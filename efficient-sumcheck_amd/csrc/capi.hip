@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "effsc_hip.h"
+#include "common.hpp"
 #include "kernels.hpp"
 #include "pass2.hpp"
 #include "prove.hpp"
@@ -19,26 +20,55 @@ using namespace effsc_hip;
 using gl::Fp;
 using gl::Fp3;
 
-namespace {
-
+namespace effsc_hip {
 thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+const char* last_error() { return g_err.c_str(); }
 
-#define HIPCHK(expr)                                                                               \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess) return fail(EFFSC_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+// Import / export / synthetic-input kernels (used by this file only).
+// AoS caller words -> canonical SoA planes (optionally from Montgomery form).
+__global__ __launch_bounds__(kBlock) void k_import(uint64_t* dst, size_t stride, const uint64_t* src, size_t n,
+                                                   int d, int mont) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    for (int c = 0; c < d; c++) {
+      uint64_t x = src[i * d + c];
+      x = mont ? gl::mul(x, gl::MONT_TO_CANON) : gl::canon(x);
+      dst[c * stride + i] = x;
+    }
+  }
+}
 
-#define TRY(expr)                 \
-  do {                            \
-    int rc_ = (expr);             \
-    if (rc_ != EFFSC_HIP_OK) return rc_; \
-  } while (0)
+// Canonical SoA planes -> AoS caller words (optionally to Montgomery form).
+__global__ __launch_bounds__(kBlock) void k_export(uint64_t* dst, const uint64_t* src, size_t stride, size_t n,
+                                                   int d, int mont) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    for (int c = 0; c < d; c++) {
+      uint64_t x = src[c * stride + i];
+      dst[i * d + c] = mont ? gl::mul(x, gl::CANON_TO_MONT) : x;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_uniform(uint64_t* dst, size_t n, uint64_t key, uint64_t first) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < n; i += step) {
+    uint64_t base = key + (first + i) * 0x9E3779B97F4A7C15ull;
+    uint64_t c = mix64(base);
+    for (uint64_t a = 1; c >= gl::P; a++) c = mix64(base + a * 0xD1B54A32D192ED03ull);
+    dst[i] = c;
+  }
+}
+
+}  // namespace effsc_hip
+
+namespace {
+
+int fail(int code, const std::string& msg) { return effsc_hip::set_error(code, msg); }
 
 enum Kind { K_ML = 1, K_IP = 2, K_PRODUCT = 3, K_EQ = 4 };
 
@@ -115,12 +145,6 @@ struct effsc_hip_prover {
   uint64_t fpend[2] = {0, 0};
   bool have_next = false;
   uint64_t next_coef[kMaxK + kMaxK * (kMaxK + 1)] = {};
-};
-
-struct effsc_hip_transcript {
-  int kind;  // 0: Fiat-Shamir, 1: test RNG
-  FsTranscript fs;
-  uint64_t rng;
 };
 
 namespace {
@@ -1101,7 +1125,7 @@ void effsc_hip_config_default(effsc_hip_config* cfg) {
   cfg->flags = 0;
 }
 
-const char* effsc_hip_last_error(void) { return g_err.c_str(); }
+const char* effsc_hip_last_error(void) { return effsc_hip::last_error(); }
 
 int effsc_hip_multilinear_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n, effsc_hip_prover** out) {
   const uint64_t* t[1] = {evals};

@@ -333,46 +333,12 @@ __global__ __launch_bounds__(kBlock) void k_fold(Views<FI, K> in, Views<FO, K> o
   }
 }
 
-// AoS caller words -> canonical SoA planes (optionally from Montgomery form).
-__global__ __launch_bounds__(kBlock) void k_import(uint64_t* dst, size_t stride, const uint64_t* src, size_t n,
-                                                   int d, int mont) {
-  const size_t step = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = gtid(); i < n; i += step) {
-    for (int c = 0; c < d; c++) {
-      uint64_t x = src[i * d + c];
-      x = mont ? gl::mul(x, gl::MONT_TO_CANON) : gl::canon(x);
-      dst[c * stride + i] = x;
-    }
-  }
-}
-
-// Canonical SoA planes -> AoS caller words (optionally to Montgomery form).
-__global__ __launch_bounds__(kBlock) void k_export(uint64_t* dst, const uint64_t* src, size_t stride, size_t n,
-                                                   int d, int mont) {
-  const size_t step = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = gtid(); i < n; i += step) {
-    for (int c = 0; c < d; c++) {
-      uint64_t x = src[c * stride + i];
-      dst[i * d + c] = mont ? gl::mul(x, gl::CANON_TO_MONT) : x;
-    }
-  }
-}
-
 // Counter-based uniform values in [0, p): first candidate below p of
 // mix(mix(seed) + i G + a K) (splitmix64 finaliser).
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
-}
-__global__ __launch_bounds__(kBlock) void k_fill_uniform(uint64_t* dst, size_t n, uint64_t key, uint64_t first) {
-  const size_t step = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = gtid(); i < n; i += step) {
-    uint64_t base = key + (first + i) * 0x9E3779B97F4A7C15ull;
-    uint64_t c = mix64(base);
-    for (uint64_t a = 1; c >= gl::P; a++) c = mix64(base + a * 0xD1B54A32D192ED03ull);
-    dst[i] = c;
-  }
 }
 
 }  // namespace effsc_hip

@@ -64,6 +64,11 @@ _SIGS = {
         C.c_int,
         [C.c_int, u64, u64p, sz, sz, vp, u64p, sz, HOOK, vp, u64p, u64p, szp, C.POINTER(C.c_int)],
     ),
+    "effsc_hip_batch_new": (C.c_int, [C.c_int, C.c_int, sz, szp, C.POINTER(u64p), C.POINTER(u64p), C.c_uint,
+                                      C.POINTER(vp)]),
+    "effsc_hip_batch_rounds": (C.c_int, [vp, sz, szp, szp]),
+    "effsc_hip_batch_prove": (C.c_int, [vp, C.POINTER(vp), u64p, u64p, u64p]),
+    "effsc_hip_batch_free": (None, [vp]),
     "effsc_hip_device_alloc": (C.c_int, [C.c_int, sz, C.POINTER(u64p)]),
     "effsc_hip_device_free": (C.c_int, [C.c_int, u64p]),
     "effsc_hip_copy": (C.c_int, [C.c_int, u64p, u64p, sz]),

@@ -1,0 +1,96 @@
+"""Batched proofs of many independent multilinear / inner-product instances
+(effsc_hip_batch_*).  Each instance's proof equals runner::sumcheck on that
+instance alone (src/runner.rs:29-69); the batch only shares launches."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .proof import SumcheckProof
+
+NO_GRAPH = 0x2
+
+
+class DeviceTable:
+    """A canonical base-field table in device memory (owned)."""
+
+    def __init__(self, words, device=0):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        self.n = len(w)
+        self.device = device
+        self.ptr = N.u64p()
+        N.check(N.lib().effsc_hip_device_alloc(device, max(self.n, 1), C.byref(self.ptr)))
+        if self.n:
+            N.check(N.lib().effsc_hip_copy(device, self.ptr, w.ctypes.data_as(N.u64p), self.n))
+
+    @classmethod
+    def uniform(cls, n, seed, device=0):
+        t = cls.__new__(cls)
+        t.n, t.device = n, device
+        t.ptr = N.u64p()
+        N.check(N.lib().effsc_hip_device_alloc(device, max(n, 1), C.byref(t.ptr)))
+        N.check(N.lib().effsc_hip_fill_uniform(device, t.ptr, n, seed, 0))
+        return t
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            N.lib().effsc_hip_device_free(self.device, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BatchProver:
+    """kind 1: multilinear over tables a; kind 2: inner product of a and b.
+    Tables are DeviceTable objects (kept alive by this object)."""
+
+    def __init__(self, a, b=None, kind=2, device=0, graph=True):
+        self.kind = kind
+        self.count = len(a)
+        self._keep = (a, b)
+        lens = (C.c_size_t * self.count)(*[t.n for t in a])
+        pa = (N.u64p * self.count)(*[t.ptr for t in a])
+        pb = (N.u64p * self.count)(*[t.ptr for t in (b if b is not None else a)])
+        self.h = C.c_void_p()
+        N.check(N.lib().effsc_hip_batch_new(device, kind, self.count, lens, pa, pb, 0 if graph else NO_GRAPH,
+                                            C.byref(self.h)))
+        self.v, self.first = [], []
+        for i in range(self.count):
+            v, f = C.c_size_t(), C.c_size_t()
+            N.check(N.lib().effsc_hip_batch_rounds(self.h, i, C.byref(v), C.byref(f)))
+            self.v.append(v.value)
+            self.first.append(f.value)
+        self.total = sum(self.v)
+
+    def prove(self, transcripts):
+        """transcripts: one native transcript per instance (state advanced)."""
+        K = self.kind
+        ts = (C.c_void_p * self.count)(*[t.h for t in transcripts])
+        polys = np.zeros(max(self.total, 1) * K, dtype=np.uint64)
+        chals = np.zeros(max(self.total, 1), dtype=np.uint64)
+        fins = np.zeros(self.count, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_batch_prove(self.h, ts, polys.ctypes.data_as(N.u64p), chals.ctypes.data_as(N.u64p),
+                                              fins.ctypes.data_as(N.u64p)))
+        out = []
+        for i in range(self.count):
+            f, v = self.first[i], self.v[i]
+            rp = [[int(x) for x in polys[(f + r) * K:(f + r + 1) * K]] for r in range(v)]
+            out.append(SumcheckProof(rp, [int(c) for c in chals[f:f + v]], int(fins[i])))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().effsc_hip_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

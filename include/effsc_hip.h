@@ -67,6 +67,7 @@ typedef struct {
 } effsc_hip_config;
 
 typedef struct effsc_hip_prover effsc_hip_prover;
+typedef struct effsc_hip_transcript effsc_hip_transcript;
 
 /* Fills `cfg` with ext = input_ext = 1, canonical words, w = from_raw(7),
  * device 0, no flags. */
@@ -156,8 +157,6 @@ int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size
 
 /* --------------------------------------------------------- transcripts */
 
-typedef struct effsc_hip_transcript effsc_hip_transcript;
-
 /* Fiat-Shamir transcript (SHAKE256 duplex over a fixed byte encoding; the
  * reference binds spongefish here, src/transcript/spongefish.rs:17-44). */
 effsc_hip_transcript* effsc_hip_transcript_new(const uint8_t* domain_separator, size_t len);
@@ -194,6 +193,29 @@ int effsc_hip_sumcheck_verify(int ext, uint64_t w, const uint64_t* claimed_sum, 
                               effsc_hip_transcript* t, const uint64_t* proof, size_t proof_words,
                               effsc_hip_hook hook, void* hook_ctx, uint64_t* challenges, uint64_t* final_claim,
                               size_t* err_round, int* err_kind);
+
+/* ------------------------------------------------------- batched proofs */
+
+/* Many independent multilinear (kind 1) or inner-product (kind 2) instances
+ * of uneven power-of-two size proved together: every round of every
+ * instance runs in one launch, each instance's reduction and Fiat-Shamir step
+ * in one workgroup of a second launch, and the round sequence is captured
+ * once into a hipGraph (flags & 0x2 disables the graph).  Tables are
+ * borrowed device pointers (canonical base-field words; b ignored for kind
+ * 1), unmodified, alive until effsc_hip_batch_free.  No reference
+ * counterpart: the reference proves one instance per runner::sumcheck call
+ * (src/runner.rs:29-69); each instance's proof here equals that call's. */
+typedef struct effsc_hip_batch effsc_hip_batch;
+int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, const uint64_t* const* a,
+                        const uint64_t* const* b, unsigned flags, effsc_hip_batch** out);
+/* Rounds v_i = log2(len_i) of instance i and its offset in the round arrays. */
+int effsc_hip_batch_rounds(const effsc_hip_batch* b, size_t instance, size_t* v, size_t* first_round);
+/* Proves every instance with its own transcript (state read and updated).
+ * round_polys: sum_i v_i * kind words (instance i at first_round * kind);
+ * challenges: sum_i v_i words; finals: count words.  Canonical words. */
+int effsc_hip_batch_prove(effsc_hip_batch* b, effsc_hip_transcript* const* transcripts, uint64_t* round_polys,
+                          uint64_t* challenges, uint64_t* finals);
+void effsc_hip_batch_free(effsc_hip_batch* b);
 
 /* ------------------------------------------------------------ utilities */
 
