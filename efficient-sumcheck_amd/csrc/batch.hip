@@ -279,13 +279,15 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
     for (size_t i = 0; i < count; i++) {
       if (j > b->v[i]) continue;
       size_t u = units(b->v[i], j);
-      uint32_t first = (uint32_t)chunks.size();
+      // partials of round j are indexed by the chunk's position in that
+      // round's launch (blockIdx.x), i.e. relative to chunk_off[j]
+      uint32_t first = (uint32_t)(chunks.size() - b->chunk_off[j]);
       for (size_t s = 0; s < u; s += b->chunk_units)
         chunks.push_back({(uint32_t)i, (uint32_t)s, (uint32_t)std::min(b->chunk_units, u - s), 0});
       if (j < b->v[i]) {
         act.push_back((uint32_t)i);
         cb.push_back(first);
-        cc.push_back((uint32_t)(chunks.size() - first));
+        cc.push_back((uint32_t)(chunks.size() - b->chunk_off[j] - first));
       }
     }
     b->nchunks.push_back(chunks.size() - b->chunk_off.back());
