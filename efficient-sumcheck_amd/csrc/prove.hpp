@@ -147,10 +147,13 @@ __device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64
   uint64_t ch[3];
   fs.challenge(ext, ch);
   if (lane0) {
+    // exactly ext words each: chal_cur may be one slot of a per-instance array
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      if (c < ext) chal_out[c] = ch[c];
-      chal_cur[c] = ch[c];
+      if (c < ext) {
+        chal_out[c] = ch[c];
+        chal_cur[c] = ch[c];
+      }
     }
   }
   fs.store(t);
