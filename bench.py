@@ -1,19 +1,27 @@
-"""Benchmark: Goldilocks inner-product sumcheck (InnerProductProver, MSB),
-v = 28, two 2^28-element tables uniform in [0, p) from fixed seeds, all 28
-rounds with the SHAKE256 Fiat-Shamir transcript, on one MI355X.
+"""Benchmark of the MI355X sumcheck provers on one GPU.
 
-A step is one full proof (28 rounds + finalize + final value) over tables
+Default workload (the headline line): Goldilocks inner-product sumcheck
+(InnerProductProver, MSB), v = 28, two 2^28-element tables uniform in [0, p)
+from fixed seeds, all 28 rounds with the SHAKE256 Fiat-Shamir transcript.
+
+A step is one full proof (v rounds + finalize + final value) over tables
 already resident in HBM.  Metric: evals/s = 2^v / T_proof (the reference's
 criterion convention, Throughput::Elements(2^v), benches/sumcheck.rs:28,80).
 
+Other workloads (--workload):
+  ml        MultilinearProver, v = 20
+  product3  CoefficientProver, product of 3 multilinears (degree 3), v = 24,
+            base-field inputs, challenges and folded tables in F_p^3
+  eq, eq3   EqFactoredProver, v = 24, random point, base field / F_p^3
+  batch     512 inner-product instances, v ~ U[12, 24], batched vs one by one
+
 Prints one JSON line.  Besides the contract fields it carries
-  roofline     - the dominant kernel's achieved bandwidth: the algorithmic
-                 bytes of all its launches in the timed region (each input
-                 element read once, each output element written once) divided
-                 by their summed HIP-event durations (events recorded on the
-                 prover's stream around each launch);
-  cpu_baseline - the oracle's restatement of the reference's
-                 InnerProductProver (parallel folds, serial evaluate), timed on
+  roofline     - the dominant kernel class's achieved rate: its algorithmic
+                 bytes (each input element read once, each output element
+                 written once) or Goldilocks products over the summed
+                 HIP-event durations of its launches in the timed region
+                 (events recorded on the prover's stream around each launch);
+  cpu_baseline - the oracle's restatement of the reference prover, timed on
                  this host's cores on a bounded sample.
 """
 import argparse
@@ -28,23 +36,46 @@ sys.path.insert(0, os.path.join(ROOT, "efficient-sumcheck_amd"))
 
 HBM_PEAK_GBS = 8000.0
 SEED_A, SEED_B = 0x5EED0002, 0x5EED0003
+SEED_ML = 0x5EED0001
+SEEDS_P3 = (0x5EED0031, 0x5EED0032, 0x5EED0033)
+SEED_EQ_P, SEED_EQ_W = 0x5EED0034, 0x5EED0035
 DOMSEP = b"effsc-hip/bench/ip"
+
+CLASS_NAMES = ["k_fused (fold + evaluate)", "k_eval (evaluate)", "k_fold (fold)",
+               "k_pass (two-round pass: pending folds + evaluate + next-round coefficients)"]
+
+WORKLOADS = {
+    "ip": dict(kind="ip", v=28, ext=1, input_ext=1, tables=2,
+               metric="Goldilocks sumcheck evals/s (InnerProductProver MSB, v=28)"),
+    "ml": dict(kind="ml", v=20, ext=1, input_ext=1, tables=1,
+               metric="Goldilocks sumcheck evals/s (MultilinearProver MSB, v=20)"),
+    "product3": dict(kind="product", v=24, ext=3, input_ext=1, tables=3,
+                     metric="Goldilocks sumcheck evals/s (CoefficientProver product of 3, F_p^3 challenges, v=24)"),
+    "eq": dict(kind="eq", v=24, ext=1, input_ext=1, tables=1,
+               metric="Goldilocks sumcheck evals/s (EqFactoredProver, base field, v=24)"),
+    "eq3": dict(kind="eq", v=24, ext=3, input_ext=3, tables=1,
+                metric="Goldilocks sumcheck evals/s (EqFactoredProver, F_p^3, v=24)"),
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(v, tables=2):
-    # per table: round 0 reads N; round j >= 1 reads L_j and writes L_j/2;
-    # finalize folds 2 -> 1 (reads 2, writes 1)
+def algorithmic_bytes(v, tables=2, in_words=1, work_words=1):
+    # per table: round 0 reads N input elements; round 1 reads N input
+    # elements and writes N/2 work elements; round j >= 2 reads L_j and
+    # writes L_j/2 work elements; finalize folds 2 -> 1
     n = 1 << v
-    words = n  # round 0
-    L = n
+    if v == 0:
+        return 8 * tables * in_words
+    b = n * in_words  # round 0
+    b += n * in_words + (n // 2) * work_words  # round 1 (or finalize when v = 1)
+    L = n // 2
     while L >= 2:
-        words += L + L // 2
+        b += L * work_words + (L // 2) * work_words
         L //= 2
-    return 8 * tables * words
+    return 8 * tables * b
 
 
 def recorded_traffic(path=os.path.join(ROOT, "profiles", "k_pass_ip_v28_traffic.md")):
@@ -59,14 +90,211 @@ def recorded_traffic(path=os.path.join(ROOT, "profiles", "k_pass_ip_v28_traffic.
     return None
 
 
-def cpu_baseline(v_sample, threads):
+# ---------------------------------------------------------------- CPU side
+def cpu_baseline(workload, v_sample, threads):
+    """The oracle (a C restatement of the reference prover) timed on this
+    host.  Inputs are the same counter-based uniform tables."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     n = 1 << v_sample
-    a = O.uniform(n, SEED_A)
-    b = O.uniform(n, SEED_B)
-    t, _, _, _ = O.bench_ip(a, b, v_sample, threads)
-    return n / t, t
+    if workload == "ip":
+        a, b = O.uniform(n, SEED_A), O.uniform(n, SEED_B)
+        t, _, _, _ = O.bench_ip(a, b, v_sample, threads)
+        desc = "two parallel folds per round, serial evaluate, scalar Montgomery arithmetic"
+        return n / t, t, threads, desc
+    if workload == "ml":
+        p = O.Prover.multilinear(O.uniform(n, SEED_ML))
+    elif workload == "product3":
+        p = O.Prover.product([O.uniform(n, s) for s in SEEDS_P3], ext=3)
+    else:
+        ext = 3 if workload == "eq3" else 1
+        p = O.Prover.eq_factored(O.uniform(v_sample * ext, SEED_EQ_W), O.uniform(n * ext, SEED_EQ_P), ext=ext)
+    t0 = time.perf_counter()
+    p.sumcheck(v_sample, O.Transcript.shake(b"effsc-hip/bench/cpu"))
+    t = time.perf_counter() - t0
+    return n / t, t, 1, "serial scalar Montgomery restatement of the reference prover"
+
+
+# ---------------------------------------------------------------- GPU side
+def device_tables(L, N, dev, spec):
+    """Allocates and fills the workload's input tables on the device."""
+    v, ext, kind = spec["v"], spec["ext"], spec["kind"]
+    n = 1 << v
+    made = []
+
+    def table(words, seed):
+        p = N.u64p()
+        N.check(L.effsc_hip_device_alloc(dev, words, C.byref(p)))
+        N.check(L.effsc_hip_fill_uniform(dev, p, words, seed, 0))
+        made.append(p)
+        return C.cast(p, C.c_void_p).value
+
+    if kind == "ip":
+        addrs = [table(n, SEED_A), table(n, SEED_B)]
+    elif kind == "ml":
+        addrs = [table(n, SEED_ML)]
+    elif kind == "product":
+        addrs = [table(n, s) for s in SEEDS_P3]
+    else:
+        addrs = [table(n * ext, SEED_EQ_P)]  # ext planes of n words
+    return addrs, made
+
+
+def eq_point(L, N, dev, v, ext):
+    import numpy as np
+    words = v * ext
+    p = N.u64p()
+    N.check(L.effsc_hip_device_alloc(dev, words, C.byref(p)))
+    N.check(L.effsc_hip_fill_uniform(dev, p, words, SEED_EQ_W, 0))
+    host = np.zeros(words, dtype=np.uint64)
+    N.check(L.effsc_hip_copy(dev, host.ctypes.data_as(N.u64p), p, words))
+    N.check(L.effsc_hip_device_free(dev, p))
+    return host
+
+
+def make_prover(effsc, L, N, dev, spec, addrs):
+    v, n, ext = spec["v"], 1 << spec["v"], spec["ext"]
+    kind = spec["kind"]
+    if kind == "ip":
+        return effsc.InnerProductProver(addrs[0], addrs[1], borrow=True, n=n, device=dev)
+    if kind == "ml":
+        return effsc.MultilinearProver(addrs[0], borrow=True, n=n, device=dev)
+    if kind == "product":
+        return effsc.CoefficientProver(addrs, ext=ext, input_ext=1, borrow=True, n=n, device=dev)
+    w = eq_point(L, N, dev, v, ext)  # v * ext canonical words
+    return effsc.EqFactoredProver(w, addrs[0], ext=ext, borrow=True, device=dev)
+
+
+def proof_bytes(spec):
+    v, kind, ext = spec["v"], spec["kind"], spec["ext"]
+    if kind == "product":
+        return algorithmic_bytes(v, tables=3, in_words=1, work_words=3)
+    return algorithmic_bytes(v, tables=spec["tables"], in_words=ext, work_words=ext)
+
+
+def bench_single(args, workload):
+    import effsc
+    from effsc import _native as N
+
+    spec = dict(WORKLOADS[workload])
+    if args.v is not None:
+        spec["v"] = args.v
+    L = N.lib()
+    v, ext = spec["v"], spec["ext"]
+    n = 1 << v
+    dev = 0
+    addrs, made = device_tables(L, N, dev, spec)
+    if args.calibrate:
+        cw = min(n, 1 << 27)
+        tmp = N.u64p()
+        N.check(L.effsc_hip_device_alloc(dev, cw, C.byref(tmp)))
+        N.check(L.effsc_hip_copy(dev, tmp, made[0], cw))
+        N.check(L.effsc_hip_device_free(dev, tmp))
+        log(f"bench: calibration copy of {cw * 8} bytes done")
+    prover = make_prover(effsc, L, N, dev, spec, addrs)
+    log(f"bench: {workload} v={v} ext={ext}, tables resident "
+        f"({sum(1 for _ in made)} x {n * (ext if spec['kind'] == 'eq' else 1) * 8 / 2**20:.0f} MiB)")
+
+    def step():
+        prover.reset()
+        return effsc.sumcheck(prover, v, effsc.Transcript(DOMSEP, ext=ext))
+
+    for _ in range(args.warmup):
+        step()
+    N.check(L.effsc_hip_synchronize(dev))
+    N.check(L.effsc_hip_profile(prover.h, 1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = step()
+    N.check(L.effsc_hip_synchronize(dev))
+    elapsed = time.perf_counter() - t0
+    launches = (C.c_uint64 * 4)()
+    ms = (C.c_double * 4)()
+    nbytes = (C.c_double * 4)()
+    muls = (C.c_double * 4)()
+    N.check(L.effsc_hip_profile_read(prover.h, launches, ms, nbytes))
+    N.check(L.effsc_hip_profile_read_muls(prover.h, muls))
+    N.check(L.effsc_hip_profile(prover.h, 0))
+    # the profiled region adds event records; time an unprofiled region too
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    N.check(L.effsc_hip_synchronize(dev))
+    elapsed_plain = time.perf_counter() - t1
+    T = min(elapsed, elapsed_plain) / args.steps
+
+    gbs = [nbytes[i] / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0 for i in range(4)]
+    gmuls = [muls[i] / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0 for i in range(4)]
+    alg = proof_bytes(spec)
+    log(f"bench: {T * 1e3:.3f} ms/proof; " + "; ".join(
+        f"{CLASS_NAMES[i].split(' ')[0]} {launches[i]} launches {ms[i]:.2f} ms {gbs[i]:.0f} GB/s "
+        f"{gmuls[i]:.0f} Gmul/s" for i in range(4) if launches[i]) + f"; final {proof.final_value}")
+
+    valu = spec["kind"] == "product"
+    profiled = any(launches[i] for i in range(4))
+    if profiled:
+        dom = max(range(4), key=lambda i: ms[i])
+        kernel = CLASS_NAMES[dom]
+    if valu:
+        peak = C.c_double()
+        N.check(L.effsc_hip_mul_rate(dev, C.byref(peak)))
+        peak_g = peak.value / 1e9
+        roof = {"bound": "valu", "achieved": gmuls[dom], "peak": peak_g, "unit": "Gmul/s",
+                "frac": gmuls[dom] / peak_g, "traffic": None, "kernel": kernel,
+                "peak_source": "effsc_hip_mul_rate: independent Goldilocks product chains on every CU",
+                "hbm_achieved_gbs": gbs[dom], "hbm_frac": gbs[dom] / HBM_PEAK_GBS,
+                "launches_timed": int(launches[dom]), "kernel_ms_total": ms[dom]}
+    elif profiled:
+        roof = {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs[dom] / HBM_PEAK_GBS,
+                "traffic": recorded_traffic() if (workload == "ip" and dom == 3 and v == 28) else None,
+                "kernel": kernel, "launches_timed": int(launches[dom]), "kernel_ms_total": ms[dom]}
+    else:
+        # no multi-workgroup round kernel is profiled on this path: whole-proof rate
+        roof = {"bound": "hbm", "achieved": alg / T / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": alg / T / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "whole proof (algorithmic bytes / proof time; latency-bound at this size)"}
+    roof["per_class"] = {CLASS_NAMES[i].split(" ")[0]: {"launches": int(launches[i]), "ms": ms[i], "gbs": gbs[i],
+                                                        "gmul_s": gmuls[i]} for i in range(4) if launches[i]}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        # bounded samples: a few seconds of the serial oracle at most
+        default_cpu_v = {"ip": v, "ml": v, "product3": min(v, 22), "eq": v, "eq3": min(v, 22)}[workload]
+        cv = args.cpu_v if args.cpu_v is not None else default_cpu_v
+        log(f"bench: CPU baseline (oracle port of the reference prover), v={cv}")
+        rate, secs, cores, desc = cpu_baseline(workload, cv, threads)
+        cpu = {"value": rate, "unit": "evals/s", "cores": cores, "kind": "port",
+               "sample": f"one full proof at v={cv} ({secs:.2f} s): {desc}"}
+
+    inputs = {"ip": "seeds 0x5EED0002/3", "ml": "seed 0x5EED0001", "product3": "seeds 0x5EED0031/32/33",
+              "eq": "p seed 0x5EED0034, point seed 0x5EED0035", "eq3": "p seed 0x5EED0034, point seed 0x5EED0035"}
+    out = {
+        "metric": spec["metric"] if v == WORKLOADS[workload]["v"] else spec["metric"].rsplit("v=", 1)[0] + f"v={v})",
+        "value": n / T,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"{workload}_msb_v{v}", "v": v, "field": "goldilocks" if ext == 1 else "goldilocks^3",
+                   "tables": spec["tables"], "transcript": "shake256-fs",
+                   "input": f"uniform [0,p) counter-based, {inputs[workload]}"},
+        # the reference algorithm's bytes (one fold per round) over the proof time
+        "gbps_reference_bytes": alg / T / 1e9,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    prover.close()
+    for p in made:
+        L.effsc_hip_device_free(dev, p)
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):
@@ -104,6 +332,7 @@ def bench_batch(args):
     provers = [effsc.InnerProductProver(C.cast(x.ptr, C.c_void_p).value, C.cast(y.ptr, C.c_void_p).value,
                                         borrow=True, n=x.n)
                for x, y in zip(a, b)]
+
     def single_all():
         res = []
         for i, (p, v) in enumerate(zip(provers, vs)):
@@ -118,12 +347,17 @@ def bench_batch(args):
     T_single = (time.perf_counter() - t1) / reps
     same = all(p.round_polys == [list(r) for r in q.round_polys] and p.final_value == q.final_value
                for p, q in zip(out, single))
+    alg = sum(algorithmic_bytes(v, tables=2) for v in vs)
     print(json.dumps({"metric": "batched proofs/s (512 inner-product instances, v ~ U[12,24])",
-                      "value": len(vs) / T_batch, "unit": "proofs/s", "ms_per_batch": T_batch * 1e3,
+                      "value": len(vs) / T_batch, "unit": "proofs/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": T_batch * 1e3, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+                      "config": {"workload": "batched_ip", "instances": len(vs), "vmin": args.vmin,
+                                 "vmax": args.vmax, "table_bytes": 2 * total * 8, "seed": "0x5EED0004"},
                       "per_instance_launches_proofs_per_s": len(vs) / T_single,
                       "per_instance_ms": T_single * 1e3, "speedup": T_single / T_batch,
-                      "bit_identical_to_per_instance": same, "instances": len(vs),
-                      "table_bytes": 2 * total * 8}), flush=True)
+                      "bit_identical_to_per_instance": same,
+                      "gbps_reference_bytes": alg / T_batch / 1e9}), flush=True)
     for p in provers:
         p.close()
     bp.close()
@@ -134,11 +368,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--v", type=int, default=28)
-    ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size (default: v)")
+    ap.add_argument("--v", type=int, default=None, help="number of variables (default: the workload's)")
+    ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["ip", "batch"], default="ip",
-                    help="ip: the v=28 inner-product proof (default, the headline line); batch: batched proofs")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch"], default="ip",
+                    help="ip: the v=28 inner-product proof (default, the headline line)")
     ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--vmin", type=int, default=12)
     ap.add_argument("--vmax", type=int, default=24)
@@ -150,108 +384,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one GPU).",
               file=sys.stderr)
         sys.exit(2)
-
     if args.workload == "batch":
         return bench_batch(args)
-
-    import effsc
-    from effsc import _native as N
-
-    L = N.lib()
-    v = args.v
-    n = 1 << v
-    dev = 0
-    ptrs = []
-    for seed in (SEED_A, SEED_B):
-        p = N.u64p()
-        N.check(L.effsc_hip_device_alloc(dev, n, C.byref(p)))
-        N.check(L.effsc_hip_fill_uniform(dev, p, n, seed, 0))
-        ptrs.append(p)
-    addr = [C.cast(p, C.c_void_p).value for p in ptrs]
-    if args.calibrate:
-        cw = min(n, 1 << 27)
-        tmp = N.u64p()
-        N.check(L.effsc_hip_device_alloc(dev, cw, C.byref(tmp)))
-        N.check(L.effsc_hip_copy(dev, tmp, ptrs[0], cw))
-        N.check(L.effsc_hip_device_free(dev, tmp))
-        log(f"bench: calibration copy of {cw * 8} bytes done")
-    prover = effsc.InnerProductProver(addr[0], addr[1], borrow=True, n=n, device=dev)
-    log(f"bench: inner product v={v}, tables resident ({2 * n * 8 / 2**30:.1f} GiB)")
-
-    def step():
-        prover.reset()
-        return effsc.sumcheck(prover, v, effsc.Transcript(DOMSEP))
-
-    for _ in range(args.warmup):
-        step()
-    N.check(L.effsc_hip_synchronize(dev))
-    N.check(L.effsc_hip_profile(prover.h, 1))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        proof = step()
-    N.check(L.effsc_hip_synchronize(dev))
-    elapsed = time.perf_counter() - t0
-    launches = (C.c_uint64 * 4)()
-    ms = (C.c_double * 4)()
-    nbytes = (C.c_double * 4)()
-    N.check(L.effsc_hip_profile_read(prover.h, launches, ms, nbytes))
-    N.check(L.effsc_hip_profile(prover.h, 0))
-    # the profiled region adds event records; time an unprofiled region too
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    N.check(L.effsc_hip_synchronize(dev))
-    elapsed_plain = time.perf_counter() - t1
-    T = min(elapsed, elapsed_plain) / args.steps
-
-    # dominant kernel class: 0 fused fold+evaluate, 1 evaluate, 2 fold, 3 two-round pass
-    names = {0: "k_fused<Fp,Fp,2,2> (fold + evaluate)", 1: "k_eval<Fp,2,2> (evaluate)", 2: "k_fold (fold)",
-             3: "k_pass<2,P,2> (two-round pass: P pending folds + evaluate + next-round coefficients)"}
-    dom = max(range(4), key=lambda i: ms[i])
-    gbs = [nbytes[i] / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0 for i in range(4)]
-    alg = algorithmic_bytes(v)
-    log(f"bench: {T * 1e3:.3f} ms/proof; " + "; ".join(
-        f"{names[i].split(' ')[0]} {launches[i]} launches {ms[i]:.2f} ms {gbs[i]:.0f} GB/s" for i in range(4) if launches[i])
-        + f"; final {proof.final_value}")
-
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cv = args.cpu_v if args.cpu_v is not None else v
-        log(f"bench: CPU baseline (oracle port of the reference IP prover), v={cv}, {threads} threads")
-        rate, secs = cpu_baseline(cv, threads)
-        cpu = {"value": rate, "unit": "evals/s", "cores": threads, "kind": "port",
-               "sample": f"one full InnerProductProver proof at v={cv} ({secs:.2f} s): two parallel folds "
-                         f"per round, serial evaluate, scalar Montgomery arithmetic"}
-
-    out = {
-        "metric": "Goldilocks sumcheck evals/s (InnerProductProver MSB, v=28)",
-        "value": n / T,
-        "unit": "evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": T * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": f"inner_product_msb_v{v}", "v": v, "field": "goldilocks", "tables": 2,
-                   "transcript": "shake256-fs", "input": "uniform [0,p) counter-based, seeds 0x5EED0002/3"},
-        # the reference algorithm's bytes (one fold per round, 32 N per table) over the proof time
-        "gbps_reference_bytes": alg / T / 1e9,
-        "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs[dom] / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic() if dom == 3 and v == 28 else None, "kernel": names[dom],
-                     "launches_timed": int(launches[dom]), "kernel_ms_total": ms[dom],
-                     "per_class_gbs": {names[i].split(" ")[0]: gbs[i] for i in range(4) if launches[i]}},
-        "cpu_baseline": cpu,
-    }
-    prover.close()
-    for p in ptrs:
-        L.effsc_hip_device_free(dev, p)
-    print(json.dumps(out), flush=True)
+    return bench_single(args, args.workload)
 
 
 if __name__ == "__main__":

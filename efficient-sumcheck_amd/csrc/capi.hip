@@ -117,7 +117,7 @@ struct effsc_hip_prover {
   uint64_t* eq_l = nullptr;  // ext planes, stride eq_l_stride
   uint64_t* eq_r = nullptr;
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
-  uint64_t* eq_inner = nullptr;  // unused
+  uint64_t* eq_point = nullptr;  // canonical point, v * ext words (rebuilds the eq tables on reset)
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
   unsigned last_grid = 0;
@@ -132,10 +132,15 @@ struct effsc_hip_prover {
   // 3 two-round pass.
   bool prof_on = false;
   std::vector<hipEvent_t> ev;             // pairs (2i, 2i+1)
-  std::vector<std::pair<int, double>> pend;  // (slot, bytes) of recorded pairs
+  struct Pend {
+    int slot;
+    double bytes, muls;
+  };
+  std::vector<Pend> pend;  // recorded event pairs
   uint64_t prof_launches[4] = {0, 0, 0, 0};
   double prof_ms[4] = {0, 0, 0, 0};
   double prof_bytes[4] = {0, 0, 0, 0};
+  double prof_muls[4] = {0, 0, 0, 0};  // Goldilocks products (ext3 product = 8)
   // Two-round passes (base-field ML / IP): the logical table is the
   // materialised one (len) folded by the npend challenges in fpend; when
   // have_next, the next round's values are the polynomials in next_coef
@@ -173,23 +178,37 @@ int prof_begin(effsc_hip_prover* h) {
   HIPCHK(hipEventRecord(h->ev[2 * i], h->stream));
   return EFFSC_HIP_OK;
 }
-int prof_end(effsc_hip_prover* h, int slot, double bytes) {
+int prof_end(effsc_hip_prover* h, int slot, double bytes, double muls) {
   if (!h->prof_on) return EFFSC_HIP_OK;
   size_t i = h->pend.size();
   HIPCHK(hipEventRecord(h->ev[2 * i + 1], h->stream));
-  h->pend.push_back({slot, bytes});
+  h->pend.push_back({slot, bytes, muls});
   return EFFSC_HIP_OK;
 }
+
+// Goldilocks products per field product (Karatsuba F_p^3: 6 + 2 by w) and
+// per fold product c * (hi - lo) with c in FO, the difference in FI.
+template <class F>
+constexpr double mul_cost() {
+  return F::D == 1 ? 1.0 : 8.0;
+}
+template <class FI, class FO>
+constexpr double fold_cost() {
+  return FI::D == FO::D ? mul_cost<FO>() : 3.0;
+}
+// products of one EvalsInfty pair evaluation: K values of K - 1 products
+constexpr double eval_products(int K) { return K == 1 ? 0.0 : (double)K * (K - 1); }
 int prof_collect(effsc_hip_prover* h) {
   if (!h->prof_on) return EFFSC_HIP_OK;
   for (size_t i = 0; i < h->pend.size(); i++) {
     HIPCHK(hipEventSynchronize(h->ev[2 * i + 1]));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
-    int s = h->pend[i].first;
+    int s = h->pend[i].slot;
     h->prof_launches[s]++;
     h->prof_ms[s] += ms;
-    h->prof_bytes[s] += h->pend[i].second;
+    h->prof_bytes[s] += h->pend[i].bytes;
+    h->prof_muls[s] += h->pend[i].muls;
   }
   h->pend.clear();
   return EFFSC_HIP_OK;
@@ -237,7 +256,7 @@ struct Ops {
     }
     HIPCHK(hipGetLastError());
     // algorithmic bytes: every element of the K tables read once
-    TRY(prof_end(h, 1, 8.0 * K * FI::D * (double)(half + paired)));
+    TRY(prof_end(h, 1, 8.0 * K * FI::D * (double)(half + paired), eval_products(K) * mul_cost<FI>() * half));
     return reduce<FI, K>(h, vec ? blocks_for(h, half / 2) : blocks_for(h, half));
   }
   // chal: host words of the challenge, or nullptr with cptr the device words
@@ -257,7 +276,8 @@ struct Ops {
     }
     HIPCHK(hipGetLastError());
     // algorithmic bytes: read 4q elements, write 2q elements, per table
-    TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D)));
+    TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D),
+                 2.0 * q * (K * fold_cost<FI, FO>() + eval_products(K) * mul_cost<FO>())));
     return reduce<FO, K>(h, vec ? blocks_for(h, q / 2) : blocks_for(h, q));
   }
   static int fold(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t half,
@@ -269,7 +289,8 @@ struct Ops {
     TRY(prof_begin(h));
     hipLaunchKernelGGL((k_fold<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, half, paired, h->w);
     HIPCHK(hipGetLastError());
-    TRY(prof_end(h, 2, 8.0 * K * ((double)(half + paired) * FI::D + (double)half * FO::D)));
+    TRY(prof_end(h, 2, 8.0 * K * ((double)(half + paired) * FI::D + (double)half * FO::D),
+                 (double)half * K * fold_cost<FI, FO>()));
     return EFFSC_HIP_OK;
   }
 };
@@ -469,7 +490,10 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
                        q, h->red);
   HIPCHK(hipGetLastError());
   double bytes = 8.0 * K * ((double)(L << P) + (P > 0 ? (double)L : 0.0));
-  TRY(prof_end(h, 3, bytes));
+  // per unit: 4 level-L elements per table, each 0, 1 or 3 fold products
+  // away from the stored level; IP accumulates 8 products
+  double muls = (double)q * (4.0 * K * (P == 0 ? 0 : P == 1 ? 1 : 3) + (K == 2 ? 8 : 0));
+  TRY(prof_end(h, 3, bytes, muls));
   h->last_grid = g;
   return EFFSC_HIP_OK;
 }
@@ -619,7 +643,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->red.result) (void)hipFree(h->red.result);
   if (h->eq_l) (void)hipFree(h->eq_l);
   if (h->eq_r) (void)hipFree(h->eq_r);
-  if (h->eq_inner) (void)hipFree(h->eq_inner);
+  if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
@@ -928,7 +952,7 @@ int build_eq(effsc_hip_prover* h, const uint64_t* wpt_canon_dev, size_t off, siz
   size_t n = (size_t)1 << vv;
   *stride = round_up(n, 64);
   *len = n;
-  TRY(alloc_dev(planes, *stride * h->ext));
+  if (!*planes) TRY(alloc_dev(planes, *stride * h->ext));
   unsigned g = blocks_for(h, n);
   if (h->ext == 1)
     hipLaunchKernelGGL(k_eq_table<Fp>, dim3(g), dim3(kBlock), 0, h->stream, *planes, *stride,
@@ -1155,7 +1179,11 @@ int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point
   if (n > ((size_t)1 << v))
     return fail(EFFSC_HIP_EINVAL, "p_evals length exceeds 2^v");  // src/provers/eq_factored.rs:93-99
   if (cfg->input_ext != cfg->ext) return fail(EFFSC_HIP_EINVAL, "eq-factored input must be in the prover's field");
-  if (cfg->flags & EFFSC_HIP_BORROW_INPUT) return fail(EFFSC_HIP_EINVAL, "eq-factored copies its input");
+  const bool borrow = (cfg->flags & EFFSC_HIP_BORROW_INPUT) != 0;
+  if (borrow && n != ((size_t)1 << v))
+    return fail(EFFSC_HIP_EINVAL, "a borrowed p must hold exactly 2^v elements (no padding)");
+  if (borrow && (!p || reinterpret_cast<uintptr_t>(p) % 16))
+    return fail(EFFSC_HIP_EINVAL, "borrowed table null or not 16-byte aligned");
   auto* h = new effsc_hip_prover();
   h->kind = K_EQ;
   h->ext = h->in_ext = cfg->ext;
@@ -1170,19 +1198,25 @@ int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point
   h->v = v;
   h->v_l = v / 2;
   int rc = setup_common(h, N >> 1);
+  if (rc == EFFSC_HIP_OK && borrow) {
+    h->own_in = false;
+    h->in_stride = N;
+    h->in[0] = const_cast<uint64_t*>(p);
+  }
   // p is zero-padded to 2^v: import the given prefix, clear the rest
-  if (rc == EFFSC_HIP_OK) {
+  if (rc == EFFSC_HIP_OK && !borrow) {
     h->own_in = true;
     h->in_stride = round_up(N, 64);
     rc = alloc_dev(&h->in[0], h->in_stride * h->ext);
   }
-  if (rc == EFFSC_HIP_OK) {
+  if (rc == EFFSC_HIP_OK && !borrow) {
     hipError_t e = hipMemsetAsync(h->in[0], 0, h->in_stride * h->ext * 8, h->stream);
     if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
   }
   uint64_t* staging = nullptr;
-  if (rc == EFFSC_HIP_OK) rc = alloc_dev(&staging, std::max<size_t>(n, v) * h->ext + 1);
-  if (rc == EFFSC_HIP_OK && n) {
+  if (rc == EFFSC_HIP_OK && !borrow) rc = alloc_dev(&staging, std::max<size_t>(n, 1) * h->ext + 1);
+  if (rc == EFFSC_HIP_OK) rc = alloc_dev(&h->eq_point, std::max<size_t>(v, 1) * h->ext);
+  if (rc == EFFSC_HIP_OK && n && !borrow) {
     hipError_t e = hipMemcpyAsync(staging, p, n * h->ext * 8, hipMemcpyDefault, h->stream);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_import, dim3(blocks_for(h, n)), dim3(kBlock), 0, h->stream, h->in[0], h->in_stride,
@@ -1199,10 +1233,11 @@ int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point
       uint64_t x = point[i];
       wc[i] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(x, gl::MONT_TO_CANON) : gl::canon(x);
     }
-    hipError_t e = hipMemcpyAsync(staging, wc.data(), wc.size() * 8, hipMemcpyHostToDevice, h->stream);
+    hipError_t e = hipMemcpyAsync(h->eq_point, wc.data(), wc.size() * 8, hipMemcpyHostToDevice, h->stream);
     if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
-    if (rc == EFFSC_HIP_OK) rc = build_eq(h, staging, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len);
-    if (rc == EFFSC_HIP_OK) rc = build_eq(h, staging, h->v_l, v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len);
+    if (rc == EFFSC_HIP_OK) rc = build_eq(h, h->eq_point, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len);
+    if (rc == EFFSC_HIP_OK)
+      rc = build_eq(h, h->eq_point, h->v_l, v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len);
     if (rc == EFFSC_HIP_OK) {
       e = hipStreamSynchronize(h->stream);
       if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
@@ -1315,7 +1350,13 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
 
 int effsc_hip_reset(effsc_hip_prover* h) {
   if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
-  if (h->kind == K_EQ) return fail(EFFSC_HIP_ESTATE, "eq-factored prover folds its eq tables in place; rebuild it");
+  if (h->kind == K_EQ) {
+    // the eq tables are folded in place: rebuild them from the stored point
+    HIPCHK(hipSetDevice(h->dev));
+    TRY(build_eq(h, h->eq_point, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len));
+    TRY(build_eq(h, h->eq_point, h->v_l, h->v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len));
+    h->rounds_elapsed = 0;
+  }
   h->len = h->n;
   h->in_work = false;
   h->npend = 0;
@@ -1334,6 +1375,7 @@ int effsc_hip_profile(effsc_hip_prover* h, int enable) {
     h->prof_launches[i] = 0;
     h->prof_ms[i] = 0;
     h->prof_bytes[i] = 0;
+    h->prof_muls[i] = 0;
   }
   return EFFSC_HIP_OK;
 }
@@ -1345,6 +1387,12 @@ int effsc_hip_profile_read(const effsc_hip_prover* h, uint64_t* launches, double
     ms[i] = h->prof_ms[i];
     bytes[i] = h->prof_bytes[i];
   }
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_profile_read_muls(const effsc_hip_prover* h, double* muls) {
+  if (!h || !muls) return fail(EFFSC_HIP_EINVAL, "null argument");
+  for (int i = 0; i < 4; i++) muls[i] = h->prof_muls[i];
   return EFFSC_HIP_OK;
 }
 
@@ -1554,6 +1602,53 @@ extern "C" int effsc_hip_sumcheck_verify(int ext, uint64_t w, const uint64_t* cl
 }
 
 // ------------------------------------------------------------------ utilities
+// Throughput probe: each thread runs 8 independent chains x <- x * y of
+// Goldilocks products (the VALU roofline of the degree >= 3 rounds).
+__global__ __launch_bounds__(kBlock) void k_mul_rate(uint64_t* out, uint64_t y, int iters) {
+  uint64_t x[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[i] = gtid() * 8 + i + 1;
+  for (int it = 0; it < iters; it++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = gl::mul(x[i], y);
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) s ^= x[i];
+  out[gtid()] = s;
+}
+
+extern "C" int effsc_hip_mul_rate(int device, double* muls_per_s) {
+  if (!muls_per_s) return fail(EFFSC_HIP_EINVAL, "null out");
+  HIPCHK(hipSetDevice(device));
+  const unsigned grid = 8192;
+  const int iters = 2048;
+  uint64_t* out = nullptr;
+  TRY(alloc_dev(&out, (size_t)grid * kBlock));
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float ms = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mul_rate, dim3(grid), dim3(kBlock), 0, st, out, 0x123456789ABCDEFull, iters);
+    e = hipEventRecord(e0, st);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mul_rate, dim3(grid), dim3(kBlock), 0, st, out, 0xFEDCBA987654321ull, iters);
+    e = hipEventRecord(e1, st);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(out);
+  if (e != hipSuccess) return fail(EFFSC_HIP_EDEVICE, std::string("mul rate: ") + hipGetErrorString(e));
+  *muls_per_s = (double)grid * kBlock * 8 * iters / (ms * 1e-3);
+  return EFFSC_HIP_OK;
+}
+
 extern "C" int effsc_hip_device_alloc(int device, size_t words, uint64_t** out) {
   if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
   HIPCHK(hipSetDevice(device));

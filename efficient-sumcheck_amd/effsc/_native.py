@@ -52,6 +52,7 @@ _SIGS = {
     "effsc_hip_free": (None, [vp]),
     "effsc_hip_profile": (C.c_int, [vp, C.c_int]),
     "effsc_hip_profile_read": (C.c_int, [vp, u64p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "effsc_hip_profile_read_muls": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "effsc_hip_synchronize": (C.c_int, [C.c_int]),
     "effsc_hip_fold": (C.c_int, [C.c_int, u64p, sz, u64, szp]),
     "effsc_hip_transcript_new": (vp, [C.c_char_p, sz]),
@@ -73,6 +74,7 @@ _SIGS = {
     "effsc_hip_device_free": (C.c_int, [C.c_int, u64p]),
     "effsc_hip_copy": (C.c_int, [C.c_int, u64p, u64p, sz]),
     "effsc_hip_fill_uniform": (C.c_int, [C.c_int, u64p, sz, u64, u64]),
+    "effsc_hip_mul_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
 }
 
 # Entry points declared in include/effsc_hip.h (checked by the CPU tests).

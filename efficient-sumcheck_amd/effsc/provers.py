@@ -218,14 +218,20 @@ ProductProver = CoefficientProver
 class EqFactoredProver(NativeProver):
     """EqFactoredProver (src/provers/eq_factored.rs:67-289): sum_x eq(w,x) p(x)."""
 
-    def __init__(self, w_point, p_evals, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0):
+    def __init__(self, w_point, p_evals, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False):
         super().__init__(ext, repr)
-        cfg = _config(ext, ext, repr, w, device, 0)
+        cfg = _config(ext, ext, repr, w, device, BORROW_INPUT if borrow else 0)
         W = _words(w_point, ext)
-        Pv = _words(p_evals, ext)
         h = C.c_void_p()
-        N.check(N.lib().effsc_hip_eq_factored_new(C.byref(cfg), W.ctypes.data_as(N.u64p), len(W) // ext,
-                                                   Pv.ctypes.data_as(N.u64p), len(Pv) // ext, C.byref(h)))
+        if borrow:
+            # p_evals is a device address of a canonical 2^v-element table
+            v = len(W) // ext
+            N.check(N.lib().effsc_hip_eq_factored_new(C.byref(cfg), W.ctypes.data_as(N.u64p), v,
+                                                       C.cast(C.c_void_p(p_evals), N.u64p), 1 << v, C.byref(h)))
+        else:
+            Pv = _words(p_evals, ext)
+            N.check(N.lib().effsc_hip_eq_factored_new(C.byref(cfg), W.ctypes.data_as(N.u64p), len(W) // ext,
+                                                       Pv.ctypes.data_as(N.u64p), len(Pv) // ext, C.byref(h)))
         self._finish(h)
 
     def final_factors(self):

@@ -94,7 +94,9 @@ int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* ta
                           effsc_hip_prover** out);
 
 /* EqFactoredProver::new(w, p) — src/provers/eq_factored.rs:90-114.
- * Degree 2; `point` holds v elements, p holds n <= 2^v elements (zero-padded). */
+ * Degree 2; `point` holds v elements (host memory), p holds n <= 2^v elements
+ * (zero-padded).  With EFFSC_HIP_BORROW_INPUT, p is a canonical device table
+ * of exactly 2^v elements that the prover never writes. */
 int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point, size_t v, const uint64_t* p,
                               size_t n, effsc_hip_prover** out);
 
@@ -125,7 +127,8 @@ int effsc_hip_table_len(const effsc_hip_prover* h, size_t table, size_t* out);
  * with EINVAL if cap_words is too small. */
 int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_t cap_words, size_t* len);
 
-/* Restarts the prover from its (unmodified) input tables.  No reference
+/* Restarts the prover from its (unmodified) input tables; an EqFactored
+ * prover also rebuilds its eq tables from the point.  No reference
  * counterpart: the reference consumes its tables. */
 int effsc_hip_reset(effsc_hip_prover* h);
 
@@ -133,13 +136,16 @@ int effsc_hip_reset(effsc_hip_prover* h);
 void effsc_hip_free(effsc_hip_prover* h);
 
 /* Kernel timing, no reference counterpart.  effsc_hip_profile(h, 1) clears
- * the counters and brackets every later launch with HIP events on the
- * prover's stream; effsc_hip_profile_read returns, per kernel class
- * (0 fused fold+evaluate, 1 evaluate, 2 fold), the launch count, the summed
- * event time in ms and the summed algorithmic bytes (each input element read
- * once, each output element written once). */
+ * the counters and brackets every later multi-workgroup round launch with HIP
+ * events on the prover's stream; effsc_hip_profile_read returns, per kernel
+ * class (0 fused fold+evaluate, 1 evaluate, 2 fold, 3 two-round pass; arrays
+ * of 4), the launch count, the summed event time in ms and the summed
+ * algorithmic bytes (each input element read once, each output element
+ * written once); effsc_hip_profile_read_muls the summed Goldilocks products
+ * (an F_p^3 product counts 8: Karatsuba's 6 plus 2 by w). */
 int effsc_hip_profile(effsc_hip_prover* h, int enable);
 int effsc_hip_profile_read(const effsc_hip_prover* h, uint64_t* launches, double* ms, double* bytes);
+int effsc_hip_profile_read_muls(const effsc_hip_prover* h, double* muls);
 
 /* hipDeviceSynchronize on `device`. */
 int effsc_hip_synchronize(int device);
@@ -225,6 +231,10 @@ int effsc_hip_device_free(int device, uint64_t* ptr);
 int effsc_hip_copy(int device, uint64_t* dst, const uint64_t* src, size_t words);
 /* Counter-based uniform values in [0, p): dst[i] = U(seed, first + i). */
 int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint64_t seed, uint64_t first);
+/* Measured Goldilocks multiply throughput of the device (products per
+ * second over independent chains filling every CU): the compute roofline of
+ * the degree >= 3 rounds.  No reference counterpart. */
+int effsc_hip_mul_rate(int device, double* muls_per_s);
 
 #ifdef __cplusplus
 }

@@ -234,6 +234,49 @@ def test_borrowed_device_input_and_fill_uniform():
             N.lib().effsc_hip_device_free(0, p)
 
 
+@pytest.mark.parametrize("ext", [1, 3])
+def test_eq_factored_borrowed_input_and_reset(ext):
+    v = 11
+    n = 1 << v
+    aos = O.uniform(n * ext, 610 + ext)
+    wpt = O.uniform(v * ext, 620 + ext)
+    planes = np.ascontiguousarray(aos.reshape(n, ext).T).reshape(-1)  # one plane per component
+    p = N.u64p()
+    N.check(N.lib().effsc_hip_device_alloc(0, n * ext, C.byref(p)))
+    try:
+        N.check(N.lib().effsc_hip_copy(0, p, planes.ctypes.data_as(N.u64p), n * ext))
+        say(f"case eq borrowed ext={ext} v={v}")
+        g = EqFactoredProver(wpt, C.cast(p, C.c_void_p).value, ext=ext, borrow=True)
+        po = O.Prover.eq_factored(wpt, aos, ext=ext).sumcheck(v, O.Transcript.shake(b"eqb"))
+        for rep in range(2):
+            pg = sumcheck(g, v, Transcript(b"eqb", ext=ext))
+            assert tuple(as_oracle(pg, ext)) == tuple(po), f"proof {rep}"
+            g.reset()
+        # a partial proof, reset, then the full proof again
+        sumcheck(g, 3, Transcript(b"eqb", ext=ext))
+        g.reset()
+        assert tuple(as_oracle(sumcheck(g, v, Transcript(b"eqb", ext=ext)), ext)) == tuple(po)
+        g.close()
+        back = np.zeros(n * ext, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_copy(0, back.ctypes.data_as(N.u64p), p, n * ext))
+        assert np.array_equal(back, planes)
+    finally:
+        N.lib().effsc_hip_device_free(0, p)
+
+
+def test_eq_factored_borrow_requires_full_length():
+    p = N.u64p()
+    N.check(N.lib().effsc_hip_device_alloc(0, 64, C.byref(p)))
+    try:
+        cfg = effsc.provers._config(1, 1, 0, effsc.provers.W_FROM_RAW7, 0, 1)
+        h = C.c_void_p()
+        wpt = O.uniform(7, 1)
+        rc = N.lib().effsc_hip_eq_factored_new(C.byref(cfg), wpt.ctypes.data_as(N.u64p), 7, p, 64, C.byref(h))
+        assert rc == 1 and not h.value  # EINVAL: 64 < 2^7
+    finally:
+        N.lib().effsc_hip_device_free(0, p)
+
+
 def test_free_fold_matches_reference_semantics():
     for n in (2, 3, 7, 8, 1000, 1 << 15):
         vals = [int(x) for x in O.uniform(n, n)]
