@@ -56,12 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_export(uint64_t* dst, const uint64_t
 
 __global__ __launch_bounds__(kBlock) void k_fill_uniform(uint64_t* dst, size_t n, uint64_t key, uint64_t first) {
   const size_t step = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = gtid(); i < n; i += step) {
-    uint64_t base = key + (first + i) * 0x9E3779B97F4A7C15ull;
-    uint64_t c = mix64(base);
-    for (uint64_t a = 1; c >= gl::P; a++) c = mix64(base + a * 0xD1B54A32D192ED03ull);
-    dst[i] = c;
-  }
+  for (size_t i = gtid(); i < n; i += step) dst[i] = uniform_word(key, first + i);
 }
 
 }  // namespace effsc_hip
@@ -1671,10 +1666,7 @@ extern "C" int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint6
   if (!dst && n) return fail(EFFSC_HIP_EINVAL, "null destination");
   HIPCHK(hipSetDevice(device));
   if (!n) return EFFSC_HIP_OK;
-  uint64_t z = seed;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
+  uint64_t z = uniform_key(seed);
   size_t b = std::min<size_t>((n + kBlock - 1) / kBlock, 16384);
   hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)b), dim3(kBlock), 0, nullptr, dst, n, z, first);
   HIPCHK(hipGetLastError());

@@ -333,12 +333,20 @@ __global__ __launch_bounds__(kBlock) void k_fold(Views<FI, K> in, Views<FO, K> o
   }
 }
 
-// Counter-based uniform values in [0, p): first candidate below p of
-// mix(mix(seed) + i G + a K) (splitmix64 finaliser).
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+// Counter-based uniform values in [0, p): element i of seed s is the first
+// candidate below p of mix(mix(s) + i G + a K), a = 0, 1, ... (splitmix64
+// finaliser).
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t uniform_key(uint64_t seed) { return mix64(seed); }
+__device__ __forceinline__ uint64_t uniform_word(uint64_t key, uint64_t i) {
+  uint64_t base = key + i * 0x9E3779B97F4A7C15ull;
+  uint64_t c = mix64(base);
+  for (uint64_t a = 1; c >= gl::P; a++) c = mix64(base + a * 0xD1B54A32D192ED03ull);
+  return c;
 }
 
 }  // namespace effsc_hip

@@ -31,6 +31,19 @@ class Config(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    """effsc_hip_request (include/effsc_hip.h)."""
+
+    _fields_ = [
+        ("kind", C.c_int),
+        ("v", C.c_uint),
+        ("seeds", u64 * 3),
+        ("tables", u64p * 3),
+        ("transcript", vp),
+        ("priority", C.c_double),
+    ]
+
+
 HOOK = C.CFUNCTYPE(C.c_int, vp, sz, vp)
 
 _SIGS = {
@@ -75,6 +88,13 @@ _SIGS = {
     "effsc_hip_copy": (C.c_int, [C.c_int, u64p, u64p, sz]),
     "effsc_hip_fill_uniform": (C.c_int, [C.c_int, u64p, sz, u64, u64]),
     "effsc_hip_mul_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "effsc_hip_server_new": (C.c_int, [C.c_int, sz, sz, C.POINTER(vp)]),
+    "effsc_hip_server_footprint": (C.c_int, [C.POINTER(Request), szp]),
+    "effsc_hip_server_submit": (C.c_int, [vp, C.POINTER(Request), u64p]),
+    "effsc_hip_server_wait": (C.c_int, [vp, u64, u64p, u64p, u64p, C.POINTER(C.c_double)]),
+    "effsc_hip_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
+    "effsc_hip_server_now": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "effsc_hip_server_free": (None, [vp]),
 }
 
 # Entry points declared in include/effsc_hip.h (checked by the CPU tests).

@@ -225,6 +225,55 @@ void effsc_hip_batch_free(effsc_hip_batch* b);
 
 /* ------------------------------------------------------------ utilities */
 
+/* ---------------------------------------------------------------------------
+ * Served proofs: a request loop with round-level continuous batching.  No
+ * reference counterpart: each request is one runner::sumcheck
+ * (src/runner.rs:29-69) call with the no-op hook, and its proof is exactly
+ * that call's for the request alone.
+ *
+ * A scheduler thread admits requests in priority order (lower first, then
+ * submission order) while a slot and room in the server's HBM arena are free;
+ * every admitted request advances one round per tick alongside all others,
+ * with tables, transcript state and challenges resident on the device.  A
+ * finished request frees its slot and arena bytes, so requests that arrive
+ * while others run join the batch at the next tick.
+ * ------------------------------------------------------------------------- */
+typedef struct effsc_hip_server effsc_hip_server;
+
+#define EFFSC_HIP_SERVE_MULTILINEAR 1   /* MultilinearProver, base field, degree 1 */
+#define EFFSC_HIP_SERVE_INNER_PRODUCT 2 /* InnerProductProver, base field, degree 2 */
+#define EFFSC_HIP_SERVE_PRODUCT3_EXT3 3 /* CoefficientProver product of 3, base inputs, F_p^3 challenges */
+
+typedef struct {
+  int kind;                         /* EFFSC_HIP_SERVE_* */
+  unsigned v;                       /* variables: tables of 2^v elements */
+  uint64_t seeds[3];                /* table i = uniform(seeds[i]) made on the device when tables[i] is NULL */
+  const uint64_t* tables[3];        /* or caller-owned canonical device tables of 2^v words (not counted
+                                       in the footprint, kept alive until the request completes) */
+  const effsc_hip_transcript* transcript; /* its state is copied at submit */
+  double priority;                  /* admission order: lower first; ties in submission order */
+} effsc_hip_request;
+
+/* hbm_budget: bytes of the arena holding every admitted request's tables and
+ * proof words; max_active: requests on the device at once (<= 4096). */
+int effsc_hip_server_new(int device, size_t hbm_budget, size_t max_active, effsc_hip_server** out);
+/* Arena bytes a request occupies while admitted. */
+int effsc_hip_server_footprint(const effsc_hip_request* req, size_t* bytes);
+/* Queues a request; EINVAL if its footprint exceeds the budget. */
+int effsc_hip_server_submit(effsc_hip_server* s, const effsc_hip_request* req, uint64_t* id);
+/* Blocks until request `id` completes and copies its proof: round_polys
+ * (v * K * ext words, K = 1, 2, 3 values per round), challenges (v * ext),
+ * final_value (ext); times (3 doubles, server clock in seconds): submitted,
+ * admitted, completed.  Each id is waited for once. */
+int effsc_hip_server_wait(effsc_hip_server* s, uint64_t id, uint64_t* round_polys, uint64_t* challenges,
+                          uint64_t* final_value, double* times);
+int effsc_hip_server_stats(effsc_hip_server* s, uint64_t* ticks, uint64_t* peak_active, uint64_t* peak_bytes,
+                           uint64_t* completed);
+/* The server clock (seconds since effsc_hip_server_new). */
+int effsc_hip_server_now(effsc_hip_server* s, double* t);
+/* Completes every submitted request, then stops the scheduler and frees. */
+void effsc_hip_server_free(effsc_hip_server* s);
+
 /* Device buffers for callers that build tables on the device. */
 int effsc_hip_device_alloc(int device, size_t words, uint64_t** out);
 int effsc_hip_device_free(int device, uint64_t* ptr);
