@@ -363,6 +363,148 @@ def bench_batch(args):
     bp.close()
 
 
+def serve_mix(count, seed):
+    """cfg5 request mix: 40 % multilinear, 40 % inner product, 20 % product of
+    3 over F_p^3; v log-uniform in [10, 26] (ln v ~ U[ln 10, ln 26], rounded)."""
+    import math
+    import random
+
+    from effsc.serve import INNER_PRODUCT, MULTILINEAR, PRODUCT3_EXT3
+    rng = random.Random(seed)
+    out = []
+    for i in range(count):
+        u = rng.random()
+        kind = MULTILINEAR if u < 0.4 else INNER_PRODUCT if u < 0.8 else PRODUCT3_EXT3
+        v = int(round(math.exp(rng.uniform(math.log(10), math.log(26)))))
+        out.append((kind, v, [rng.getrandbits(48) for _ in range(3)]))
+    return out
+
+
+def bench_serve(args):
+    # Served proofs at a fixed offered load: requests arrive as a Poisson
+    # process (rate = --load x the measured saturated proofs/s) and join the
+    # running batch as slots and HBM arena bytes free up.  Tables are made on
+    # the device beforehand (resident inputs, borrowed by both sides); the
+    # baseline proves the same requests one by one with per-request provers.
+    import random
+    import threading
+
+    import numpy as np
+
+    import effsc
+    from effsc import _native as N
+    from effsc.batch import DeviceTable
+    from effsc.serve import INNER_PRODUCT, MULTILINEAR, PRODUCT3_EXT3, Server
+
+    K = {MULTILINEAR: 1, INNER_PRODUCT: 2, PRODUCT3_EXT3: 3}
+    EXT = {MULTILINEAR: 1, INNER_PRODUCT: 1, PRODUCT3_EXT3: 3}
+    reqs = serve_mix(args.count, 0x5EED0005)
+    tabs = [[DeviceTable.uniform(1 << v, s) for s in seeds[:K[kind]]] for kind, v, seeds in reqs]
+    addrs = [[C.cast(t.ptr, C.c_void_p).value for t in ts] for ts in tabs]
+    in_bytes = sum(t.n * 8 for ts in tabs for t in ts)
+    log(f"bench: serve {len(reqs)} requests (40/40/20 ML/IP/P3-ext3, v log-uniform in [10,26]), "
+        f"{in_bytes / 2**30:.1f} GiB of resident tables, budget {args.budget_gib} GiB, max_active {args.max_active}")
+
+    def ts(i):
+        return effsc.Transcript(b"effsc-hip/serve/%d" % i, ext=EXT[reqs[i][0]])
+
+    srv = Server(hbm_budget=int(args.budget_gib * 2**30), max_active=args.max_active)
+    # warm-up: a few requests of every kind
+    for i in range(min(6, len(reqs))):
+        srv.wait(srv.submit(reqs[i][0], reqs[i][1], ts(i), tables=addrs[i]))
+    # 1. saturation: every request submitted at once
+    t0 = srv.now()
+    ids = [srv.submit(k, v, ts(i), tables=addrs[i]) for i, (k, v, _) in enumerate(reqs)]
+    sat = [srv.wait(r) for r in ids]
+    T_sat = max(r.completed for r in sat) - t0
+    sat_rate = len(reqs) / T_sat
+    log(f"bench: saturated {sat_rate:.0f} proofs/s ({T_sat * 1e3:.1f} ms for {len(reqs)})")
+    # 2. Poisson arrivals at the offered load
+    lam = args.load * sat_rate
+    rng = random.Random(0x5EED0006)
+    gaps = [rng.expovariate(lam) for _ in reqs]
+    arrivals = np.cumsum(gaps)
+    order = list(range(len(reqs)))
+    rng.shuffle(order)
+    ids = [None] * len(reqs)
+
+    def feeder():
+        start = srv.now()
+        for slot, i in enumerate(order):
+            target = start + arrivals[slot]
+            while True:
+                d = target - srv.now()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.0005) if d > 0.0002 else 0)
+            k, v, _ = reqs[i]
+            ids[i] = srv.submit(k, v, ts(i), tables=addrs[i])
+
+    th = threading.Thread(target=feeder)
+    t1 = srv.now()
+    th.start()
+    th.join()
+    served = [srv.wait(r) for r in ids]
+    T_poi = max(r.completed for r in served) - t1
+    lat = np.array([r.latency for r in served]) * 1e3
+    by_kind = {}
+    for (k, v, _), r in zip(reqs, served):
+        by_kind.setdefault(k, []).append(r.latency * 1e3)
+    st = srv.stats()
+    srv.close()
+    same_as_sat = all(a.proof == b.proof for a, b in zip(served, sat))
+    # 3. baseline: the same requests one by one, per-request device-resident proofs
+    provers = []
+    for (k, v, _), a in zip(reqs, addrs):
+        n = 1 << v
+        if k == MULTILINEAR:
+            provers.append(effsc.MultilinearProver(a[0], borrow=True, n=n))
+        elif k == INNER_PRODUCT:
+            provers.append(effsc.InnerProductProver(a[0], a[1], borrow=True, n=n))
+        else:
+            provers.append(effsc.CoefficientProver(a, ext=3, input_ext=1, borrow=True, n=n))
+
+    def one_by_one():
+        out = []
+        for i, (p, (k, v, _)) in enumerate(zip(provers, reqs)):
+            p.reset()
+            out.append(effsc.sumcheck(p, v, ts(i)))
+        return out
+    one_by_one()
+    t2 = time.perf_counter()
+    single = one_by_one()
+    T_single = time.perf_counter() - t2
+    for p in provers:
+        p.close()
+
+    def norm(x):
+        return x if isinstance(x, int) else tuple(x)
+    identical = all(
+        [[norm(y) for y in r] for r in q.round_polys] == [[norm(y) for y in r] for r in s.proof.round_polys]
+        and norm(q.final_value) == norm(s.proof.final_value) for q, s in zip(single, served))
+    alg = sum(algorithmic_bytes(v, tables=K[k], in_words=1, work_words=EXT[k]) for k, v, _ in reqs)
+    print(json.dumps({
+        "metric": "served proofs/s and p50/p99 latency (Poisson arrivals, 40% ML / 40% IP / 20% P3-ext3, "
+                  "v log-uniform [10,26])",
+        "value": sat_rate, "unit": "proofs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": T_sat * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "served_mix", "requests": len(reqs), "budget_gib": args.budget_gib,
+                   "max_active": args.max_active, "offered_load": args.load, "seed": "0x5EED0005",
+                   "v_sampler": "round(exp(U(ln 10, ln 26)))", "inputs": "resident device tables, borrowed"},
+        "saturated_proofs_per_s": sat_rate,
+        "poisson": {"rate_offered": lam, "proofs_per_s": len(reqs) / T_poi, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                    "p50_ms_by_kind": {str(k): float(np.percentile(x, 50)) for k, x in by_kind.items()},
+                    "p99_ms_by_kind": {str(k): float(np.percentile(x, 99)) for k, x in by_kind.items()}},
+        "per_instance_launches_proofs_per_s": len(reqs) / T_single, "speedup_vs_per_instance": T_single / T_sat,
+        "bit_identical_to_per_instance": identical, "poisson_equals_saturated": same_as_sat,
+        "server": st, "gbps_reference_bytes": alg / T_sat / 1e9}), flush=True)
+    for tsl in tabs:
+        for t in tsl:
+            t.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,9 +513,12 @@ def main():
     ap.add_argument("--v", type=int, default=None, help="number of variables (default: the workload's)")
     ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch"], default="ip",
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serve"], default="ip",
                     help="ip: the v=28 inner-product proof (default, the headline line)")
-    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--count", type=int, default=512, help="batch instances / served requests")
+    ap.add_argument("--load", type=float, default=0.7, help="serve: offered load, fraction of saturated proofs/s")
+    ap.add_argument("--budget-gib", type=float, default=16.0, help="serve: HBM arena of the server")
+    ap.add_argument("--max-active", type=int, default=256, help="serve: requests on the device at once")
     ap.add_argument("--vmin", type=int, default=12)
     ap.add_argument("--vmax", type=int, default=24)
     ap.add_argument("--calibrate", action="store_true",
@@ -386,6 +531,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     return bench_single(args, args.workload)
 
 
