@@ -218,9 +218,130 @@ int enqueue_rounds(effsc_hip_batch* b) {
   return EFFSC_HIP_OK;
 }
 
+// (Re)builds the per-round chunk tables for b->chunk_units units per
+// workgroup and, when graphs are on, captures the round sequence.
+int build_plan(effsc_hip_batch* b) {
+  const size_t count = b->count;
+  if (b->stream) HIPCHK(hipStreamSynchronize(b->stream));  // no launch may still use the old tables
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (void** p : {(void**)&b->d_chunks, (void**)&b->d_act, (void**)&b->d_cbegin, (void**)&b->d_ccount,
+                   (void**)&b->d_partials}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  b->chunk_off.clear();
+  b->act_off.clear();
+  b->nchunks.clear();
+  b->nact.clear();
+  std::vector<BChunk> chunks;
+  std::vector<uint32_t> act, cb, cc;
+  for (uint32_t j = 0; j <= b->max_v; j++) {
+    b->chunk_off.push_back(chunks.size());
+    b->act_off.push_back(act.size());
+    for (size_t i = 0; i < count; i++) {
+      if (j > b->v[i]) continue;
+      size_t u = units(b->v[i], j);
+      // partials of round j are indexed by the chunk's position in that
+      // round's launch (blockIdx.x), i.e. relative to chunk_off[j]
+      uint32_t first = (uint32_t)(chunks.size() - b->chunk_off[j]);
+      for (size_t s = 0; s < u; s += b->chunk_units)
+        chunks.push_back({(uint32_t)i, (uint32_t)s, (uint32_t)std::min(b->chunk_units, u - s), 0});
+      if (j < b->v[i]) {
+        act.push_back((uint32_t)i);
+        cb.push_back(first);
+        cc.push_back((uint32_t)(chunks.size() - b->chunk_off[j] - first));
+      }
+    }
+    b->nchunks.push_back(chunks.size() - b->chunk_off.back());
+    b->nact.push_back(act.size() - b->act_off.back());
+  }
+  size_t max_chunks = 0;
+  for (size_t x : b->nchunks) max_chunks = std::max(max_chunks, x);
+  TRY(balloc((void**)&b->d_chunks, chunks.size() * sizeof(BChunk)));
+  TRY(balloc((void**)&b->d_act, act.size() * 4));
+  TRY(balloc((void**)&b->d_cbegin, act.size() * 4));
+  TRY(balloc((void**)&b->d_ccount, act.size() * 4));
+  TRY(balloc((void**)&b->d_partials, max_chunks * b->K * 8));
+  hipError_t e = hipSuccess;
+  if (!chunks.empty()) e = hipMemcpy(b->d_chunks, chunks.data(), chunks.size() * sizeof(BChunk), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_act, act.data(), act.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_cbegin, cb.data(), cb.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_ccount, cc.data(), cc.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
+  if (!b->use_graph) return EFFSC_HIP_OK;
+  // capture the round sequence once
+  e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, std::string("begin capture: ") + hipGetErrorString(e));
+  int rr = enqueue_rounds(b);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+  b->graph = g;
+  if (rr != EFFSC_HIP_OK) return rr;
+  if (e2 != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e2));
+  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+  return EFFSC_HIP_OK;
+}
+
+// Enqueues one whole batched proof from the staged transcripts.
+int run_once(effsc_hip_batch* b) {
+  HIPCHK(hipMemcpyAsync(b->d_fs, b->h_fs, b->count * sizeof(DevFs), hipMemcpyHostToDevice, b->stream));
+  if (b->use_graph)
+    HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  else
+    TRY(enqueue_rounds(b));
+  return EFFSC_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int effsc_hip_batch_autotune(effsc_hip_batch* b, const size_t* candidates, size_t ncand, int reps, size_t* chosen,
+                             double* ms_out) {
+  if (!b || !candidates || ncand == 0 || reps < 1) return set_error(EFFSC_HIP_EINVAL, "bad argument");
+  HIPCHK(hipSetDevice(b->dev));
+  // any transcript state gives the same work: time proofs with a zero sponge
+  std::memset(b->h_fs, 0, b->count * sizeof(DevFs));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  size_t best = b->chunk_units;
+  double best_ms = 1e300;
+  int rc = EFFSC_HIP_OK;
+  for (size_t c = 0; c < ncand && rc == EFFSC_HIP_OK; c++) {
+    if (candidates[c] < 1) {
+      rc = set_error(EFFSC_HIP_EINVAL, "chunk size must be >= 1");
+      break;
+    }
+    b->chunk_units = candidates[c];
+    rc = build_plan(b);
+    if (rc == EFFSC_HIP_OK) rc = run_once(b);  // warm-up
+    float ms = 0;
+    if (rc == EFFSC_HIP_OK && hipEventRecord(e0, b->stream) != hipSuccess) rc = set_error(EFFSC_HIP_EDEVICE, "event");
+    for (int r = 0; r < reps && rc == EFFSC_HIP_OK; r++) rc = run_once(b);
+    if (rc == EFFSC_HIP_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = set_error(EFFSC_HIP_EDEVICE, "autotune timing");
+    if (rc != EFFSC_HIP_OK) break;
+    double per = ms / reps;
+    if (ms_out) ms_out[c] = per;
+    if (per < best_ms) {
+      best_ms = per;
+      best = candidates[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  TRY(rc);
+  b->chunk_units = best;
+  TRY(build_plan(b));
+  if (chosen) *chosen = best;
+  return EFFSC_HIP_OK;
+}
 
 int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, const uint64_t* const* a,
                         const uint64_t* const* b_tabs, unsigned flags, effsc_hip_batch** out) {
@@ -270,37 +391,7 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
     hi[i].v = b->v[i];
     hi[i].first_round = b->first_round[i];
   }
-  // static chunk tables per round
-  std::vector<BChunk> chunks;
-  std::vector<uint32_t> act, cb, cc;
-  for (uint32_t j = 0; j <= b->max_v && rc == EFFSC_HIP_OK; j++) {
-    b->chunk_off.push_back(chunks.size());
-    b->act_off.push_back(act.size());
-    for (size_t i = 0; i < count; i++) {
-      if (j > b->v[i]) continue;
-      size_t u = units(b->v[i], j);
-      // partials of round j are indexed by the chunk's position in that
-      // round's launch (blockIdx.x), i.e. relative to chunk_off[j]
-      uint32_t first = (uint32_t)(chunks.size() - b->chunk_off[j]);
-      for (size_t s = 0; s < u; s += b->chunk_units)
-        chunks.push_back({(uint32_t)i, (uint32_t)s, (uint32_t)std::min(b->chunk_units, u - s), 0});
-      if (j < b->v[i]) {
-        act.push_back((uint32_t)i);
-        cb.push_back(first);
-        cc.push_back((uint32_t)(chunks.size() - b->chunk_off[j] - first));
-      }
-    }
-    b->nchunks.push_back(chunks.size() - b->chunk_off.back());
-    b->nact.push_back(act.size() - b->act_off.back());
-  }
-  size_t max_chunks = 0;
-  for (size_t x : b->nchunks) max_chunks = std::max(max_chunks, x);
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_insts, count * sizeof(BInst));
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_chunks, chunks.size() * sizeof(BChunk));
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_act, act.size() * 4);
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_cbegin, act.size() * 4);
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_ccount, act.size() * 4);
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_partials, max_chunks * kind * 8);
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_fs, count * sizeof(DevFs));
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_polys, std::max<size_t>(b->total_rounds, 1) * kind * 8);
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_chals, std::max<size_t>(b->total_rounds, 1) * 8);
@@ -313,33 +404,9 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
   if (rc == EFFSC_HIP_OK) {
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemcpy(b->d_insts, hi.data(), count * sizeof(BInst), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !chunks.empty())
-      e = hipMemcpy(b->d_chunks, chunks.data(), chunks.size() * sizeof(BChunk), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_act, act.data(), act.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_cbegin, cb.data(), cb.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_ccount, cc.data(), cc.size() * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = set_error(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
   }
-  if (rc == EFFSC_HIP_OK && b->use_graph) {
-    // capture the round sequence once
-    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      int rr = enqueue_rounds(b);
-      hipGraph_t g = nullptr;
-      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-      if (rr != EFFSC_HIP_OK)
-        rc = rr;
-      else if (e2 != hipSuccess)
-        rc = set_error(EFFSC_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e2));
-      b->graph = g;
-      if (rc == EFFSC_HIP_OK) {
-        e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) rc = set_error(EFFSC_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-      }
-    } else {
-      rc = set_error(EFFSC_HIP_EDEVICE, std::string("begin capture: ") + hipGetErrorString(e));
-    }
-  }
+  if (rc == EFFSC_HIP_OK) rc = build_plan(b);
   if (rc != EFFSC_HIP_OK) {
     bdestroy(b);
     return rc;
@@ -372,11 +439,7 @@ int effsc_hip_batch_prove(effsc_hip_batch* b, effsc_hip_transcript* const* trans
     }
     d.rng = t->rng;
   }
-  HIPCHK(hipMemcpyAsync(b->d_fs, b->h_fs, b->count * sizeof(DevFs), hipMemcpyHostToDevice, b->stream));
-  if (b->use_graph)
-    HIPCHK(hipGraphLaunch(b->exec, b->stream));
-  else
-    TRY(enqueue_rounds(b));
+  TRY(run_once(b));
   if (b->total_rounds) {
     HIPCHK(hipMemcpyAsync(round_polys, b->d_polys, b->total_rounds * b->K * 8, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipMemcpyAsync(challenges, b->d_chals, b->total_rounds * 8, hipMemcpyDeviceToHost, b->stream));

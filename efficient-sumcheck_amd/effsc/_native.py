@@ -82,6 +82,7 @@ _SIGS = {
                                       C.POINTER(vp)]),
     "effsc_hip_batch_rounds": (C.c_int, [vp, sz, szp, szp]),
     "effsc_hip_batch_prove": (C.c_int, [vp, C.POINTER(vp), u64p, u64p, u64p]),
+    "effsc_hip_batch_autotune": (C.c_int, [vp, szp, sz, C.c_int, szp, C.POINTER(C.c_double)]),
     "effsc_hip_batch_free": (None, [vp]),
     "effsc_hip_device_alloc": (C.c_int, [C.c_int, sz, C.POINTER(u64p)]),
     "effsc_hip_device_free": (C.c_int, [C.c_int, u64p]),
@@ -95,6 +96,14 @@ _SIGS = {
     "effsc_hip_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
     "effsc_hip_server_now": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "effsc_hip_server_free": (None, [vp]),
+    "effsc_hip_mlp_new": (C.c_int, [C.c_int, C.c_int, C.c_int, u64, C.POINTER(vp)]),
+    "effsc_hip_mlp_param_count": (C.c_int, [vp, szp]),
+    "effsc_hip_mlp_get_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "effsc_hip_mlp_set_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "effsc_hip_mlp_forward": (C.c_int, [vp, C.POINTER(C.c_float), sz, C.POINTER(C.c_float)]),
+    "effsc_hip_mlp_loss_grad": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), sz, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "effsc_hip_mlp_train": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), sz, C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    "effsc_hip_mlp_free": (None, [vp]),
 }
 
 # Entry points declared in include/effsc_hip.h (checked by the CPU tests).

@@ -68,15 +68,32 @@ class BatchProver:
             self.first.append(f.value)
         self.total = sum(self.v)
 
+    def autotune(self, candidates=(256, 512, 1024, 2048, 4096, 8192, 16384, 65536), reps=2):
+        """Picks the units-per-workgroup tile by timing whole proofs;
+        returns (chosen, {candidate: ms per proof})."""
+        cand = (C.c_size_t * len(candidates))(*candidates)
+        ms = (C.c_double * len(candidates))()
+        chosen = C.c_size_t()
+        N.check(N.lib().effsc_hip_batch_autotune(self.h, cand, len(candidates), reps, C.byref(chosen), ms))
+        return chosen.value, dict(zip(candidates, list(ms)))
+
+    def buffers(self):
+        """Output words of one batched proof: (round polys, challenges, finals)."""
+        return (np.zeros(max(self.total, 1) * self.kind, dtype=np.uint64),
+                np.zeros(max(self.total, 1), dtype=np.uint64), np.zeros(self.count, dtype=np.uint64))
+
+    def prove_raw(self, handles, polys, chals, fins):
+        """One batched proof into preallocated word arrays; `handles` is a
+        ctypes array of native transcript handles (states advanced)."""
+        N.check(N.lib().effsc_hip_batch_prove(self.h, handles, polys.ctypes.data_as(N.u64p),
+                                              chals.ctypes.data_as(N.u64p), fins.ctypes.data_as(N.u64p)))
+
     def prove(self, transcripts):
         """transcripts: one native transcript per instance (state advanced)."""
         K = self.kind
         ts = (C.c_void_p * self.count)(*[t.h for t in transcripts])
-        polys = np.zeros(max(self.total, 1) * K, dtype=np.uint64)
-        chals = np.zeros(max(self.total, 1), dtype=np.uint64)
-        fins = np.zeros(self.count, dtype=np.uint64)
-        N.check(N.lib().effsc_hip_batch_prove(self.h, ts, polys.ctypes.data_as(N.u64p), chals.ctypes.data_as(N.u64p),
-                                              fins.ctypes.data_as(N.u64p)))
+        polys, chals, fins = self.buffers()
+        self.prove_raw(ts, polys, chals, fins)
         out = []
         for i in range(self.count):
             f, v = self.first[i], self.v[i]

@@ -221,6 +221,13 @@ int effsc_hip_batch_rounds(const effsc_hip_batch* b, size_t instance, size_t* v,
  * challenges: sum_i v_i words; finals: count words.  Canonical words. */
 int effsc_hip_batch_prove(effsc_hip_batch* b, effsc_hip_transcript* const* transcripts, uint64_t* round_polys,
                           uint64_t* challenges, uint64_t* finals);
+/* Tile autotuner: times `reps` whole batched proofs for each candidate
+ * number of units (pairs in round 0, quadruples later) per workgroup, keeps
+ * the fastest (its chunk tables and graph are rebuilt) and reports it in
+ * `chosen`; ms_out (ncand doubles, may be NULL) receives each candidate's time
+ * per proof.  Proofs are the same for every tile size. */
+int effsc_hip_batch_autotune(effsc_hip_batch* b, const size_t* candidates, size_t ncand, int reps, size_t* chosen,
+                             double* ms_out);
 void effsc_hip_batch_free(effsc_hip_batch* b);
 
 /* ------------------------------------------------------------ utilities */
@@ -273,6 +280,28 @@ int effsc_hip_server_stats(effsc_hip_server* s, uint64_t* ticks, uint64_t* peak_
 int effsc_hip_server_now(effsc_hip_server* s, double* t);
 /* Completes every submitted request, then stops the scheduler and frees. */
 void effsc_hip_server_free(effsc_hip_server* s);
+
+/* ---------------------------------------------------------------------------
+ * Learned service-time model for serving admission.  No reference
+ * counterpart; proofs never depend on it (it only orders admissions).
+ * A fp32 network x (inputs) -> tanh (hidden) -> tanh (hidden) -> y, trained
+ * on the device with its own forward / backward / Adam kernels on the mean
+ * squared error.  Parameters are one flat array: W1 [hidden][inputs], b1,
+ * W2 [hidden][hidden], b2, w3 [hidden], b3.  x is row-major [n][inputs].
+ * ------------------------------------------------------------------------- */
+typedef struct effsc_hip_mlp effsc_hip_mlp;
+int effsc_hip_mlp_new(int device, int inputs, int hidden, uint64_t seed, effsc_hip_mlp** out);
+int effsc_hip_mlp_param_count(const effsc_hip_mlp* m, size_t* out);
+int effsc_hip_mlp_get_params(effsc_hip_mlp* m, float* out);
+/* Also resets the optimiser state. */
+int effsc_hip_mlp_set_params(effsc_hip_mlp* m, const float* in);
+int effsc_hip_mlp_forward(effsc_hip_mlp* m, const float* x, size_t n, float* y);
+/* Mean squared error over the n samples and its gradient (param_count floats). */
+int effsc_hip_mlp_loss_grad(effsc_hip_mlp* m, const float* x, const float* t, size_t n, float* loss, float* grad);
+/* `steps` full-batch Adam steps at learning rate lr; the loss afterwards. */
+int effsc_hip_mlp_train(effsc_hip_mlp* m, const float* x, const float* t, size_t n, int steps, float lr,
+                        float* final_loss);
+void effsc_hip_mlp_free(effsc_hip_mlp* m);
 
 /* Device buffers for callers that build tables on the device. */
 int effsc_hip_device_alloc(int device, size_t words, uint64_t** out);

@@ -51,3 +51,21 @@ def test_batch_equals_single_prover_at_larger_sizes():
                           Transcript(b"big-%d" % i))
         assert proofs[i].round_polys == [list(r) for r in single.round_polys]
         assert proofs[i].final_value == single.final_value
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_autotuned_batch_matches_oracle(graph):
+    vs = _instances(16, 0, 11, 4321)
+    a = [DeviceTable.uniform(1 << v, 0xD0000 + i) for i, v in enumerate(vs)]
+    b = [DeviceTable.uniform(1 << v, 0xD1000 + i) for i, v in enumerate(vs)]
+    bp = BatchProver(a, b, kind=2, graph=graph)
+    print(f"autotune graph={graph}", flush=True)
+    chosen, times = bp.autotune(candidates=(1, 7, 64, 2048), reps=1)
+    assert chosen in (1, 7, 64, 2048) and all(t > 0 for t in times.values())
+    assert times[chosen] == min(times.values())
+    proofs = bp.prove([Transcript(b"tuned-%d" % i) for i in range(len(vs))])
+    for i, v in enumerate(vs):
+        o = O.Prover.inner_product(O.uniform(1 << v, 0xD0000 + i), O.uniform(1 << v, 0xD1000 + i))
+        rp, ch, fin = o.sumcheck(v, O.Transcript.shake(b"tuned-%d" % i))
+        assert proofs[i].round_polys == [[x[0] for x in r] for r in rp], f"instance {i} v={v}"
+        assert proofs[i].final_value == fin[0]
