@@ -477,6 +477,40 @@ def bench_serve(args):
     st = srv.stats()
     srv.close()
     same_as_sat = all(a.proof == b.proof for a, b in zip(served, sat))
+    # 2b. learned admission: a service-time network trained on the GPU from
+    # requests measured alone; a server limited to --learn-slots requests at
+    # once admits a burst of every request shortest-predicted first, against
+    # first come first served
+    from effsc.learn import ServiceTimeModel, measure_service_times
+    grid = [(k, v) for k in (MULTILINEAR, INNER_PRODUCT, PRODUCT3_EXT3) for v in range(10, 27)]
+    cal = Server(hbm_budget=int(args.budget_gib * 2**30), max_active=1)
+    samples = measure_service_times(cal, grid, lambda i, k: effsc.Transcript(b"cal", ext=EXT[k]))
+    cal.close()
+    model = ServiceTimeModel(hidden=32, seed=3)
+    t_fit = time.perf_counter()
+    fit_loss = model.fit(samples, steps=3000, lr=1e-2)
+    t_fit = time.perf_counter() - t_fit
+    prio = [model.priority(k, v) for k, v, _ in reqs]
+    lim = Server(hbm_budget=int(args.budget_gib * 2**30), max_active=args.learn_slots)
+
+    def burst(prios):
+        s0 = lim.now()
+        rs = [lim.submit(k, v, ts(i), tables=addrs[i], priority=pr) for i, ((k, v, _), pr) in enumerate(zip(reqs, prios))]
+        out = [lim.wait(r) for r in rs]
+        return out, max(r.completed for r in out) - s0
+    burst([0.0] * len(reqs))  # warm-up
+    fifo, T_fifo = burst([0.0] * len(reqs))
+    lrn, T_lrn = burst(prio)
+    lim.close()
+
+    def lat_stats(rs):
+        x = np.array([r.latency for r in rs]) * 1e3
+        return {"mean_ms": float(x.mean()), "p50_ms": float(np.percentile(x, 50)), "p99_ms": float(np.percentile(x, 99))}
+    learned = {"slots": args.learn_slots, "calibration_requests": len(samples), "fit_loss_log_ms": fit_loss,
+               "train_s": t_fit, "fifo": dict(lat_stats(fifo), proofs_per_s=len(reqs) / T_fifo),
+               "learned": dict(lat_stats(lrn), proofs_per_s=len(reqs) / T_lrn),
+               "identical_proofs": all(a.proof == b.proof for a, b in zip(fifo, lrn))}
+    log(f"bench: learned admission mean {learned['learned']['mean_ms']:.2f} ms vs FIFO {learned['fifo']['mean_ms']:.2f} ms")
     # 3. baseline: the same requests one by one, per-request device-resident proofs
     provers = []
     for (k, v, _), a in zip(reqs, addrs):
@@ -523,7 +557,7 @@ def bench_serve(args):
                     "p99_ms_by_kind": {str(k): float(np.percentile(x, 99)) for k, x in by_kind.items()}},
         "per_instance_launches_proofs_per_s": len(reqs) / T_single, "speedup_vs_per_instance": T_single / T_sat,
         "bit_identical_to_per_instance": identical, "poisson_equals_saturated": same_as_sat,
-        "server": st, "gbps_reference_bytes": alg / T_sat / 1e9}), flush=True)
+        "server": st, "gbps_reference_bytes": alg / T_sat / 1e9, "learned_admission": learned}), flush=True)
     for tsl in tabs:
         for t in tsl:
             t.free()
@@ -543,6 +577,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7, help="serve: offered load, fraction of saturated proofs/s")
     ap.add_argument("--budget-gib", type=float, default=16.0, help="serve: HBM arena of the server")
     ap.add_argument("--max-active", type=int, default=256, help="serve: requests on the device at once")
+    ap.add_argument("--learn-slots", type=int, default=16,
+                    help="serve: max_active of the server comparing learned and FIFO admission")
     ap.add_argument("--vmin", type=int, default=12)
     ap.add_argument("--vmax", type=int, default=24)
     ap.add_argument("--calibrate", action="store_true",

@@ -225,3 +225,22 @@ def claimed_sum_product(F, tables, lift=True):
             term = F.mul(term, T[i])
         acc = F.add(acc, term)
     return acc
+
+
+def prove_gkr(F, add, mult, wit, transcript, lift=True):
+    """GKR round sumcheck from its definition: the summand is
+    A(z) (Wb(z) + Wc(z)) + M(z) Wb(z) Wc(z) over the multilinear extensions
+    of the (zero-padded) tables add, mult, w_b[b n + c] = W(b) and
+    w_c[b n + c] = W(c); the wire's second value is the leading coefficient
+    of the degree-2 interpolant through q(0), q(1), q(2)."""
+    n = len(wit)
+    nbc = n * n
+    v = (nbc - 1).bit_length() if nbc > 1 else 0
+    wb = [wit[i // n] for i in range(nbc)]
+    wc = [wit[i % n] for i in range(nbc)]
+    A, M, B, Cc = [_pad(t, v, F, lift) for t in (add, mult, wb, wc)]
+
+    def summand(z):
+        a, m, b, c = mle(F, A, z), mle(F, M, z), mle(F, B, z), mle(F, Cc, z)
+        return F.add(F.mul(a, F.add(b, c)), F.mul(m, F.mul(b, c)))
+    return prove(F, summand, v, 2, transcript)

@@ -481,6 +481,31 @@ or_prover *or_prover_new_eq(int ext, uint64_t w, const uint64_t *point, size_t v
     return p;
 }
 
+/* GkrProver::new: src/provers/gkr.rs:58-87 (the witness is expanded into
+ * broadcast tables w_b[b n + c] = W(b), w_c[b n + c] = W(c)) */
+or_prover *or_prover_new_gkr(int ext, uint64_t w, const uint64_t *add, const uint64_t *mult, const uint64_t *wit,
+                             size_t n) {
+    or_prover *p = alloc_prover(OR_GKR, ext, w);
+    if (!p) return NULL;
+    const fctx *f = &p->f;
+    size_t nbc = n * n;
+    p->deg = 2;
+    p->ntab = 4;
+    p->tab[0] = load_table(f, add, nbc, 0);
+    p->tab[1] = load_table(f, mult, nbc, 0);
+    el *wv = load_table(f, wit, n, 0);
+    p->tab[2] = (el *)malloc(sizeof(el) * (nbc ? nbc : 1));
+    p->tab[3] = (el *)malloc(sizeof(el) * (nbc ? nbc : 1));
+    for (size_t b = 0; b < n; b++)
+        for (size_t c = 0; c < n; c++) {
+            p->tab[2][b * n + c] = wv[b];
+            p->tab[3][b * n + c] = wv[c];
+        }
+    free(wv);
+    for (int t = 0; t < 4; t++) p->len[t] = nbc;
+    return p;
+}
+
 int or_prover_degree(const or_prover *p) { return p->deg; }
 
 /* Horner: eval_poly_at, src/provers/coefficient.rs:249-259 */
@@ -613,8 +638,47 @@ static void round_eq(or_prover *p, el *out) {
     }
 }
 
+/* GKR gate polynomial at one point: add (wb + wc) + mult wb wc */
+static inline el gkr_f(const fctx *f, el a, el m, el wb, el wc) {
+    return e_add(f, e_mul(f, a, e_add(f, wb, wc)), e_mul(f, m, e_mul(f, wb, wc)));
+}
+
+static void round_gkr(or_prover *p, el *out) {
+    /* src/provers/gkr.rs:104-182: q(0), q(1), q(2) over the half-split pairs
+     * (unpaired tail: hi = 0), then q(inf) = (q(0) + q(2) - 2 q(1)) / 2 */
+    const fctx *f = &p->f;
+    size_t n = p->len[0];
+    el *A = p->tab[0], *M = p->tab[1], *WB = p->tab[2], *WC = p->tab[3];
+    if (n <= 1) {
+        out[0] = n == 1 ? gkr_f(f, A[0], M[0], WB[0], WC[0]) : e_zero();
+        out[1] = e_zero();
+        return;
+    }
+    size_t half = next_pow2(n) >> 1;
+    size_t paired = n - half;
+    el q0 = e_zero(), q1 = e_zero(), q2 = e_zero();
+    for (size_t i = 0; i < paired; i++) {
+        el al = A[i], ah = A[half + i], ml = M[i], mh = M[half + i];
+        el bl = WB[i], bh = WB[half + i], cl = WC[i], ch = WC[half + i];
+        q0 = e_add(f, q0, gkr_f(f, al, ml, bl, cl));
+        q1 = e_add(f, q1, gkr_f(f, ah, mh, bh, ch));
+        el a2 = e_sub(f, e_add(f, ah, ah), al), m2 = e_sub(f, e_add(f, mh, mh), ml);
+        el b2 = e_sub(f, e_add(f, bh, bh), bl), c2 = e_sub(f, e_add(f, ch, ch), cl);
+        q2 = e_add(f, q2, gkr_f(f, a2, m2, b2, c2));
+    }
+    for (size_t i = paired; i < half; i++) {
+        el al = A[i], ml = M[i], bl = WB[i], cl = WC[i];
+        q0 = e_add(f, q0, gkr_f(f, al, ml, bl, cl));
+        q2 = e_add(f, q2, e_sub(f, e_mul(f, al, e_add(f, bl, cl)), e_mul(f, ml, e_mul(f, bl, cl))));
+    }
+    el two = e_add(f, e_one(), e_one());
+    out[0] = q0;
+    out[1] = e_mul(f, e_sub(f, e_add(f, q0, q2), e_add(f, q1, q1)), e_inv(f, two));
+}
+
 static void fold_all(or_prover *p, el c) {
     switch (p->kind) {
+    case OR_GKR: /* src/provers/gkr.rs:93-101, 185-190 */
     case OR_ML:
     case OR_IP:
         for (int t = 0; t < p->ntab; t++) fold_msb(&p->f, p->tab[t], &p->len[t], c);
@@ -644,6 +708,10 @@ void or_prover_round(or_prover *p, const uint64_t *chal, uint64_t *out) {
     case OR_PRODUCT: /* src/provers/coefficient.rs:272-276 */
         if (chal) fold_all(p, c);
         round_product(p, res);
+        break;
+    case OR_GKR:
+        if (chal) fold_all(p, c);
+        round_gkr(p, res);
         break;
     case OR_EQ: /* src/provers/eq_factored.rs:249-270 */
         if (chal) {
@@ -705,6 +773,9 @@ void or_prover_final_value(const or_prover *p, uint64_t *out) {
     }
     case OR_EQ: /* src/provers/eq_factored.rs:282-288 */
         if (p->len[0] == 1) r = e_mul(f, e_mul(f, p->tab[0][0], p->tab[1][0]), p->tab[2][0]);
+        break;
+    case OR_GKR: /* src/provers/gkr.rs:192-201 */
+        if (p->len[0] == 1) r = gkr_f(f, p->tab[0][0], p->tab[1][0], p->tab[2][0], p->tab[3][0]);
         break;
     }
     e_store(f, r, out);

@@ -57,7 +57,7 @@ void or_tr_challenge(or_transcript *t, int d, uint64_t *out);
 void or_tr_free(or_transcript *t);
 
 /* ---- provers ---- */
-enum { OR_ML = 1, OR_IP = 2, OR_PRODUCT = 3, OR_EQ = 4 };
+enum { OR_ML = 1, OR_IP = 2, OR_PRODUCT = 3, OR_EQ = 4, OR_GKR = 5 };
 typedef struct or_prover or_prover;
 /* ext: extension degree 1..3, w: canonical non-residue (ignored for ext=1) */
 or_prover *or_prover_new_ml(int ext, uint64_t w, const uint64_t *evals, size_t n);
@@ -69,6 +69,10 @@ or_prover *or_prover_new_product(int ext, uint64_t w, int k, const uint64_t *con
 /* Same, but each table already holds n extension elements (ext words each). */
 or_prover *or_prover_new_product_ext(int ext, uint64_t w, int k, const uint64_t *const *tables, size_t n);
 or_prover *or_prover_new_eq(int ext, uint64_t w, const uint64_t *point, size_t v, const uint64_t *p, size_t n);
+/* GkrProver: add and mult hold n*n elements over (b, c), wit holds n
+ * elements W(x); tables 0..3 are add, mult, w_b, w_c (ext words each). */
+or_prover *or_prover_new_gkr(int ext, uint64_t w, const uint64_t *add, const uint64_t *mult, const uint64_t *wit,
+                             size_t n);
 int or_prover_degree(const or_prover *p);
 void or_prover_round(or_prover *p, const uint64_t *chal_or_null, uint64_t *out);
 void or_prover_finalize(or_prover *p, const uint64_t *chal);

@@ -62,6 +62,7 @@ def lib():
             "or_prover_new_product": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_new_product_ext": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_new_eq": (vp, [C.c_int, u64, u64p, sz, u64p, sz]),
+            "or_prover_new_gkr": (vp, [C.c_int, u64, u64p, u64p, u64p, sz]),
             "or_prover_degree": (C.c_int, [vp]),
             "or_prover_round": (None, [vp, u64p, u64p]),
             "or_prover_finalize": (None, [vp, u64p]),
@@ -181,6 +182,20 @@ class Prover:
         W, pw = _arr(point)
         Pv, pp = _arr(p_evals)
         return cls(lib().or_prover_new_eq(ext, w, pw, len(W) // ext, pp, len(Pv) // ext), ext)
+
+    @classmethod
+    def gkr(cls, add, mult, wit, ext=1, w=W_FROM_RAW7):
+        """GkrProver::new(add_evals, mult_evals, w_evals), src/provers/gkr.rs:58-87."""
+        A, pa = _arr(add)
+        M, pm = _arr(mult)
+        Wt, pw = _arr(wit)
+        return cls(lib().or_prover_new_gkr(ext, w, pa, pm, pw, len(Wt) // ext), ext)
+
+    def claimed_w_values(self):
+        """(W(b*), W(c*)) after the full proof, src/provers/gkr.rs:89-97."""
+        if len(self.table(2)) // self.ext == 1:
+            return self.table(2), self.table(3)
+        return None
 
     def degree(self):
         return lib().or_prover_degree(self.h)

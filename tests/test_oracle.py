@@ -251,3 +251,52 @@ def test_plonky3_style_counter_closed_form():
     pr.finalize([5])
     assert out[0] == [[36]]
     assert pr.final_value() == [42]
+
+
+# ---------------------------------------------------------------- GKR
+@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+def test_gkr_oracle_matches_brute_force(ext, n):
+    # src/provers/gkr.rs restated in C against the definition (tails included)
+    add = O.uniform(n * n * ext, 70 + n)
+    mult = O.uniform(n * n * ext, 80 + n)
+    wit = O.uniform(n * ext, 90 + n)
+    F = B.Ext(ext)
+
+    def els(words):
+        return [tuple(int(x) for x in words[i * ext:(i + 1) * ext]) for i in range(len(words) // ext)]
+    v = (n * n - 1).bit_length() if n > 1 else 0
+    rp, ch, fin = O.Prover.gkr(add, mult, wit, ext=ext).sumcheck(v, O.Transcript.shake(b"gkr"))
+    brp, bch, bfin = B.prove_gkr(F, els(add), els(mult), els(wit), B.ShakeTranscript(b"gkr"), lift=False)
+    assert [[list(x) for x in r] for r in brp] == rp
+    assert [list(c) for c in bch] == ch
+    assert list(bfin) == fin
+
+
+def test_gkr_reference_identities():
+    # src/provers/gkr.rs:208-266 (claim chain, claimed W values = MLE of W at
+    # the b and c halves of the challenges) and :291-316 (zero gates)
+    for k in (1, 2, 3, 4):
+        n = 1 << k
+        add, mult, wit = O.uniform(n * n, 0x100 * k), O.uniform(n * n, 0x100 * k + 1), O.uniform(n, 0x100 * k + 2)
+        claim = 0
+        for b in range(n):
+            for c in range(n):
+                i = b * n + c
+                wb, wc = int(wit[b]), int(wit[c])
+                claim = (claim + int(add[i]) * (wb + wc) + int(mult[i]) * wb * wc) % P
+        p = O.Prover.gkr(add, mult, wit)
+        rp, ch, fin = p.sumcheck(2 * k, O.Transcript.rng(99))
+        for (q0, qi), (r,) in zip(rp, ch):
+            q0, qi = q0[0], qi[0]
+            q1 = (claim - q0) % P
+            claim = (q0 + r * (q1 - q0 - qi) + qi * r * r) % P
+        assert claim == fin[0]
+        wb, wc = p.claimed_w_values()
+        F = B.Ext(1)
+        T = [(int(x),) for x in wit]
+        assert int(wb[0]) == B.mle(F, T, [(c[0],) for c in ch[:k]])[0]
+        assert int(wc[0]) == B.mle(F, T, [(c[0],) for c in ch[k:]])[0]
+    z = np.zeros(16, dtype=np.uint64)
+    rp, _, _ = O.Prover.gkr(z, z, O.uniform(4, 0x500)).sumcheck(4, O.Transcript.rng(99))
+    assert all(x == [0] for r in rp for x in r)
