@@ -65,7 +65,7 @@ namespace {
 
 int fail(int code, const std::string& msg) { return effsc_hip::set_error(code, msg); }
 
-enum Kind { K_ML = 1, K_IP = 2, K_PRODUCT = 3, K_EQ = 4 };
+enum Kind { K_ML = 1, K_IP = 2, K_PRODUCT = 3, K_EQ = 4, K_GKR = 5 };
 
 size_t next_pow2(size_t n) {
   size_t p = 1;
@@ -465,6 +465,8 @@ void to_wire(const effsc_hip_prover* h, const uint64_t* vals, uint64_t* out) {
 int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
 int eq_finalize(effsc_hip_prover* h, const uint64_t* chal);
 int eq_final_value(effsc_hip_prover* h, uint64_t* out);
+int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
+int gkr_final_value(effsc_hip_prover* h, uint64_t* out);
 
 // ---------------------------------------------------------- two-round passes
 template <int K, int P>
@@ -566,6 +568,7 @@ int round_twostep(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
 
 int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   if (h->kind == K_EQ) return eq_round(h, chal, out);
+  if (h->kind == K_GKR) return gkr_round(h, chal, out);
   if (h->twostep) return round_twostep(h, chal, out);
   bool evaluated = false;
   if (chal) TRY(fold_step(h, chal, true, &evaluated));
@@ -585,6 +588,7 @@ int finalize_impl(effsc_hip_prover* h, const uint64_t* chal) {
 
 int final_value_impl(effsc_hip_prover* h, uint64_t* out) {
   if (h->kind == K_EQ) return eq_final_value(h, out);
+  if (h->kind == K_GKR) return gkr_final_value(h, out);
   TRY(materialize(h));
   const int E = h->ext;
   std::fill(out, out + E, 0);
@@ -962,12 +966,132 @@ int build_eq(effsc_hip_prover* h, const uint64_t* wpt_canon_dev, size_t off, siz
 }  // namespace
 
 // ===========================================================================
+// GkrProver: f(b, c) = add(b, c) (W(b) + W(c)) + mult(b, c) W(b) W(c) over
+// (b, c) in {0,1}^{2k} (src/provers/gkr.rs:38-202).  Four tables of n^2
+// elements as in the reference: add, mult and the witness broadcast over the
+// hypercube, w_b[b n + c] = W(b) and w_c[b n + c] = W(c) (built on the device
+// from W).  Every round folds the four tables with ip::fold semantics
+// (k_fold, K = 4) and evaluates q(0), q(1), q(2) over the half-split pairs
+// (k_gkr_eval); the wire's second value is q(inf) = (q(0) + q(2) - 2 q(1)) / 2
+// (src/provers/gkr.rs:176-181).
+// ===========================================================================
+namespace {
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_gkr_broadcast(View<F> wit, size_t n, View<F> wb, View<F> wc) {
+  const size_t nbc = n * n;
+  for (size_t i = gtid(); i < nbc; i += (size_t)gridDim.x * blockDim.x) {
+    wb.st(i, wit.ld(i / n));
+    wc.st(i, wit.ld(i % n));
+  }
+}
+
+template <class F>
+__host__ __device__ __forceinline__ F gkr_f(F a, F m, F b, F c, uint64_t w) {
+  return gl::add(gl::mul(a, gl::add(b, c), w), gl::mul(m, gl::mul(b, c, w), w));
+}
+
+// q(0), q(1), q(2) of the current tables; pairs k >= paired have hi = 0
+// (src/provers/gkr.rs:121-171).
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_gkr_eval(Views<F, 4> t, size_t half, size_t paired, uint64_t w,
+                                                     Reduce r) {
+  F acc[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < half; k += step) {
+    F lo[4], hi[4], x2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      lo[j] = t.t[j].ld(k);
+      hi[j] = k < paired ? t.t[j].ld(k + half) : gl::zero<F>();
+      x2[j] = gl::sub(gl::add(hi[j], hi[j]), lo[j]);
+    }
+    acc[0] = gl::add(acc[0], gkr_f(lo[0], lo[1], lo[2], lo[3], w));
+    acc[1] = gl::add(acc[1], gkr_f(hi[0], hi[1], hi[2], hi[3], w));
+    acc[2] = gl::add(acc[2], gkr_f(x2[0], x2[1], x2[2], x2[3], w));
+  }
+  finish<F, 3>(acc, r);
+}
+
+// f at element 0 of the four tables (lifted words), as ext words.
+void gkr_f_host(const effsc_hip_prover* h, const uint64_t* el, uint64_t* out) {
+  if (h->ext == 1) {
+    out[0] = gkr_f<Fp>(Fp{el[0]}, Fp{el[1]}, Fp{el[2]}, Fp{el[3]}, 0).v;
+  } else {
+    Fp3 v = gkr_f<Fp3>(from_words<Fp3>(el), from_words<Fp3>(el + 3), from_words<Fp3>(el + 6),
+                       from_words<Fp3>(el + 9), h->w);
+    gl::store_words(v, out);
+  }
+}
+
+int gkr_eval(effsc_hip_prover* h, uint64_t* out) {
+  const int E = h->ext;
+  std::fill(out, out + 2 * E, 0);
+  const size_t L = h->len;
+  if (L <= 1) {
+    if (L == 1) {
+      uint64_t el[kMaxK * 3];
+      TRY(read_elements(h, 0, el));
+      gkr_f_host(h, el, out);
+    }
+    return EFFSC_HIP_OK;
+  }
+  size_t half, paired;
+  geometry(h, L, &half, &paired);
+  const int fi = src_field(h);
+  uint64_t* const* src = src_planes(h);
+  const size_t sst = src_stride(h);
+  const unsigned g = blocks_for(h, half);
+  if (fi == 1) {
+    hipLaunchKernelGGL(k_gkr_eval<Fp>, dim3(g), dim3(kBlock), 0, h->stream, views<Fp, 4>(src, sst), half, paired,
+                       h->w, h->red);
+    HIPCHK(hipGetLastError());
+    TRY((reduce<Fp, 3>(h, g)));
+  } else {
+    hipLaunchKernelGGL(k_gkr_eval<Fp3>, dim3(g), dim3(kBlock), 0, h->stream, views<Fp3, 4>(src, sst), half, paired,
+                       h->w, h->red);
+    HIPCHK(hipGetLastError());
+    TRY((reduce<Fp3, 3>(h, g)));
+  }
+  uint64_t q[3 * 3];
+  TRY(fetch_result(h, fi, q, 3));  // lifted to E words each
+  const uint64_t inv2 = (gl::P + 1) / 2;
+  for (int c = 0; c < E; c++) {
+    uint64_t q0 = q[c], q1 = q[E + c], q2 = q[2 * E + c];
+    out[c] = q0;
+    out[E + c] = gl::mul(gl::sub(gl::add(q0, q2), gl::add(q1, q1)), inv2);
+  }
+  return EFFSC_HIP_OK;
+}
+
+int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
+  // src/provers/gkr.rs:93-101: fold all four tables with the challenge
+  if (chal) {
+    bool ev = false;
+    TRY(fold_step(h, chal, false, &ev));
+  }
+  return gkr_eval(h, out);
+}
+
+int gkr_final_value(effsc_hip_prover* h, uint64_t* out) {
+  // src/provers/gkr.rs:192-201
+  std::fill(out, out + h->ext, 0);
+  if (h->len != 1) return EFFSC_HIP_OK;
+  uint64_t el[kMaxK * 3];
+  TRY(read_elements(h, 0, el));
+  gkr_f_host(h, el, out);
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
 // Device-resident proof loop (effsc_hip_sumcheck with no hook)
 // ===========================================================================
 namespace {
 
 bool device_loop_applies(const effsc_hip_prover* h, size_t num_rounds) {
-  if (h->kind == K_EQ || num_rounds == 0 || num_rounds > kMaxRounds) return false;
+  if (h->kind == K_EQ || h->kind == K_GKR || num_rounds == 0 || num_rounds > kMaxRounds) return false;
   if (!is_pow2(h->len)) return false;
   size_t logl = 0;
   while (((size_t)1 << logl) < h->len) logl++;
@@ -1163,6 +1287,85 @@ int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* ta
   if (!tables) return fail(EFFSC_HIP_EINVAL, "null tables");
   if (n && !is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "product prover tables must have power-of-two length");
   return new_product_like(cfg, K_PRODUCT, k, k, tables, n, out);
+}
+
+int effsc_hip_gkr_new(const effsc_hip_config* cfg, const uint64_t* add, const uint64_t* mult, const uint64_t* wit,
+                      size_t n, effsc_hip_prover** out) {
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  *out = nullptr;
+  TRY(check_cfg(cfg));
+  if (cfg->flags & EFFSC_HIP_BORROW_INPUT) return fail(EFFSC_HIP_EINVAL, "the GKR prover copies its inputs");
+  if (n > ((size_t)1 << 20)) return fail(EFFSC_HIP_EINVAL, "witness too long");
+  if (n && (!add || !mult || !wit)) return fail(EFFSC_HIP_EINVAL, "null table");
+  const size_t nbc = n * n;
+  auto* h = new effsc_hip_prover();
+  h->kind = K_GKR;
+  h->ext = cfg->ext;
+  h->in_ext = cfg->input_ext;
+  h->K = 4;
+  h->deg = 2;
+  h->repr = cfg->repr;
+  h->dev = cfg->device;
+  h->w = cfg->ext == 3 ? gl::canon(cfg->ext_w) : 0;
+  h->n = nbc;
+  h->len = nbc;
+  int rc = setup_common(h, next_pow2(std::max<size_t>(nbc, 1)) >> 1);
+  // add and mult as tables 0, 1 (src/provers/gkr.rs:58-63)
+  const uint64_t* gates[2] = {add, mult};
+  if (rc == EFFSC_HIP_OK) {
+    h->K = 2;
+    rc = import_tables(h, cfg, gates, nbc);
+    h->K = 4;
+  }
+  // the witness, then its broadcast tables 2, 3 (src/provers/gkr.rs:65-76)
+  const int d = h->in_ext;
+  uint64_t *staging = nullptr, *wplanes = nullptr;
+  for (int i = 2; i < 4 && rc == EFFSC_HIP_OK; i++) rc = alloc_dev(&h->in[i], h->in_stride * d);
+  if (rc == EFFSC_HIP_OK && n) rc = alloc_dev(&staging, n * d);
+  if (rc == EFFSC_HIP_OK && n) rc = alloc_dev(&wplanes, n * d);
+  if (rc == EFFSC_HIP_OK && n) {
+    hipError_t e = hipMemcpyAsync(staging, wit, n * d * 8, hipMemcpyDefault, h->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_import, dim3(blocks_for(h, n)), dim3(kBlock), 0, h->stream, wplanes, n, staging, n, d,
+                         h->repr == EFFSC_HIP_MONT64 ? 1 : 0);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      const unsigned g = blocks_for(h, nbc);
+      if (d == 1)
+        hipLaunchKernelGGL(k_gkr_broadcast<Fp>, dim3(g), dim3(kBlock), 0, h->stream, View<Fp>{wplanes, n}, n,
+                           View<Fp>{h->in[2], h->in_stride}, View<Fp>{h->in[3], h->in_stride});
+      else
+        hipLaunchKernelGGL(k_gkr_broadcast<Fp3>, dim3(g), dim3(kBlock), 0, h->stream, View<Fp3>{wplanes, n}, n,
+                           View<Fp3>{h->in[2], h->in_stride}, View<Fp3>{h->in[3], h->in_stride});
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, std::string("gkr witness: ") + hipGetErrorString(e));
+  }
+  if (staging) (void)hipFree(staging);
+  if (wplanes) (void)hipFree(wplanes);
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_gkr_claimed_w(effsc_hip_prover* h, uint64_t* out) {
+  // claimed_w_values, src/provers/gkr.rs:89-97: (W(b*), W(c*)) once the
+  // tables have length 1, else zeros
+  if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  if (h->kind != K_GKR) return fail(EFFSC_HIP_EINVAL, "not a GKR prover");
+  HIPCHK(hipSetDevice(h->dev));
+  const int E = h->ext;
+  std::fill(out, out + 2 * E, 0);
+  if (h->len != 1) return EFFSC_HIP_OK;
+  uint64_t el[kMaxK * 3];
+  TRY(read_elements(h, 0, el));
+  words_out(h, el + 2 * E, out, 2 * E);
+  return EFFSC_HIP_OK;
 }
 
 int effsc_hip_eq_factored_new(const effsc_hip_config* cfg, const uint64_t* point, size_t v, const uint64_t* p,

@@ -53,6 +53,8 @@ _SIGS = {
     "effsc_hip_inner_product_new": (C.c_int, [C.POINTER(Config), u64p, u64p, sz, C.POINTER(vp)]),
     "effsc_hip_product_new": (C.c_int, [C.POINTER(Config), C.POINTER(u64p), C.c_int, sz, C.POINTER(vp)]),
     "effsc_hip_eq_factored_new": (C.c_int, [C.POINTER(Config), u64p, sz, u64p, sz, C.POINTER(vp)]),
+    "effsc_hip_gkr_new": (C.c_int, [C.POINTER(Config), u64p, u64p, u64p, sz, C.POINTER(vp)]),
+    "effsc_hip_gkr_claimed_w": (C.c_int, [vp, u64p]),
     "effsc_hip_degree": (C.c_int, [vp, szp]),
     "effsc_hip_ext": (C.c_int, [vp, szp]),
     "effsc_hip_round": (C.c_int, [vp, u64p, u64p]),

@@ -242,6 +242,32 @@ class EqFactoredProver(NativeProver):
         return self.field.zero(), self.field.zero()
 
 
+class GkrProver(NativeProver):
+    """GkrProver (src/provers/gkr.rs:38-202): the GKR layer polynomial
+    add(b,c) (W(b) + W(c)) + mult(b,c) W(b) W(c) over 2k variables, degree 2."""
+
+    def __init__(self, add_evals, mult_evals, w_evals, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7,
+                 device=0):
+        super().__init__(ext, repr)
+        input_ext = ext if input_ext is None else input_ext
+        cfg = _config(ext, input_ext, repr, w, device, 0)
+        A, M, Wt = _words(add_evals, input_ext), _words(mult_evals, input_ext), _words(w_evals, input_ext)
+        n = len(Wt) // input_ext
+        if len(A) != n * n * input_ext:
+            raise ValueError("add_evals must have len w^2")  # src/provers/gkr.rs:61
+        if len(M) != n * n * input_ext:
+            raise ValueError("mult_evals must have len w^2")  # src/provers/gkr.rs:62
+        h = C.c_void_p()
+        N.check(N.lib().effsc_hip_gkr_new(C.byref(cfg), A.ctypes.data_as(N.u64p), M.ctypes.data_as(N.u64p),
+                                           Wt.ctypes.data_as(N.u64p), n, C.byref(h)))
+        self._finish(h)
+
+    def claimed_w_values(self):
+        out = np.zeros(2 * self.ext, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_gkr_claimed_w(self.h, out.ctypes.data_as(N.u64p)))
+        return self._elem(out[:self.ext]), self._elem(out[self.ext:])
+
+
 def fold(values, weight, device=0):
     """fold(&mut Vec<F>, weight) (src/fold.rs:14) for base-field tables:
     returns the folded list (length next_pow2(n)/2), computed on the device."""

@@ -93,6 +93,17 @@ int effsc_hip_inner_product_new(const effsc_hip_config* cfg, const uint64_t* a, 
 int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
                           effsc_hip_prover** out);
 
+/* GkrProver::new(add_evals, mult_evals, w_evals) — src/provers/gkr.rs:58-87.
+ * Degree 2 over 2k variables: add and mult hold n * n elements over (b, c)
+ * (index b n + c), wit holds the n witness values W(x).  Tables 0..3 of the
+ * post-state are add, mult, w_b and w_c (the witness broadcast over c and
+ * over b, as in the reference). */
+int effsc_hip_gkr_new(const effsc_hip_config* cfg, const uint64_t* add, const uint64_t* mult, const uint64_t* wit,
+                      size_t n, effsc_hip_prover** out);
+/* claimed_w_values() — src/provers/gkr.rs:89-97: (W(b*), W(c*)), 2 * ext
+ * words, once every variable is bound (zeros before). */
+int effsc_hip_gkr_claimed_w(effsc_hip_prover* h, uint64_t* out);
+
 /* EqFactoredProver::new(w, p) — src/provers/eq_factored.rs:90-114.
  * Degree 2; `point` holds v elements (host memory), p holds n <= 2^v elements
  * (zero-padded).  With EFFSC_HIP_BORROW_INPUT, p is a canonical device table
