@@ -54,6 +54,40 @@ __global__ __launch_bounds__(kBlock) void k_export(uint64_t* dst, const uint64_t
   }
 }
 
+// Bit-reversal permutation of planes of n = 2^v words: dst[i] = src[rev_v(i)]
+// (reorder_vec_msb, src/streams/memory/core.rs:20-47).  The index splits
+// into (a, m, b) with t = 5 top and bottom bits; a workgroup owns one middle
+// value m and moves the 32 x 32 tile of (a, b) through LDS, so both the reads
+// (b contiguous) and the writes (rev(a) contiguous) are coalesced:
+// dst[(rho, rev(m), kappa)] = src[(rev(kappa), m, rev(rho))].
+constexpr int kRevT = 5;
+__device__ __forceinline__ uint32_t rev5(uint32_t x) { return __brev(x) >> 27; }
+
+__global__ __launch_bounds__(kBlock) void k_bitrev_tiled(uint64_t* dst, size_t dstr, const uint64_t* src, size_t sstr,
+                                                          int v) {
+  __shared__ uint64_t tile[32][33];
+  src += (size_t)blockIdx.y * sstr;
+  dst += (size_t)blockIdx.y * dstr;
+  const int mbits = v - 2 * kRevT;
+  const uint64_t m = blockIdx.x;
+  const uint64_t mrev = mbits ? (__brevll(m) >> (64 - mbits)) : 0;
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (uint32_t a = ty; a < 32; a += kBlock / 32)
+    tile[a][tx] = src[((uint64_t)a << (v - kRevT)) | (m << kRevT) | tx];
+  __syncthreads();
+  for (uint32_t rho = ty; rho < 32; rho += kBlock / 32)
+    dst[((uint64_t)rho << (v - kRevT)) | (mrev << kRevT) | tx] = tile[rev5(tx)][rev5(rho)];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bitrev_small(uint64_t* dst, size_t dstr, const uint64_t* src, size_t sstr,
+                                                          int v) {
+  src += (size_t)blockIdx.y * sstr;
+  dst += (size_t)blockIdx.y * dstr;
+  const uint64_t n = (uint64_t)1 << v;
+  for (uint64_t i = gtid(); i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[v ? (__brevll(i) >> (64 - v)) : 0];
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_uniform(uint64_t* dst, size_t n, uint64_t key, uint64_t first) {
   const size_t step = (size_t)gridDim.x * blockDim.x;
   for (size_t i = gtid(); i < n; i += step) dst[i] = uniform_word(key, first + i);
@@ -113,6 +147,9 @@ struct effsc_hip_prover {
   uint64_t* eq_r = nullptr;
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
   uint64_t* eq_point = nullptr;  // canonical point, v * ext words (rebuilds the eq tables on reset)
+  // LSB (pair-split) provers hold their tables bit-reversed and run the MSB
+  // path; post-state reads reverse back (src/streams/memory/core.rs:20-47)
+  bool lsb = false;
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
   unsigned last_grid = 0;
@@ -611,6 +648,12 @@ int final_value_impl(effsc_hip_prover* h, uint64_t* out) {
 }
 
 // ----------------------------------------------------- word conversions
+inline uint64_t gl_brev64(uint64_t x) {
+  uint64_t r = 0;
+  for (int b = 0; b < 64; b++) r |= ((x >> b) & 1) << (63 - b);
+  return r;
+}
+
 void words_in(const effsc_hip_prover* h, const uint64_t* src, uint64_t* dst, size_t n) {
   for (size_t i = 0; i < n; i++) dst[i] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(src[i], gl::MONT_TO_CANON) : gl::canon(src[i]);
 }
@@ -1270,6 +1313,91 @@ void effsc_hip_config_default(effsc_hip_config* cfg) {
 
 const char* effsc_hip_last_error(void) { return effsc_hip::last_error(); }
 
+namespace {
+
+// Launches the bit reversal of `planes` planes of n = 2^v words.
+int launch_bitrev(hipStream_t st, uint64_t* dst, size_t dstr, const uint64_t* src, size_t sstr, size_t n, int planes) {
+  int v = 0;
+  while (((size_t)1 << v) < n) v++;
+  if (v >= 2 * kRevT) {
+    hipLaunchKernelGGL(k_bitrev_tiled, dim3((unsigned)((size_t)1 << (v - 2 * kRevT)), (unsigned)planes), dim3(kBlock),
+                       0, st, dst, dstr, src, sstr, v);
+  } else {
+    hipLaunchKernelGGL(k_bitrev_small, dim3(1, (unsigned)planes), dim3(kBlock), 0, st, dst, dstr, src, sstr, v);
+  }
+  HIPCHK(hipGetLastError());
+  return EFFSC_HIP_OK;
+}
+
+// LSB provers (src/provers/{multilinear,inner_product,coefficient}_lsb.rs):
+// the MSB prover of the same kind over the bit-reversed tables.  Pairing
+// (2k, 2k+1) of the table is pairing (rev k, rev k + n/2) of its reversal, and
+// folding keeps the two tables bit-reversals of each other, so every round
+// polynomial, challenge and final value is the same field element.
+int new_lsb(const effsc_hip_config* cfg, int kind, int K, int deg, const uint64_t* const* tables, size_t n,
+            effsc_hip_prover** out) {
+  if (out) *out = nullptr;
+  if (!cfg) return fail(EFFSC_HIP_EINVAL, "null config");
+  if (!is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "LSB provers take power-of-two tables");
+  effsc_hip_config c2 = *cfg;
+  c2.flags &= ~EFFSC_HIP_BORROW_INPUT;  // the reordered tables are the prover's own
+  effsc_hip_prover* h = nullptr;
+  TRY(new_product_like(&c2, kind, K, deg, tables, n, &h));
+  int rc = EFFSC_HIP_OK;
+  for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++) {
+    uint64_t* tmp = nullptr;
+    rc = alloc_dev(&tmp, h->in_stride * h->in_ext);
+    if (rc == EFFSC_HIP_OK) rc = launch_bitrev(h->stream, tmp, h->in_stride, h->in[i], h->in_stride, n, h->in_ext);
+    if (rc == EFFSC_HIP_OK) {
+      hipError_t e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, std::string("bit reverse: ") + hipGetErrorString(e));
+    }
+    if (rc == EFFSC_HIP_OK) {
+      (void)hipFree(h->in[i]);
+      h->in[i] = tmp;
+    } else if (tmp) {
+      (void)hipFree(tmp);
+    }
+  }
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  h->lsb = true;
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
+}  // namespace
+
+int effsc_hip_multilinear_lsb_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n, effsc_hip_prover** out) {
+  const uint64_t* t[1] = {evals};
+  return new_lsb(cfg, K_ML, 1, 1, t, n, out);
+}
+
+int effsc_hip_inner_product_lsb_new(const effsc_hip_config* cfg, const uint64_t* a, const uint64_t* b, size_t n,
+                                    effsc_hip_prover** out) {
+  const uint64_t* t[2] = {a, b};
+  return new_lsb(cfg, K_IP, 2, 2, t, n, out);
+}
+
+int effsc_hip_product_lsb_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
+                              effsc_hip_prover** out) {
+  if (out) *out = nullptr;
+  if (!tables) return fail(EFFSC_HIP_EINVAL, "null tables");
+  return new_lsb(cfg, K_PRODUCT, k, k, tables, n, out);
+}
+
+int effsc_hip_bit_reverse(int device, uint64_t* dst, const uint64_t* src, size_t n, int planes) {
+  if (!dst || !src || planes < 1) return fail(EFFSC_HIP_EINVAL, "bad argument");
+  if (!is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "bit reversal needs a power-of-two length");
+  if (dst == src) return fail(EFFSC_HIP_EINVAL, "bit reversal is out of place");
+  HIPCHK(hipSetDevice(device));
+  TRY(launch_bitrev(nullptr, dst, n, src, n, n, planes));
+  HIPCHK(hipDeviceSynchronize());
+  return EFFSC_HIP_OK;
+}
+
 int effsc_hip_multilinear_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n, effsc_hip_prover** out) {
   const uint64_t* t[1] = {evals};
   return new_product_like(cfg, K_ML, 1, 1, t, n, out);
@@ -1538,11 +1666,16 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
   for (int c = 0; c < d && L; c++)
     HIPCHK(hipMemcpyAsync(buf.data() + c * L, planes + c * stride, L * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  for (size_t i = 0; i < L; i++)
+  // LSB provers: element i of the reference's table is element rev(i) here
+  int lv = 0;
+  while (h->lsb && ((size_t)1 << lv) < L) lv++;
+  for (size_t i = 0; i < L; i++) {
+    const size_t j = (h->lsb && lv) ? (size_t)(gl_brev64(i) >> (64 - lv)) : i;
     for (int c = 0; c < E; c++) {
-      uint64_t x = c < d ? buf[c * L + i] : 0;
+      uint64_t x = c < d ? buf[c * L + j] : 0;
       dst[i * E + c] = h->repr == EFFSC_HIP_MONT64 ? gl::mul(x, gl::CANON_TO_MONT) : x;
     }
+  }
   return EFFSC_HIP_OK;
 }
 

@@ -54,6 +54,10 @@ _SIGS = {
     "effsc_hip_product_new": (C.c_int, [C.POINTER(Config), C.POINTER(u64p), C.c_int, sz, C.POINTER(vp)]),
     "effsc_hip_eq_factored_new": (C.c_int, [C.POINTER(Config), u64p, sz, u64p, sz, C.POINTER(vp)]),
     "effsc_hip_gkr_new": (C.c_int, [C.POINTER(Config), u64p, u64p, u64p, sz, C.POINTER(vp)]),
+    "effsc_hip_multilinear_lsb_new": (C.c_int, [C.POINTER(Config), u64p, sz, C.POINTER(vp)]),
+    "effsc_hip_inner_product_lsb_new": (C.c_int, [C.POINTER(Config), u64p, u64p, sz, C.POINTER(vp)]),
+    "effsc_hip_product_lsb_new": (C.c_int, [C.POINTER(Config), C.POINTER(u64p), C.c_int, sz, C.POINTER(vp)]),
+    "effsc_hip_bit_reverse": (C.c_int, [C.c_int, u64p, u64p, sz, C.c_int]),
     "effsc_hip_gkr_claimed_w": (C.c_int, [vp, u64p]),
     "effsc_hip_degree": (C.c_int, [vp, szp]),
     "effsc_hip_ext": (C.c_int, [vp, szp]),

@@ -242,6 +242,101 @@ class EqFactoredProver(NativeProver):
         return self.field.zero(), self.field.zero()
 
 
+def _lsb(cls_fn, tables, ext, input_ext, repr, w, device, borrow, n):
+    """Common constructor of the LSB provers: host word arrays or, with
+    borrow, device addresses of canonical tables of n elements."""
+    input_ext = ext if input_ext is None else input_ext
+    cfg = _config(ext, input_ext, repr, w, device, BORROW_INPUT if borrow else 0)
+    if borrow:
+        ptrs = [C.cast(C.c_void_p(t), N.u64p) for t in tables]
+        count = n
+    else:
+        arrs = [_words(t, input_ext) for t in tables]
+        if len({len(a) for a in arrs}) != 1:
+            raise ValueError("a and b must have equal length")  # src/provers/inner_product_lsb.rs
+        ptrs = [a.ctypes.data_as(N.u64p) for a in arrs]
+        count = len(arrs[0]) // input_ext
+    h = C.c_void_p()
+    N.check(cls_fn(C.byref(cfg), ptrs, count, h))
+    return h
+
+
+class MultilinearProverLSB(NativeProver):
+    """MultilinearProverLSB (src/provers/multilinear_lsb.rs): pairs (2k, 2k+1),
+    degree 1; power-of-two tables."""
+
+    def __init__(self, evals, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False,
+                 n=None):
+        super().__init__(ext, repr)
+        self._finish(_lsb(lambda cfg, p, cnt, h: N.lib().effsc_hip_multilinear_lsb_new(cfg, p[0], cnt, C.byref(h)),
+                          [evals], ext, input_ext, repr, w, device, borrow, n))
+
+    def evals(self):
+        return self.table(0)
+
+    def num_variables(self):
+        n = self._size(N.lib().effsc_hip_table_len, 0)
+        return 0 if n == 0 else (n - 1).bit_length()
+
+
+class InnerProductProverLSB(NativeProver):
+    """InnerProductProverLSB (src/provers/inner_product_lsb.rs), degree 2."""
+
+    def __init__(self, a, b, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False,
+                 n=None):
+        super().__init__(ext, repr)
+        self._finish(_lsb(lambda cfg, p, cnt, h: N.lib().effsc_hip_inner_product_lsb_new(cfg, p[0], p[1], cnt,
+                                                                                          C.byref(h)),
+                          [a, b], ext, input_ext, repr, w, device, borrow, n))
+
+    def evaluations(self):
+        return self.table(0), self.table(1)
+
+    def final_evaluations(self):
+        a, b = self.evaluations()
+        if len(a) == 1:
+            return a[0], b[0]
+        return self.field.zero(), self.field.zero()
+
+
+class CoefficientProverLSB(NativeProver):
+    """CoefficientProverLSB with the product-of-k evaluator
+    (src/provers/coefficient_lsb.rs), degree k."""
+
+    def __init__(self, pairwise, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False,
+                 n=None):
+        super().__init__(ext, repr)
+        k = len(pairwise)
+
+        def make(cfg, p, cnt, h):
+            arr = (N.u64p * k)(*p)
+            return N.lib().effsc_hip_product_lsb_new(cfg, arr, k, cnt, C.byref(h))
+        self._finish(_lsb(make, pairwise, ext, input_ext, repr, w, device, borrow, n))
+
+    def pairwise(self):
+        return [self.table(i) for i in range(self._size(N.lib().effsc_hip_table_count))]
+
+
+def bit_reverse(values, device=0):
+    """reorder_vec_msb (src/streams/memory/core.rs:20-24) on the GPU:
+    out[i] = values[bit_reverse(i)] for a power-of-two list of base-field words."""
+    w = np.ascontiguousarray(values, dtype=np.uint64)
+    n = len(w)
+    L = N.lib()
+    src, dst = N.u64p(), N.u64p()
+    N.check(L.effsc_hip_device_alloc(device, max(n, 1), C.byref(src)))
+    N.check(L.effsc_hip_device_alloc(device, max(n, 1), C.byref(dst)))
+    try:
+        out = np.zeros(n, dtype=np.uint64)
+        N.check(L.effsc_hip_copy(device, src, w.ctypes.data_as(N.u64p), n))
+        N.check(L.effsc_hip_bit_reverse(device, dst, src, n, 1))
+        N.check(L.effsc_hip_copy(device, out.ctypes.data_as(N.u64p), dst, n))
+        return out
+    finally:
+        L.effsc_hip_device_free(device, src)
+        L.effsc_hip_device_free(device, dst)
+
+
 class GkrProver(NativeProver):
     """GkrProver (src/provers/gkr.rs:38-202): the GKR layer polynomial
     add(b,c) (W(b) + W(c)) + mult(b,c) W(b) W(c) over 2k variables, degree 2."""

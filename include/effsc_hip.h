@@ -93,6 +93,25 @@ int effsc_hip_inner_product_new(const effsc_hip_config* cfg, const uint64_t* a, 
 int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
                           effsc_hip_prover** out);
 
+/* LSB (pair-split) provers: pairs (2k, 2k+1), the least significant variable
+ * bound first.  MultilinearProverLSB::new (src/provers/multilinear_lsb.rs:33-36),
+ * InnerProductProverLSB::new (src/provers/inner_product_lsb.rs),
+ * CoefficientProverLSB::new with the product-of-k evaluator
+ * (src/provers/coefficient_lsb.rs).  Tables have power-of-two length; the
+ * prover bit-reverses them once into its own device planes (so
+ * EFFSC_HIP_BORROW_INPUT sources are read, never kept) and post-state reads
+ * return the reference's pair-split order. */
+int effsc_hip_multilinear_lsb_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n,
+                                  effsc_hip_prover** out);
+int effsc_hip_inner_product_lsb_new(const effsc_hip_config* cfg, const uint64_t* a, const uint64_t* b, size_t n,
+                                    effsc_hip_prover** out);
+int effsc_hip_product_lsb_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
+                              effsc_hip_prover** out);
+/* reorder_vec_msb (src/streams/memory/core.rs:20-24) on the device:
+ * dst[i] = src[bit_reverse(i)] for `planes` consecutive planes of n = 2^v
+ * words each (out of place; device pointers). */
+int effsc_hip_bit_reverse(int device, uint64_t* dst, const uint64_t* src, size_t n, int planes);
+
 /* GkrProver::new(add_evals, mult_evals, w_evals) — src/provers/gkr.rs:58-87.
  * Degree 2 over 2k variables: add and mult hold n * n elements over (b, c)
  * (index b n + c), wit holds the n witness values W(x).  Tables 0..3 of the

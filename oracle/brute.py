@@ -244,3 +244,18 @@ def prove_gkr(F, add, mult, wit, transcript, lift=True):
         a, m, b, c = mle(F, A, z), mle(F, M, z), mle(F, B, z), mle(F, Cc, z)
         return F.add(F.mul(a, F.add(b, c)), F.mul(m, F.mul(b, c)))
     return prove(F, summand, v, 2, transcript)
+
+
+def prove_lsb(F, tables, v, transcript, degree, lift=True):
+    """An LSB (pair-split) prover binds the least significant index bit
+    first: the brute force of the product of the tables' extensions with the
+    point's coordinates taken in reverse order."""
+    Ts = [_pad(t, v, F, lift) for t in tables]
+
+    def summand(z):
+        zr = list(reversed(z))
+        acc = F.one()
+        for T in Ts:
+            acc = F.mul(acc, mle(F, T, zr))
+        return acc
+    return prove(F, summand, v, degree, transcript)

@@ -344,6 +344,7 @@ struct or_prover {
     size_t len[MAXTAB];
     /* EqFactored: tab[0] = p, tab[1] = eq_l, tab[2] = eq_r */
     size_t v, v_l, rounds_elapsed;
+    int lsb; /* pair-split layout, pairs (2k, 2k+1): the provers in src/provers/..._lsb.rs */
 };
 
 static size_t next_pow2(size_t n) {
@@ -503,6 +504,26 @@ or_prover *or_prover_new_gkr(int ext, uint64_t w, const uint64_t *add, const uin
         }
     free(wv);
     for (int t = 0; t < 4; t++) p->len[t] = nbc;
+    return p;
+}
+
+/* LSB (pair-split) provers: MultilinearProverLSB::new
+ * (src/provers/multilinear_lsb.rs:33-36), InnerProductProverLSB::new
+ * (src/provers/inner_product_lsb.rs), CoefficientProverLSB::new with the
+ * product-of-k evaluator (src/provers/coefficient_lsb.rs), base inputs lifted. */
+or_prover *or_prover_new_ml_lsb(int ext, uint64_t w, const uint64_t *evals, size_t n) {
+    or_prover *p = or_prover_new_ml(ext, w, evals, n);
+    if (p) p->lsb = 1;
+    return p;
+}
+or_prover *or_prover_new_ip_lsb(int ext, uint64_t w, const uint64_t *a, const uint64_t *b, size_t n) {
+    or_prover *p = or_prover_new_ip(ext, w, a, b, n);
+    if (p) p->lsb = 1;
+    return p;
+}
+or_prover *or_prover_new_product_lsb(int ext, uint64_t w, int k, const uint64_t *const *tables, size_t n) {
+    or_prover *p = or_prover_new_product(ext, w, k, tables, n);
+    if (p) p->lsb = 1;
     return p;
 }
 
@@ -676,7 +697,67 @@ static void round_gkr(or_prover *p, el *out) {
     out[1] = e_mul(f, e_sub(f, e_add(f, q0, q2), e_add(f, q1, q1)), e_inv(f, two));
 }
 
+/* fold_lsb: new[k] = f[2k] + w (f[2k+1] - f[2k]), k < n/2 (an odd trailing
+ * element is dropped), src/provers/multilinear_lsb.rs:104-126,
+ * inner_product_lsb.rs fold_lsb, src/reductions/pairwise.rs:38-73 */
+static void fold_lsb(const fctx *f, el *t, size_t *len, el w) {
+    size_t n = *len;
+    if (n <= 1) return;
+    size_t h = n / 2;
+    for (size_t k = 0; k < h; k++) t[k] = e_add(f, t[2 * k], e_mul(f, w, e_sub(f, t[2 * k + 1], t[2 * k])));
+    *len = h;
+}
+
+/* compute_lsb (src/provers/multilinear_lsb.rs:59-99): s0 = sum of the
+ * even-indexed elements (an odd trailing element counts in s0) */
+static void round_ml_lsb(or_prover *p, el *out) {
+    const fctx *f = &p->f;
+    el s0 = e_zero();
+    for (size_t i = 0; i < p->len[0]; i += 2) s0 = e_add(f, s0, p->tab[0][i]);
+    out[0] = s0;
+}
+
+/* compute_lsb (src/provers/inner_product_lsb.rs): q0 = sum a_even b_even,
+ * q_inf = sum (a_odd - a_even)(b_odd - b_even); one element: [a0 b0, 0] */
+static void round_ip_lsb(or_prover *p, el *out) {
+    const fctx *f = &p->f;
+    size_t n = p->len[0];
+    const el *a = p->tab[0], *b = p->tab[1];
+    out[0] = out[1] = e_zero();
+    if (n == 1) out[0] = e_mul(f, a[0], b[0]);
+    if (n <= 1) return;
+    for (size_t i = 0; i + 1 < n; i += 2) {
+        out[0] = e_add(f, out[0], e_mul(f, a[i], b[i]));
+        out[1] = e_add(f, out[1], e_mul(f, e_sub(f, a[i + 1], a[i]), e_sub(f, b[i + 1], b[i])));
+    }
+}
+
+/* evaluate_coefficients over pairs (2i, 2i+1), src/provers/coefficient_lsb.rs */
+static void round_product_lsb(or_prover *p, el *out) {
+    const fctx *f = &p->f;
+    int k = p->ntab, d = p->deg;
+    size_t npairs = p->len[0] / 2;
+    el coeffs[MAXTAB + 1];
+    for (int j = 0; j <= d; j++) coeffs[j] = e_zero();
+    el lo[MAXTAB], hi[MAXTAB];
+    for (size_t i = 0; i < npairs; i++) {
+        for (int t = 0; t < k; t++) {
+            lo[t] = p->tab[t][2 * i];
+            hi[t] = p->tab[t][2 * i + 1];
+        }
+        product_accumulate(f, k, lo, hi, coeffs);
+    }
+    out[0] = horner(f, coeffs, d + 1, e_zero());
+    if (d <= 1) return;
+    out[1] = coeffs[d];
+    for (int i = 2; i < d; i++) out[i] = horner(f, coeffs, d + 1, e_from_u64((uint64_t)i));
+}
+
 static void fold_all(or_prover *p, el c) {
+    if (p->lsb) {
+        for (int t = 0; t < p->ntab; t++) fold_lsb(&p->f, p->tab[t], &p->len[t], c);
+        return;
+    }
     switch (p->kind) {
     case OR_GKR: /* src/provers/gkr.rs:93-101, 185-190 */
     case OR_ML:
@@ -696,6 +777,18 @@ void or_prover_round(or_prover *p, const uint64_t *chal, uint64_t *out) {
     el res[MAXTAB + 1];
     el c = e_zero();
     if (chal) c = e_load(&p->f, chal);
+    if (p->lsb) {
+        if (chal) fold_all(p, c);
+        if (p->kind == OR_ML)
+            round_ml_lsb(p, res);
+        else if (p->kind == OR_IP)
+            round_ip_lsb(p, res);
+        else
+            round_product_lsb(p, res);
+        int nw = p->deg < 1 ? 1 : p->deg;
+        for (int i = 0; i < nw; i++) e_store(&p->f, res[i], out + (size_t)i * p->f.d);
+        return;
+    }
     switch (p->kind) {
     case OR_ML: /* src/provers/multilinear.rs:58-67 */
         if (chal) fold_all(p, c);

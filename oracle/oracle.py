@@ -63,6 +63,9 @@ def lib():
             "or_prover_new_product_ext": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_new_eq": (vp, [C.c_int, u64, u64p, sz, u64p, sz]),
             "or_prover_new_gkr": (vp, [C.c_int, u64, u64p, u64p, u64p, sz]),
+            "or_prover_new_ml_lsb": (vp, [C.c_int, u64, u64p, sz]),
+            "or_prover_new_ip_lsb": (vp, [C.c_int, u64, u64p, u64p, sz]),
+            "or_prover_new_product_lsb": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_degree": (C.c_int, [vp]),
             "or_prover_round": (None, [vp, u64p, u64p]),
             "or_prover_finalize": (None, [vp, u64p]),
@@ -182,6 +185,26 @@ class Prover:
         W, pw = _arr(point)
         Pv, pp = _arr(p_evals)
         return cls(lib().or_prover_new_eq(ext, w, pw, len(W) // ext, pp, len(Pv) // ext), ext)
+
+    @classmethod
+    def multilinear_lsb(cls, evals, ext=1, w=W_FROM_RAW7):
+        """MultilinearProverLSB (src/provers/multilinear_lsb.rs), pairs (2k, 2k+1)."""
+        E, p = _arr(evals)
+        return cls(lib().or_prover_new_ml_lsb(ext, w, p, len(E) // ext), ext)
+
+    @classmethod
+    def inner_product_lsb(cls, a, b, ext=1, w=W_FROM_RAW7):
+        """InnerProductProverLSB (src/provers/inner_product_lsb.rs)."""
+        A, pa = _arr(a)
+        B, pb = _arr(b)
+        return cls(lib().or_prover_new_ip_lsb(ext, w, pa, pb, len(A) // ext), ext)
+
+    @classmethod
+    def product_lsb(cls, tables, ext=1, w=W_FROM_RAW7):
+        """CoefficientProverLSB with the product evaluator; base tables lifted."""
+        arrs = [_arr(t) for t in tables]
+        ptrs = (u64p * len(arrs))(*[p for _, p in arrs])
+        return cls(lib().or_prover_new_product_lsb(ext, w, len(arrs), ptrs, len(arrs[0][0])), ext)
 
     @classmethod
     def gkr(cls, add, mult, wit, ext=1, w=W_FROM_RAW7):

@@ -300,3 +300,42 @@ def test_gkr_reference_identities():
     z = np.zeros(16, dtype=np.uint64)
     rp, _, _ = O.Prover.gkr(z, z, O.uniform(4, 0x500)).sumcheck(4, O.Transcript.rng(99))
     assert all(x == [0] for r in rp for x in r)
+
+
+# ---------------------------------------------------------------- LSB provers
+@pytest.mark.parametrize("kind,k", [("ml", 1), ("ip", 2), ("p3", 3)])
+@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 5])
+def test_lsb_oracle_matches_brute_force(kind, k, ext, v):
+    n = 1 << v
+    tabs = [O.uniform(n, 300 + 10 * k + v + i) for i in range(k)]
+    if kind == "ml":
+        o = O.Prover.multilinear_lsb(tabs[0] if ext == 1 else _lift(tabs[0], ext), ext=ext)
+    elif kind == "ip":
+        o = O.Prover.inner_product_lsb(*[t if ext == 1 else _lift(t, ext) for t in tabs], ext=ext)
+    else:
+        o = O.Prover.product_lsb(tabs, ext=ext)
+    rp, ch, fin = o.sumcheck(v, O.Transcript.shake(b"lsb"))
+    F = B.Ext(ext)
+    brp, bch, bfin = B.prove_lsb(F, [[int(x) for x in t] for t in tabs], v, B.ShakeTranscript(b"lsb"), k)
+    assert [[list(x) for x in r] for r in brp] == rp
+    assert [list(c) for c in bch] == ch
+    assert list(bfin) == fin
+
+
+def _lift(words, ext):
+    out = np.zeros(len(words) * ext, dtype=np.uint64)
+    out[::ext] = words
+    return out
+
+
+def test_lsb_equals_msb_on_bit_reversed_table():
+    # src/streams/memory/core.rs:20-47: reorder_vec_msb(out[i] = src[rev(i)])
+    # turns the LSB statement into the MSB one, proof for proof
+    for v in (1, 4, 7):
+        n = 1 << v
+        a, b = O.uniform(n, 7 + v), O.uniform(n, 8 + v)
+        rev = np.array([int(format(i, f"0{v}b")[::-1], 2) for i in range(n)])
+        lsb = O.Prover.inner_product_lsb(a, b).sumcheck(v, O.Transcript.shake(b"r"))
+        msb = O.Prover.inner_product(a[rev], b[rev]).sumcheck(v, O.Transcript.shake(b"r"))
+        assert lsb == msb
