@@ -150,6 +150,12 @@ struct effsc_hip_prover {
   // LSB (pair-split) provers hold their tables bit-reversed and run the MSB
   // path; post-state reads reverse back (src/streams/memory/core.rs:20-47)
   bool lsb = false;
+  // Provers built from streams: the first two-round pass (round 0 values and
+  // round 1 coefficients over the input) was accumulated during the upload
+  // and its reduced values live in d_pass0 (the input is never written, so
+  // they stay valid across resets)
+  uint64_t* d_pass0 = nullptr;
+  bool pass0_ready = false;
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
   unsigned last_grid = 0;
@@ -686,6 +692,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_l) (void)hipFree(h->eq_l);
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
+  if (h->d_pass0) (void)hipFree(h->d_pass0);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
@@ -1191,17 +1198,25 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
     while (num_rounds - r >= 2) {
       size_t L = h->len >> p;
       if (L <= tail_max() || L < 4) break;
-      TRY(launch_pass(h, p, L / 4, p ? h->d_cur : nullptr));
+      Reduce red = h->red;
+      unsigned g = 0;
+      if (p == 0 && h->pass0_ready && !h->in_work && h->len == h->n) {
+        // the pass over the input was accumulated while it streamed in
+        red = Reduce{h->d_pass0, h->red.result};
+        g = 1;
+      } else {
+        TRY(launch_pass(h, p, L / 4, p ? h->d_cur : nullptr));
+        g = h->last_grid;
+      }
       if (p > 0) {
         h->len = L;
         h->in_work = true;
       }
-      unsigned g = h->last_grid;
       if (K == 1)
-        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs,
                            h->d_polys + r, h->d_chals + r, h->d_cur);
       else
-        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs,
                            h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
       HIPCHK(hipGetLastError());
       r += 2;
@@ -2009,3 +2024,395 @@ extern "C" int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint6
   HIPCHK(hipDeviceSynchronize());
   return EFFSC_HIP_OK;
 }
+
+// ===========================================================================
+// Streamed input.  FileStream (src/streams/file/core.rs:13-76): a file of
+// 2^v serialized field elements (ark-serialize uncompressed: each component
+// one little-endian canonical u64), mapped read-only.  Tables move to the
+// device in chunks through pinned double buffers on a copy stream while the
+// compute stream works on the previous chunk, so the transfer hides the
+// first pass over the data.
+// ===========================================================================
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <thread>
+
+struct effsc_hip_stream {
+  int fd = -1;
+  const uint8_t* map = nullptr;
+  size_t bytes = 0;
+  int ext = 1;
+  size_t n = 0;
+  unsigned v = 0;
+};
+
+namespace {
+
+// Host copy of `bytes` with up to 8 threads (the mapped file is pageable, so
+// it is staged through pinned memory by the CPU).
+void par_copy(void* dst, const void* src, size_t bytes) {
+  const size_t per = (size_t)4 << 20;
+  int nt = (int)std::min<size_t>(8, (bytes + per - 1) / per);
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t chunk = (bytes + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    size_t o = (size_t)t * chunk;
+    if (o >= bytes) break;
+    size_t len = std::min(chunk, bytes - o);
+    th.emplace_back([=] { std::memcpy((uint8_t*)dst + o, (const uint8_t*)src + o, len); });
+  }
+  for (auto& x : th) x.join();
+}
+
+size_t stream_chunk_units() {
+  if (const char* e = std::getenv("EFFSC_HIP_STREAM_CHUNK")) {
+    long long x = std::atoll(e);
+    if (x >= 1 && !(x & (x - 1))) return (size_t)x;
+  }
+  return (size_t)1 << 20;
+}
+
+// Round 0 values and round 1 coefficients (the P = 0 two-round pass) of the
+// units [u0, u0 + count) of resident base tables, one partial per block;
+// flags any word >= p (a serialized element must be canonical).
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_stream_pass(Views<Fp, K> t, size_t q, size_t u0, size_t count,
+                                                        uint64_t* partials, unsigned* bad) {
+  PassAcc<K> acc;
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
+  unsigned b = 0;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < count; k += step) {
+    const size_t u = u0 + k;
+    uint64_t E[K][4];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        E[i][j] = t.t[i].p[u + j * q];
+        b |= E[i][j] >= gl::P;
+      }
+    pass_accumulate<K>(E, acc);
+  }
+  if (b) atomicOr(bad, 1u);
+  Fp raw[pass_raw<K>()], vals[pass_values<K>()];
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
+  finish<Fp, pass_values<K>()>(vals, Reduce{partials, nullptr});
+}
+
+// Flags non-canonical words of a resident base table.
+__global__ __launch_bounds__(kBlock) void k_check_canonical(const uint64_t* t, size_t n, unsigned* bad) {
+  unsigned b = 0;
+  for (size_t i = gtid(); i < n; i += (size_t)gridDim.x * blockDim.x) b |= t[i] >= gl::P;
+  if (b) atomicOr(bad, 1u);
+}
+
+// sum_i prod_j s_j[i] over a chunk of `count` elements held AoS (D words) in
+// the device staging of each stream; one partial per block.
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_stream_claim(const uint64_t* const* st, int k, size_t count,
+                                                         uint64_t w, uint64_t* partials, unsigned* bad) {
+  F acc[1] = {gl::zero<F>()};
+  unsigned b = 0;
+  for (size_t i = gtid(); i < count; i += (size_t)gridDim.x * blockDim.x) {
+    F p = gl::one<F>();
+    for (int j = 0; j < k; j++) {
+      uint64_t wd[3] = {0, 0, 0};
+#pragma unroll
+      for (int c = 0; c < F::D; c++) {
+        wd[c] = st[j][i * F::D + c];
+        b |= wd[c] >= gl::P;
+      }
+      p = gl::mul(p, gl::load_words<F>(wd), w);
+    }
+    acc[0] = gl::add(acc[0], p);
+  }
+  if (b) atomicOr(bad, 1u);
+  finish<F, 1>(acc, Reduce{partials, nullptr});
+}
+
+}  // namespace
+
+extern "C" {
+
+int effsc_hip_stream_open(const char* path, int ext, effsc_hip_stream** out) {
+  if (!path || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  *out = nullptr;
+  if (ext != 1 && ext != 3) return fail(EFFSC_HIP_EINVAL, "ext must be 1 or 3");
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fail(EFFSC_HIP_EINVAL, std::string("cannot open ") + path);
+  struct stat st;
+  if (fstat(fd, &st) != 0) {
+    close(fd);
+    return fail(EFFSC_HIP_EINVAL, "fstat failed");
+  }
+  const size_t bytes = (size_t)st.st_size, el = 8 * (size_t)ext;
+  const size_t n = bytes / el;
+  // FileStream::new asserts a power-of-two element count (file/core.rs:34-36)
+  if (bytes % el || !is_pow2(n)) {
+    close(fd);
+    return fail(EFFSC_HIP_EINVAL, "a stream holds a power-of-two number of serialized elements");
+  }
+  void* m = mmap(nullptr, bytes, PROT_READ, MAP_SHARED, fd, 0);
+  if (m == MAP_FAILED) {
+    close(fd);
+    return fail(EFFSC_HIP_EINVAL, "mmap failed");
+  }
+  auto* s = new effsc_hip_stream();
+  s->fd = fd;
+  s->map = static_cast<const uint8_t*>(m);
+  s->bytes = bytes;
+  s->ext = ext;
+  s->n = n;
+  while (((size_t)1 << s->v) < n) s->v++;
+  *out = s;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_stream_num_variables(const effsc_hip_stream* s, size_t* v) {
+  if (!s || !v) return fail(EFFSC_HIP_EINVAL, "null argument");
+  *v = s->v;
+  return EFFSC_HIP_OK;
+}
+
+int effsc_hip_stream_read(const effsc_hip_stream* s, size_t first, size_t count, uint64_t* dst) {
+  // FileStream::evaluation (file/core.rs:45-49, 66-68), `count` elements
+  if (!s || (count && !dst)) return fail(EFFSC_HIP_EINVAL, "null argument");
+  if (first > s->n || count > s->n - first) return fail(EFFSC_HIP_EINVAL, "read past the end of the stream");
+  std::memcpy(dst, s->map + first * 8 * s->ext, count * 8 * s->ext);
+  for (size_t i = 0; i < count * s->ext; i++)
+    if (dst[i] >= gl::P) return fail(EFFSC_HIP_EINVAL, "non-canonical serialized element");
+  return EFFSC_HIP_OK;
+}
+
+void effsc_hip_stream_close(effsc_hip_stream* s) {
+  if (!s) return;
+  if (s->map) munmap(const_cast<uint8_t*>(s->map), s->bytes);
+  if (s->fd >= 0) close(s->fd);
+  delete s;
+}
+
+int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, uint64_t w, uint64_t* out) {
+  // multivariate_claim / multivariate_product_claim (src/streams/stream.rs:3-41)
+  if (!streams || k < 1 || k > 8 || !out) return fail(EFFSC_HIP_EINVAL, "bad argument");
+  const int ext = streams[0]->ext;
+  const size_t n = streams[0]->n;
+  for (int j = 0; j < k; j++) {
+    if (!streams[j]) return fail(EFFSC_HIP_EINVAL, "null stream");
+    if (streams[j]->n != n || streams[j]->ext != ext)
+      return fail(EFFSC_HIP_EINVAL, "streams differ in length or field");  // stream.rs:26-30
+  }
+  HIPCHK(hipSetDevice(device));
+  const size_t B = std::min(n, stream_chunk_units()), nch = (n + B - 1) / B;
+  const unsigned G = (unsigned)std::min<size_t>((B + kBlock - 1) / kBlock, 1024);
+  const size_t el = 8 * (size_t)ext;
+  hipStream_t cs = nullptr, ks = nullptr;
+  hipEvent_t copied[2] = {}, used[2] = {};
+  uint8_t* pin[2] = {};
+  uint64_t *dst[2] = {}, *parts = nullptr, *res = nullptr;
+  const uint64_t** dptrs = nullptr;
+  unsigned* bad = nullptr;
+  int rc = EFFSC_HIP_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == EFFSC_HIP_OK) rc = fail(EFFSC_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return rc == EFFSC_HIP_OK;
+  };
+  chk(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream");
+  chk(hipStreamCreateWithFlags(&ks, hipStreamNonBlocking), "stream");
+  for (int b = 0; b < 2 && rc == EFFSC_HIP_OK; b++) {
+    chk(hipHostMalloc((void**)&pin[b], k * B * el, hipHostMallocDefault), "pinned staging");
+    chk(hipMalloc((void**)&dst[b], k * B * el), "device staging");
+    chk(hipEventCreateWithFlags(&copied[b], hipEventDisableTiming), "event");
+    chk(hipEventCreateWithFlags(&used[b], hipEventDisableTiming), "event");
+  }
+  if (rc == EFFSC_HIP_OK) chk(hipMalloc((void**)&parts, nch * G * 3 * 8), "partials");
+  if (rc == EFFSC_HIP_OK) chk(hipMalloc((void**)&res, 3 * 8), "result");
+  if (rc == EFFSC_HIP_OK) chk(hipMalloc((void**)&dptrs, 2 * 8 * sizeof(uint64_t*)), "pointers");
+  if (rc == EFFSC_HIP_OK) chk(hipMalloc((void**)&bad, 4), "flag");
+  if (rc == EFFSC_HIP_OK) chk(hipMemsetAsync(bad, 0, 4, ks), "flag");
+  if (rc == EFFSC_HIP_OK) {
+    std::vector<const uint64_t*> hp(16);
+    for (int b = 0; b < 2; b++)
+      for (int j = 0; j < k; j++) hp[b * 8 + j] = dst[b] + (size_t)j * B * ext;
+    chk(hipMemcpy(dptrs, hp.data(), 16 * sizeof(uint64_t*), hipMemcpyHostToDevice), "pointers");
+  }
+  for (size_t c = 0; c < nch && rc == EFFSC_HIP_OK; c++) {
+    const int b = (int)(c & 1);
+    const size_t first = c * B, cnt = std::min(B, n - first);
+    if (c >= 2 && !chk(hipEventSynchronize(used[b]), "wait")) break;  // staging b is free again
+    for (int j = 0; j < k; j++) par_copy(pin[b] + (size_t)j * B * el, streams[j]->map + first * el, cnt * el);
+    for (int j = 0; j < k && rc == EFFSC_HIP_OK; j++)
+      chk(hipMemcpyAsync(dst[b] + (size_t)j * B * ext, pin[b] + (size_t)j * B * el, cnt * el, hipMemcpyHostToDevice,
+                         cs), "copy");
+    chk(hipEventRecord(copied[b], cs), "event");
+    chk(hipStreamWaitEvent(ks, copied[b], 0), "wait");
+    if (ext == 1)
+      hipLaunchKernelGGL(k_stream_claim<Fp>, dim3(G), dim3(kBlock), 0, ks, dptrs + b * 8, k, cnt, w,
+                         parts + c * G, bad);
+    else
+      hipLaunchKernelGGL(k_stream_claim<Fp3>, dim3(G), dim3(kBlock), 0, ks, dptrs + b * 8, k, cnt, w,
+                         parts + c * G * 3, bad);
+    chk(hipGetLastError(), "claim kernel");
+    chk(hipEventRecord(used[b], ks), "event");
+  }
+  if (rc == EFFSC_HIP_OK) {
+    if (ext == 1)
+      hipLaunchKernelGGL((k_reduce<Fp, 1>), dim3(1), dim3(1024), 0, ks, Reduce{parts, res}, (unsigned)(nch * G));
+    else
+      hipLaunchKernelGGL((k_reduce<Fp3, 1>), dim3(1), dim3(1024), 0, ks, Reduce{parts, res}, (unsigned)(nch * G));
+    chk(hipGetLastError(), "reduce");
+    uint64_t hres[4] = {0, 0, 0, 0};
+    unsigned hbad = 0;
+    chk(hipMemcpyAsync(hres, res, 8 * ext, hipMemcpyDeviceToHost, ks), "copy back");
+    chk(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ks), "copy back");
+    chk(hipStreamSynchronize(ks), "sync");
+    if (rc == EFFSC_HIP_OK && hbad) rc = fail(EFFSC_HIP_EINVAL, "non-canonical serialized element");
+    if (rc == EFFSC_HIP_OK) std::copy(hres, hres + ext, out);
+  }
+  if (ks) (void)hipStreamSynchronize(ks);
+  if (cs) (void)hipStreamSynchronize(cs);
+  for (int b = 0; b < 2; b++) {
+    if (pin[b]) (void)hipHostFree(pin[b]);
+    if (dst[b]) (void)hipFree(dst[b]);
+    if (copied[b]) (void)hipEventDestroy(copied[b]);
+    if (used[b]) (void)hipEventDestroy(used[b]);
+  }
+  for (void* p : {(void*)parts, (void*)res, (void*)dptrs, (void*)bad})
+    if (p) (void)hipFree(p);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (ks) (void)hipStreamDestroy(ks);
+  return rc;
+}
+
+int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams,
+                                effsc_hip_prover** out) {
+  // MultilinearProver / InnerProductProver over tables read from streams
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  *out = nullptr;
+  TRY(check_cfg(cfg));
+  if (kind != K_ML && kind != K_IP) return fail(EFFSC_HIP_EINVAL, "stream provers: multilinear (1) or inner product (2)");
+  if (cfg->input_ext != 1) return fail(EFFSC_HIP_EINVAL, "stream provers read base-field streams");
+  const int K = kind;
+  if (!streams) return fail(EFFSC_HIP_EINVAL, "null streams");
+  for (int i = 0; i < K; i++)
+    if (!streams[i] || streams[i]->ext != 1) return fail(EFFSC_HIP_EINVAL, "stream provers read base-field streams");
+  const size_t n = streams[0]->n;
+  if (K == 2 && streams[1]->n != n) return fail(EFFSC_HIP_EINVAL, "a and b must have equal length");
+  auto* h = new effsc_hip_prover();
+  h->kind = kind;
+  h->ext = cfg->ext;
+  h->in_ext = 1;
+  h->K = K;
+  h->deg = K;
+  h->repr = cfg->repr;
+  h->dev = cfg->device;
+  h->w = cfg->ext == 3 ? gl::canon(cfg->ext_w) : 0;
+  h->n = n;
+  h->len = n;
+  h->twostep = h->ext == 1 && !std::getenv("EFFSC_HIP_ONE_ROUND");
+  int rc = setup_common(h, std::max<size_t>(n, 1) >> 1);
+  h->own_in = true;
+  h->in_stride = round_up(std::max<size_t>(n, 1), 64);
+  for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++) rc = alloc_dev(&h->in[i], h->in_stride);
+  unsigned* bad = nullptr;
+  if (rc == EFFSC_HIP_OK) rc = alloc_dev(reinterpret_cast<uint64_t**>(&bad), 1);
+  if (rc == EFFSC_HIP_OK) rc = alloc_dev(&h->d_pass0, kMaxK + kMaxK * (kMaxK + 1));
+  hipStream_t cs = nullptr;
+  hipEvent_t copied[2] = {};
+  uint8_t* pin[2] = {};
+  uint64_t* parts = nullptr;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == EFFSC_HIP_OK) rc = fail(EFFSC_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return rc == EFFSC_HIP_OK;
+  };
+  if (rc == EFFSC_HIP_OK) chk(hipMemsetAsync(bad, 0, 4, h->stream), "flag");
+  const size_t q = n / 4;
+  const bool pass = h->twostep && n >= 16;
+  if (rc == EFFSC_HIP_OK && !pass) {
+    // small tables: one copy, no overlap worth having
+    for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++) {
+      chk(hipMemcpy(h->in[i], streams[i]->map, n * 8, hipMemcpyHostToDevice), "copy");
+      hipLaunchKernelGGL(k_check_canonical, dim3(blocks_for(h, n)), dim3(kBlock), 0, h->stream, h->in[i], n, bad);
+      chk(hipGetLastError(), "check");
+    }
+  } else if (rc == EFFSC_HIP_OK) {
+    // chunk c moves the units [c B, (c + 1) B): four quarter slices of each
+    // table, so the pass over those units can run as soon as they land
+    const size_t B = std::min(q, stream_chunk_units()), nch = (q + B - 1) / B;
+    const unsigned G = (unsigned)std::min<size_t>((B + kBlock - 1) / kBlock, 1024);
+    constexpr int NVmax = kMaxK + kMaxK * (kMaxK + 1);
+    chk(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream");
+    for (int b = 0; b < 2 && rc == EFFSC_HIP_OK; b++) {
+      chk(hipHostMalloc((void**)&pin[b], (size_t)K * 4 * B * 8, hipHostMallocDefault), "pinned staging");
+      chk(hipEventCreateWithFlags(&copied[b], hipEventDisableTiming), "event");
+    }
+    if (rc == EFFSC_HIP_OK) chk(hipMalloc((void**)&parts, nch * G * NVmax * 8), "partials");
+    Views<Fp, 2> tv{};
+    for (int i = 0; i < K; i++) tv.t[i] = View<Fp>{h->in[i], h->in_stride};
+    for (size_t c = 0; c < nch && rc == EFFSC_HIP_OK; c++) {
+      const int b = (int)(c & 1);
+      const size_t u0 = c * B, cnt = std::min(B, q - u0);
+      // pinned buffer b was last read by the copy of chunk c - 2
+      if (c >= 2 && !chk(hipEventSynchronize(copied[b]), "wait")) break;
+      for (int i = 0; i < K; i++)
+        for (int j = 0; j < 4; j++)
+          par_copy(pin[b] + ((size_t)i * 4 + j) * B * 8, streams[i]->map + (j * q + u0) * 8, cnt * 8);
+      for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++)
+        for (int j = 0; j < 4 && rc == EFFSC_HIP_OK; j++)
+          chk(hipMemcpyAsync(h->in[i] + j * q + u0, pin[b] + ((size_t)i * 4 + j) * B * 8, cnt * 8,
+                             hipMemcpyHostToDevice, cs), "copy");
+      chk(hipEventRecord(copied[b], cs), "event");
+      chk(hipStreamWaitEvent(h->stream, copied[b], 0), "wait");
+      if (K == 1) {
+        Views<Fp, 1> t1{};
+        t1.t[0] = tv.t[0];
+        hipLaunchKernelGGL(k_stream_pass<1>, dim3(G), dim3(kBlock), 0, h->stream, t1, q, u0, cnt,
+                           parts + c * G * pass_values<1>(), bad);
+      } else {
+        hipLaunchKernelGGL(k_stream_pass<2>, dim3(G), dim3(kBlock), 0, h->stream, tv, q, u0, cnt,
+                           parts + c * G * pass_values<2>(), bad);
+      }
+      chk(hipGetLastError(), "stream pass");
+    }
+    if (rc == EFFSC_HIP_OK) {
+      if (K == 1)
+        hipLaunchKernelGGL((k_reduce<Fp, pass_values<1>()>), dim3(1), dim3(1024), 0, h->stream,
+                           Reduce{parts, h->d_pass0}, (unsigned)(nch * G));
+      else
+        hipLaunchKernelGGL((k_reduce<Fp, pass_values<2>()>), dim3(1), dim3(1024), 0, h->stream,
+                           Reduce{parts, h->d_pass0}, (unsigned)(nch * G));
+      chk(hipGetLastError(), "reduce");
+    }
+  }
+  unsigned hbad = 0;
+  if (rc == EFFSC_HIP_OK) chk(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, h->stream), "copy back");
+  if (rc == EFFSC_HIP_OK) chk(hipStreamSynchronize(h->stream), "sync");
+  if (cs) (void)hipStreamSynchronize(cs);
+  if (rc == EFFSC_HIP_OK && hbad) rc = fail(EFFSC_HIP_EINVAL, "non-canonical serialized element");
+  for (int b = 0; b < 2; b++) {
+    if (pin[b]) (void)hipHostFree(pin[b]);
+    if (copied[b]) (void)hipEventDestroy(copied[b]);
+  }
+  if (parts) (void)hipFree(parts);
+  if (bad) (void)hipFree(bad);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  h->pass0_ready = pass;
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
+}  // extern "C"

@@ -263,6 +263,31 @@ void effsc_hip_batch_free(effsc_hip_batch* b);
 /* ------------------------------------------------------------ utilities */
 
 /* ---------------------------------------------------------------------------
+ * Streamed input.  FileStream (src/streams/file/core.rs:13-76): a file of
+ * 2^v serialized elements, each component one little-endian canonical u64
+ * (ark-serialize uncompressed), mapped read-only.
+ * ------------------------------------------------------------------------- */
+typedef struct effsc_hip_stream effsc_hip_stream;
+/* FileStream::new (file/core.rs:29-43): EINVAL unless the file holds a
+ * power-of-two number of ext-word elements. */
+int effsc_hip_stream_open(const char* path, int ext, effsc_hip_stream** out);
+int effsc_hip_stream_num_variables(const effsc_hip_stream* s, size_t* v);
+/* FileStream::evaluation for `count` consecutive points (canonical words). */
+int effsc_hip_stream_read(const effsc_hip_stream* s, size_t first, size_t count, uint64_t* dst);
+void effsc_hip_stream_close(effsc_hip_stream* s);
+/* multivariate_claim (k = 1) / multivariate_product_claim
+ * (src/streams/stream.rs:3-41): sum_i prod_j s_j[i] on the device, the
+ * streams moved in chunks through pinned buffers (ext words out; w is the
+ * extension non-residue when ext = 3). */
+int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, uint64_t w, uint64_t* out);
+/* MultilinearProver (kind 1) / InnerProductProver (kind 2) over base-field
+ * tables read from streams.  The upload runs chunk by chunk on a copy stream
+ * while the first two-round pass over the landed chunks runs on the prover's
+ * stream; effsc_hip_sumcheck then starts from that pass.  cfg.input_ext = 1. */
+int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams,
+                                effsc_hip_prover** out);
+
+/* ---------------------------------------------------------------------------
  * Served proofs: a request loop with round-level continuous batching.  No
  * reference counterpart: each request is one runner::sumcheck
  * (src/runner.rs:29-69) call with the no-op hook, and its proof is exactly
