@@ -194,6 +194,8 @@ namespace {
 
 constexpr size_t kMaxRounds = 64;
 
+constexpr int kPassGridScale = 4;
+
 int alloc_dev(uint64_t** p, size_t words) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(words, 1) * 8);
   if (e != hipSuccess) {
@@ -304,8 +306,13 @@ struct Ops {
     Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
     FO c = chal ? from_words<FO>(chal) : gl::zero<FO>();
     bool vec = (q % 2 == 0) && (src_stride % 2 == 0) && (h->work_stride % 2 == 0);
+    if constexpr (FO::D == 3 && K >= 3) vec = false;
     TRY(prof_begin(h));
-    if (vec) {
+    if constexpr (FO::D == 3 && K >= 3) {
+      unsigned g = blocks_for(h, q);
+      hipLaunchKernelGGL((k_fused_seq<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w,
+                         h->red);
+    } else if (vec) {
       unsigned g = blocks_for(h, q / 2);
       hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w, h->red);
     } else {
@@ -314,8 +321,9 @@ struct Ops {
     }
     HIPCHK(hipGetLastError());
     // algorithmic bytes: read 4q elements, write 2q elements, per table
+    // per quadruple: 2 folded elements per table, one evaluated pair
     TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D),
-                 2.0 * q * (K * fold_cost<FI, FO>() + eval_products(K) * mul_cost<FO>())));
+                 (double)q * (2.0 * K * fold_cost<FI, FO>() + eval_products(K) * mul_cost<FO>())));
     return reduce<FO, K>(h, vec ? blocks_for(h, q / 2) : blocks_for(h, q));
   }
   static int fold(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t half,
@@ -520,7 +528,12 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   Views<Fp, K> out = views<Fp, K>(h->work, h->work_stride);
   const size_t L = 4 * q;
   bool vec = (q % 2 == 0) && (sst % 2 == 0) && (h->work_stride % 2 == 0);
-  unsigned g = blocks_for(h, vec ? q / 2 : q);
+  // a pass streams the whole table once: four workgroups per CU-slot of
+  // grid_max keep more loads in flight (measured 5.3 against 5.05 TB/s for
+  // the v = 28 inner-product read pass)
+  const size_t units = vec ? q / 2 : q;
+  unsigned g = (unsigned)std::max<size_t>(
+      1, std::min<size_t>((units + kBlock - 1) / kBlock, (size_t)h->grid_max * kPassGridScale));
   TRY(prof_begin(h));
   if (vec)
     hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
@@ -708,7 +721,8 @@ int setup_common(effsc_hip_prover* h, size_t cap) {
   HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->grid_max = grid_max_for(h->dev);
-  TRY(alloc_dev(&h->red.partials, (size_t)h->grid_max * kMaxK * 3));
+  // the two-round passes launch up to kPassGridScale * grid_max workgroups
+  TRY(alloc_dev(&h->red.partials, (size_t)h->grid_max * kPassGridScale * kMaxK * 3));
   TRY(alloc_dev(&h->red.result, kMaxK * 3));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_res), kMaxK * 3 * 8, hipHostMallocDefault));
   h->work_stride = round_up(std::max<size_t>(cap, 1), 64);

@@ -265,6 +265,53 @@ __global__ __launch_bounds__(kBlock) void k_eval(Views<F, K> t, size_t half, siz
 
 // Fold the pow2 length-L tables `in` with c into `out` (length L/2) and
 // evaluate the next round polynomial over (k, k + L/4).  `in` and `out` may
+// F_p^3 products of K >= 3 tables: the same fold and evaluation as k_fused,
+// one table at a time, keeping running products h(0) = prod lo, h(inf) =
+// prod (hi - lo), h(t) = prod (lo + t (hi - lo)) instead of every folded pair.
+// At most 256 VGPRs (two waves per SIMD): this kernel is bound by the
+// integer-multiply rate, and the all-tables-at-once form needs 1 wave.
+template <class FI, class FO, int K>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fused_seq(
+    Views<FI, K> in, Views<FO, K> out, FO c, const uint64_t* cptr, size_t q, uint64_t w, Reduce r) {
+  if (cptr) c = gl::load_words<FO>(cptr);
+  FO acc[K];
+#pragma unroll
+  for (int i = 0; i < K; i++) acc[i] = gl::zero<FO>();
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < q; k += step) {
+    FO p[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      FI x0 = in.t[i].ld(k), x1 = in.t[i].ld(k + q), x2 = in.t[i].ld(k + 2 * q), x3 = in.t[i].ld(k + 3 * q);
+      const FO lo = fold1<FO>(x0, x2, c, w), hi = fold1<FO>(x1, x3, c, w);
+      out.t[i].st(k, lo);
+      out.t[i].st(k + q, hi);
+      const FO d = gl::sub(hi, lo);
+      FO x = hi;
+      if (i == 0) {
+        p[0] = lo;
+        p[1] = d;
+#pragma unroll
+        for (int t = 2; t < K; t++) {
+          x = gl::add(x, d);
+          p[t] = x;
+        }
+      } else {
+        p[0] = gl::mul(p[0], lo, w);
+        p[1] = gl::mul(p[1], d, w);
+#pragma unroll
+        for (int t = 2; t < K; t++) {
+          x = gl::add(x, d);
+          p[t] = gl::mul(p[t], x, w);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) acc[t] = gl::add(acc[t], p[t]);
+  }
+  finish<FO, K>(acc, r);
+}
+
 // alias: positions k and k+q are read and written by the same thread only,
 // positions k+2q and k+3q are only read.  q = L/4.  VEC = 2 requires even q.
 template <class FI, class FO, int K, int VEC>
