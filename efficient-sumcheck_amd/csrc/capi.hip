@@ -343,11 +343,15 @@ struct Ops {
 
 // Tail threshold: once the tables have at most this many elements, one
 // workgroup runs every remaining round (k_tail).
-size_t tail_max() {
+size_t tail_max(const effsc_hip_prover* h) {
   if (const char* e = std::getenv("EFFSC_HIP_TAIL")) {
     long long x = std::atoll(e);
     if (x >= 2) return (size_t)x;
   }
+  // one workgroup is one CU: hand it only what a CU finishes faster than a
+  // multi-workgroup round plus its reduction (about 20-30 us).  A base-field
+  // pair costs a few products, an F_p^3 product-of-K pair about 12 K.
+  if (h->ext == 3) return h->K >= 3 ? (size_t)1 << 10 : (size_t)1 << 12;
   return (size_t)1 << 14;
 }
 
@@ -363,7 +367,7 @@ struct DevOps {
                   bool have, size_t first_round, const uint64_t* cur) {
     Views<FI, K> in = views<FI, K>(src, sst);
     Views<F, K> work = views<F, K>(h->work, h->work_stride);
-    hipLaunchKernelGGL((k_tail<FI, F, K>), dim3(1), dim3(kTailBlock), 0, h->stream, in, work, from_input ? 1 : 0, L,
+    hipLaunchKernelGGL((k_tail<FI, F, K>), dim3(1), dim3(tail_block<F, K>()), 0, h->stream, in, work, from_input ? 1 : 0, L,
                        rounds, have ? 1 : 0, h->d_fs, h->ext, h->d_polys + first_round * K * h->ext,
                        h->d_chals + first_round * h->ext, const_cast<uint64_t*>(cur), h->d_fin, h->w);
     HIPCHK(hipGetLastError());
@@ -893,6 +897,193 @@ int eq_fold_table(effsc_hip_prover* h, uint64_t* planes, size_t stride, size_t* 
 }
 
 // Reads element 0 of an ext-plane table.
+// ---------------------------------------------------------------------------
+// Device-resident EqFactored proof.  A round is the inner-product round of
+// (p, E) with E = eq_l (x) eq_r while the left variables are bound and
+// E = eq_l[0] eq_r afterwards (round_poly_left / round_poly_right,
+// src/provers/eq_factored.rs:139-209): q0 = sum E_lo p_lo and
+// q_inf = sum (E_hi - E_lo)(p_hi - p_lo).  The eq tables are tiny next to p
+// and stay in L2; each is folded in place right after the challenge of its
+// variable is squeezed.
+// ---------------------------------------------------------------------------
+
+// Weights of pair k (half H2 of the evaluated table): left phase, index
+// k = a m + b with m = 2^m_log = |eq_r|; right phase, eq_r alone.
+template <class F>
+__device__ __forceinline__ void eq_weights(View<F> eql, size_t eql_len, View<F> eqr, size_t m_log, int left, size_t k,
+                                           size_t H2, uint64_t w, F& el, F& eh) {
+  if (left) {
+    const size_t a = k >> m_log, b = k & (((size_t)1 << m_log) - 1);
+    const F er = eqr.ld(b);
+    el = gl::mul(eql.ld(a), er, w);
+    eh = gl::mul(eql.ld(a + (eql_len >> 1)), er, w);
+  } else {
+    el = eqr.ld(k);
+    eh = eqr.ld(k + H2);
+  }
+}
+
+// One round over p of length L before the round's fold.  With `fold`, p is
+// folded with the device challenge on the fly (quadruples, in place when pin
+// and pout alias: thread k alone reads k, k+q, k+2q, k+3q and writes k, k+q).
+template <class FI, class F>
+__global__ __launch_bounds__(kBlock) void k_eq_round(View<FI> pin, View<F> pout, size_t L, int fold,
+                                                     const uint64_t* cptr, View<F> eql, size_t eql_len, View<F> eqr,
+                                                     size_t m_log, int left, uint64_t w, Reduce r) {
+  const F c = fold ? gl::load_words<F>(cptr) : gl::zero<F>();
+  const size_t H2 = fold ? L / 4 : L / 2;
+  F acc[2] = {gl::zero<F>(), gl::zero<F>()};
+  for (size_t k = gtid(); k < H2; k += (size_t)gridDim.x * blockDim.x) {
+    F lo, hi;
+    if (fold) {
+      FI x0 = pin.ld(k), x1 = pin.ld(k + H2), x2 = pin.ld(k + 2 * H2), x3 = pin.ld(k + 3 * H2);
+      lo = fold1<F>(x0, x2, c, w);
+      hi = fold1<F>(x1, x3, c, w);
+      pout.st(k, lo);
+      pout.st(k + H2, hi);
+    } else {
+      lo = gl::lift_to<F>(pin.ld(k));
+      hi = gl::lift_to<F>(pin.ld(k + H2));
+    }
+    F el, eh;
+    eq_weights<F>(eql, eql_len, eqr, m_log, left, k, H2, w, el, eh);
+    acc[0] = gl::add(acc[0], gl::mul(el, lo, w));
+    acc[1] = gl::add(acc[1], gl::mul(gl::sub(eh, el), gl::sub(hi, lo), w));
+  }
+  finish<F, 2>(acc, r);
+}
+
+// Sums the round's partials, scales by eq_l[0] in the right phase, runs the
+// transcript step, then folds the eq table of this round's variable in
+// place (fold_half > 0) with the new challenge.
+template <class F>
+__global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs(Reduce r, unsigned nblocks, View<F> eql, int right,
+                                                               DevFs* t, int ext, uint64_t* poly, uint64_t* chal_out,
+                                                               uint64_t* chal_cur, View<F> ft, size_t fold_half,
+                                                               uint64_t w) {
+  __shared__ uint64_t chal_sh[3];
+  F s[2] = {gl::zero<F>(), gl::zero<F>()};
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < 2; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * 2 + i) * F::D));
+  block_reduce<F, 2>(s);
+  if (right) {
+    const F sc = eql.ld(0);
+    s[0] = gl::mul(s[0], sc, w);
+    s[1] = gl::mul(s[1], sc, w);
+  }
+  if (threadIdx.x < 64) emit_round<F, 2>(s, ext, t, poly, chal_out, chal_sh);
+  __syncthreads();
+  if (threadIdx.x < (unsigned)ext) chal_cur[threadIdx.x] = chal_sh[threadIdx.x];
+  const F c = gl::load_words<F>(chal_sh);
+  for (size_t k = threadIdx.x; k < fold_half; k += blockDim.x) ft.st(k, fold1<F>(ft.ld(k), ft.ld(k + fold_half), c, w));
+}
+
+template <class F>
+constexpr int eq_tail_block() {
+  return F::D == 3 ? 256 : 1024;
+}
+
+// Rounds [j0, v) in one workgroup, then the final fold and the final value
+// p(r) eq_l(r_L) eq_r(r_R) (src/provers/eq_factored.rs:272-288).
+template <class FI, class F>
+__global__ __launch_bounds__((eq_tail_block<F>())) void k_eq_tail(View<FI> pin, View<F> work, int from_input,
+                                                                   size_t L, int j0, int v, int v_l, View<F> eql,
+                                                                   size_t eql_len, View<F> eqr, size_t eqr_len,
+                                                                   int have_chal, const uint64_t* cur_in, DevFs* t,
+                                                                   int ext, uint64_t* polys, uint64_t* chals,
+                                                                   uint64_t* cur_out, uint64_t* fin, uint64_t w) {
+  __shared__ uint64_t chal_sh[3];
+  if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal && (int)threadIdx.x < ext ? cur_in[threadIdx.x] : 0;
+  __syncthreads();
+  const unsigned tid = threadIdx.x, nt = blockDim.x;
+  bool src_in = from_input != 0;
+  for (int j = j0; j < v; j++) {
+    const int left = j < v_l;
+    size_t m_log = 0;
+    while (((size_t)1 << m_log) < eqr_len) m_log++;
+    F acc[2] = {gl::zero<F>(), gl::zero<F>()};
+    if (have_chal) {
+      const F c = gl::load_words<F>(chal_sh);
+      const size_t q = L / 4;
+      for (size_t k = tid; k < q; k += nt) {
+        F lo, hi;
+        if (src_in) {
+          FI x0 = pin.ld(k), x1 = pin.ld(k + q), x2 = pin.ld(k + 2 * q), x3 = pin.ld(k + 3 * q);
+          lo = fold1<F>(x0, x2, c, w);
+          hi = fold1<F>(x1, x3, c, w);
+        } else {
+          F x0 = work.ld(k), x1 = work.ld(k + q), x2 = work.ld(k + 2 * q), x3 = work.ld(k + 3 * q);
+          lo = fold1<F>(x0, x2, c, w);
+          hi = fold1<F>(x1, x3, c, w);
+        }
+        work.st(k, lo);
+        work.st(k + q, hi);
+        F el, eh;
+        eq_weights<F>(eql, eql_len, eqr, m_log, left, k, q, w, el, eh);
+        acc[0] = gl::add(acc[0], gl::mul(el, lo, w));
+        acc[1] = gl::add(acc[1], gl::mul(gl::sub(eh, el), gl::sub(hi, lo), w));
+      }
+      L /= 2;
+      src_in = false;
+    } else {
+      const size_t half = L / 2;
+      for (size_t k = tid; k < half; k += nt) {
+        F lo, hi;
+        if (src_in) {
+          lo = gl::lift_to<F>(pin.ld(k));
+          hi = gl::lift_to<F>(pin.ld(k + half));
+        } else {
+          lo = work.ld(k);
+          hi = work.ld(k + half);
+        }
+        F el, eh;
+        eq_weights<F>(eql, eql_len, eqr, m_log, left, k, half, w, el, eh);
+        acc[0] = gl::add(acc[0], gl::mul(el, lo, w));
+        acc[1] = gl::add(acc[1], gl::mul(gl::sub(eh, el), gl::sub(hi, lo), w));
+      }
+    }
+    block_reduce<F, 2>(acc);
+    if (!left) {
+      const F sc = eql.ld(0);
+      acc[0] = gl::mul(acc[0], sc, w);
+      acc[1] = gl::mul(acc[1], sc, w);
+    }
+    if (tid < 64)
+      emit_round<F, 2>(acc, ext, t, polys + (size_t)(j - j0) * 2 * ext, chals + (size_t)(j - j0) * ext, chal_sh);
+    __syncthreads();
+    // fold the eq table of variable j with its challenge
+    const F c = gl::load_words<F>(chal_sh);
+    if (left) {
+      const size_t hh = eql_len >> 1;
+      for (size_t k = tid; k < hh; k += nt) eql.st(k, fold1<F>(eql.ld(k), eql.ld(k + hh), c, w));
+      eql_len = hh;
+    } else {
+      const size_t hh = eqr_len >> 1;
+      for (size_t k = tid; k < hh; k += nt) eqr.st(k, fold1<F>(eqr.ld(k), eqr.ld(k + hh), c, w));
+      eqr_len = hh;
+    }
+    __syncthreads();
+    have_chal = 1;
+  }
+  if (tid == 0) {
+    // finalize: p folds 2 -> 1 with the last challenge (the eq table of the
+    // last variable was folded above)
+    F p0;
+    if (have_chal && L >= 2) {
+      const F c = gl::load_words<F>(chal_sh);
+      F lo = src_in ? gl::lift_to<F>(pin.ld(0)) : work.ld(0);
+      F hi = src_in ? gl::lift_to<F>(pin.ld(1)) : work.ld(1);
+      p0 = fold1<F>(lo, hi, c, w);
+      work.st(0, p0);
+    } else {
+      p0 = src_in ? gl::lift_to<F>(pin.ld(0)) : work.ld(0);
+    }
+    gl::store_words(gl::mul(gl::mul(p0, eql.ld(0), w), eqr.ld(0), w), fin);
+    for (int i = 0; i < ext; i++) cur_out[i] = chal_sh[i];
+  }
+}
+
 int read0(effsc_hip_prover* h, const uint64_t* planes, size_t stride, int d, uint64_t* out) {
   for (int c = 0; c < h->ext; c++) {
     out[c] = 0;
@@ -1211,7 +1402,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
     // two rounds per pass while the tables are large
     while (num_rounds - r >= 2) {
       size_t L = h->len >> p;
-      if (L <= tail_max() || L < 4) break;
+      if (L <= tail_max(h) || L < 4) break;
       Reduce red = h->red;
       unsigned g = 0;
       if (p == 0 && h->pass0_ready && !h->in_work && h->len == h->n) {
@@ -1239,7 +1430,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   } else {
     while (r < num_rounds) {
       size_t M = h->len;
-      if (M <= tail_max()) break;
+      if (M <= tail_max(h)) break;
       int fi = src_field(h);
       uint64_t* const* src = src_planes(h);
       size_t sst = src_stride(h);
@@ -1274,7 +1465,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   size_t rounds_left = num_rounds - r;
   size_t M = h->len;
   const bool have = p == 1;
-  if (rounds_left > 0 || M <= tail_max()) {
+  if (rounds_left > 0 || M <= tail_max(h)) {
     // remaining rounds, finalize and final value in one workgroup
     int fi = src_field(h);
     bool from_input = !h->in_work;
@@ -1320,6 +1511,119 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   } else {
     TRY(final_value_impl(h, final_value));
   }
+  return EFFSC_HIP_OK;
+}
+
+// EqFactored: a full proof from the fresh state on the device.  Rounds on
+// large p: k_eq_round (fold p with the device challenge, evaluate with eq
+// weights) and k_eq_reduce_fs (sum, transcript, fold the eq table of the
+// round's variable in place); once p is small, k_eq_tail runs the remaining
+// rounds, the final fold and the final value in one workgroup.
+bool eq_device_applies(const effsc_hip_prover* h, size_t num_rounds) {
+  return h->kind == K_EQ && h->rounds_elapsed == 0 && num_rounds == h->v && h->v >= 1 && h->v <= kMaxRounds &&
+         h->len == ((size_t)1 << h->v) && !h->in_work && h->eq_l_len == ((size_t)1 << h->v_l) &&
+         h->eq_r_len == ((size_t)1 << (h->v - h->v_l));
+}
+
+template <class FI, class F>
+int eq_device_run(effsc_hip_prover* h, size_t v) {
+  const int E = h->ext;
+  const size_t v_l = h->v_l;
+  // rounds on the device until p is small enough for one workgroup; eq
+  // tables folded inside the reduction when small, else by k_fold
+  const size_t tail = h->ext == 3 ? ((size_t)1 << 10) : ((size_t)1 << 12);
+  const size_t fold_in_reduce = (size_t)1 << 14;
+  size_t L = h->len, j = 0;
+  bool have = false;
+  View<F> eql{h->eq_l, h->eq_l_stride}, eqr{h->eq_r, h->eq_r_stride};
+  while (j < v && L > tail && L >= 4) {
+    const int left = j < v_l;
+    size_t m_log = 0;
+    while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+    const size_t units = have ? L / 4 : L / 2;
+    const unsigned g = blocks_for(h, units);
+    const View<FI> pin{h->in_work ? nullptr : h->in[0], h->in_stride};
+    const View<F> pw{h->work[0], h->work_stride};
+    if (have && h->in_work) {
+      hipLaunchKernelGGL((k_eq_round<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, h->d_cur, eql,
+                         h->eq_l_len, eqr, m_log, left, h->w, h->red);
+    } else {
+      hipLaunchKernelGGL((k_eq_round<FI, F>), dim3(g), dim3(kBlock), 0, h->stream, pin, pw, L, have ? 1 : 0,
+                         h->d_cur, eql, h->eq_l_len, eqr, m_log, left, h->w, h->red);
+    }
+    HIPCHK(hipGetLastError());
+    if (have) {
+      L /= 2;
+      h->in_work = true;
+    }
+    size_t* tl = left ? &h->eq_l_len : &h->eq_r_len;
+    View<F> tv = left ? eql : eqr;
+    const size_t half = *tl >> 1;
+    hipLaunchKernelGGL(k_eq_reduce_fs<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, eql, left ? 0 : 1,
+                       h->d_fs, E, h->d_polys + j * 2 * E, h->d_chals + j * E, h->d_cur, tv,
+                       half <= fold_in_reduce ? half : 0, h->w);
+    HIPCHK(hipGetLastError());
+    if (half > fold_in_reduce) {
+      Views<F, 1> t1{};
+      t1.t[0] = tv;
+      hipLaunchKernelGGL((k_fold<F, F, 1>), dim3(blocks_for(h, half)), dim3(kBlock), 0, h->stream, t1, t1,
+                         gl::zero<F>(), h->d_cur, half, half, h->w);
+      HIPCHK(hipGetLastError());
+    }
+    *tl = half;
+    have = true;
+    j++;
+  }
+  // remaining rounds, finalize and final value
+  const View<FI> pin{h->in[0], h->in_stride};
+  const View<F> pw{h->work[0], h->work_stride};
+  hipLaunchKernelGGL((k_eq_tail<FI, F>), dim3(1), dim3(eq_tail_block<F>()), 0, h->stream, pin, pw,
+                     h->in_work ? 0 : 1, L, (int)j, (int)v, (int)v_l, eql, h->eq_l_len, eqr, h->eq_r_len,
+                     have ? 1 : 0, h->d_cur, h->d_fs, E, h->d_polys + j * 2 * E, h->d_chals + j * E, h->d_cur,
+                     h->d_fin, h->w);
+  HIPCHK(hipGetLastError());
+  // post-state: every variable bound
+  h->len = 1;
+  h->in_work = true;
+  h->eq_l_len = 1;
+  h->eq_r_len = 1;
+  h->rounds_elapsed = v;
+  return EFFSC_HIP_OK;
+}
+
+int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, uint64_t* round_polys,
+                    uint64_t* challenges, uint64_t* final_value) {
+  TRY(ensure_dev_bufs(h));
+  const int E = h->ext;
+  DevFs* hs = reinterpret_cast<DevFs*>(h->h_proof + kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4);
+  std::memset(hs, 0, sizeof(DevFs));
+  hs->kind = (uint32_t)t->kind;
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) hs->st[i] = t->fs.sp.st[i];
+    hs->pos = t->fs.sp.pos;
+  }
+  hs->rng = t->rng;
+  HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+  if (E == 1)
+    TRY((eq_device_run<Fp, Fp>(h, num_rounds)));
+  else
+    TRY((eq_device_run<Fp3, Fp3>(h, num_rounds)));
+  uint64_t* hp = h->h_proof;
+  uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
+  uint64_t* hf = hc + kMaxRounds * 3;
+  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * 2 * E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::copy(hp, hp + num_rounds * 2 * E, round_polys);
+  std::copy(hc, hc + num_rounds * E, challenges);
+  std::copy(hf, hf + E, final_value);
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) t->fs.sp.st[i] = hs->st[i];
+    t->fs.sp.pos = hs->pos;
+  }
+  t->rng = hs->rng;
   return EFFSC_HIP_OK;
 }
 
@@ -1830,6 +2134,8 @@ int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcr
   HIPCHK(hipSetDevice(h->dev));
   if (!hook && device_loop_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
     return prove_device(h, num_rounds, t, round_polys, challenges, final_value);
+  if (!hook && eq_device_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
+    return prove_device_eq(h, num_rounds, t, round_polys, challenges, final_value);
   const int E = h->ext;
   const size_t nw = (size_t)std::max(h->deg, 1);
   uint64_t prev[3];

@@ -9,8 +9,9 @@
 //
 // Kernels (VALU; the layers are far too small for matrix cores):
 //   k_mlp_forward   one thread per sample, parameters staged in LDS
-//   k_mlp_backward  one thread per sample: forward, then the per-sample
-//                   gradient of (y - t)^2 / n into row s of a scratch matrix
+//   k_mlp_backward  one workgroup per sample (a thread per hidden unit):
+//                   forward, then the per-sample gradient of (y - t)^2 / n
+//                   into row s of a scratch matrix
 //   k_mlp_reduce    one thread per parameter sums its column in sample order
 //                   (deterministic), plus the loss
 //   k_mlp_adam      one thread per parameter
@@ -74,41 +75,62 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(Shape S, const float*
 }
 
 // Row s of G: gradient of (y_s - t_s)^2 / n; E[s] = (y_s - t_s)^2 / n.
+// One workgroup per sample: thread i < H owns hidden unit i, activations and
+// deltas live in LDS next to the parameters.
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_backward(Shape S, const float* params, const float* X,
                                                             const float* T, size_t n, float* G, float* E) {
   extern __shared__ float sp[];
-  stage_params(params, sp, S.count());
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  float x[kMaxIn], h1[kMaxHidden], h2[kMaxHidden];
-  for (int k = 0; k < S.D; k++) x[k] = X[s * S.D + k];
-  const float y = forward1(S, sp, x, h1, h2);
-  const float r = y - T[s];
-  E[s] = r * r / (float)n;
+  const int P = S.count(), D = S.D, H = S.H;
+  float* xs = sp + P;
+  float* h1 = xs + kMaxIn;
+  float* h2 = h1 + kMaxHidden;
+  float* d2 = h2 + kMaxHidden;
+  float* ys = d2 + kMaxHidden;
+  const size_t s = blockIdx.x;
+  const int i = threadIdx.x;
+  for (int k = i; k < P; k += blockDim.x) sp[k] = params[k];
+  if (i < D) xs[i] = X[s * D + i];
+  __syncthreads();
+  if (i < H) {
+    float a = sp[S.b1() + i];
+    for (int k = 0; k < D; k++) a = fmaf(sp[S.w1() + i * D + k], xs[k], a);
+    h1[i] = tanhf(a);
+  }
+  __syncthreads();
+  if (i < H) {
+    float a = sp[S.b2() + i];
+    for (int k = 0; k < H; k++) a = fmaf(sp[S.w2() + i * H + k], h1[k], a);
+    h2[i] = tanhf(a);
+  }
+  __syncthreads();
+  if (i == 0) {
+    float y = sp[S.b3()];
+    for (int k = 0; k < H; k++) y = fmaf(sp[S.w3() + k], h2[k], y);
+    ys[0] = y;
+  }
+  __syncthreads();
+  const float r = ys[0] - T[s];
   const float dy = 2.0f * r / (float)n;
-  float* g = G + s * (size_t)S.count();
-  // output layer
-  g[S.b3()] = dy;
-  float d2[kMaxHidden];
-  for (int k = 0; k < S.H; k++) {
-    g[S.w3() + k] = dy * h2[k];
-    d2[k] = dy * sp[S.w3() + k] * (1.0f - h2[k] * h2[k]);  // through tanh
+  float* g = G + s * (size_t)P;
+  if (i == 0) {
+    E[s] = r * r / (float)n;
+    g[S.b3()] = dy;
   }
-  // second layer
-  float d1[kMaxHidden];
-  for (int k = 0; k < S.H; k++) d1[k] = 0.0f;
-  for (int i = 0; i < S.H; i++) {
+  if (i < H) {
+    g[S.w3() + i] = dy * h2[i];
+    d2[i] = dy * sp[S.w3() + i] * (1.0f - h2[i] * h2[i]);  // through tanh
+  }
+  __syncthreads();
+  // second layer: dW2[a][b] = d2[a] h1[b], db2 = d2
+  for (int e = i; e < H * H; e += blockDim.x) g[S.w2() + e] = d2[e / H] * h1[e % H];
+  if (i < H) {
     g[S.b2() + i] = d2[i];
-    for (int k = 0; k < S.H; k++) {
-      g[S.w2() + i * S.H + k] = d2[i] * h1[k];
-      d1[k] = fmaf(d2[i], sp[S.w2() + i * S.H + k], d1[k]);
-    }
-  }
-  // first layer
-  for (int i = 0; i < S.H; i++) {
-    const float di = d1[i] * (1.0f - h1[i] * h1[i]);
+    // first layer: d1[i] = sum_a d2[a] W2[a][i], through tanh
+    float a = 0.0f;
+    for (int k = 0; k < H; k++) a = fmaf(d2[k], sp[S.w2() + k * H + i], a);
+    const float di = a * (1.0f - h1[i] * h1[i]);
     g[S.b1() + i] = di;
-    for (int k = 0; k < S.D; k++) g[S.w1() + i * S.D + k] = di * x[k];
+    for (int k = 0; k < D; k++) g[S.w1() + i * D + k] = di * xs[k];
   }
 }
 
@@ -201,10 +223,10 @@ int upload(effsc_hip_mlp* m, const float* x, const float* t, size_t n) {
 
 int launch_grad(effsc_hip_mlp* m, size_t n) {
   const int P = m->S.count();
-  const size_t lds = (size_t)P * 4;
-  const unsigned g = (unsigned)((n + kMlpBlock - 1) / kMlpBlock);
-  hipLaunchKernelGGL(k_mlp_backward, dim3(g), dim3(kMlpBlock), lds, m->stream, m->S, m->d_params, m->d_x, m->d_t, n,
-                     m->d_G, m->d_E);
+  // parameters, x, h1, h2, d2 and y in LDS
+  const size_t lds = ((size_t)P + kMaxIn + 3 * kMaxHidden + 4) * 4;
+  hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)n), dim3(kMlpBlock), lds, m->stream, m->S, m->d_params, m->d_x,
+                     m->d_t, n, m->d_G, m->d_E);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_mlp_reduce, dim3((P + 1 + kMlpBlock - 1) / kMlpBlock), dim3(kMlpBlock), 0, m->stream, m->S,
                      m->d_G, m->d_E, n, m->d_grad);

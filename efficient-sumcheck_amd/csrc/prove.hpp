@@ -225,6 +225,12 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
 }
 
 constexpr int kTailBlock = 1024;
+// F_p^3 products of two or more tables need more than the 128 VGPRs a
+// 1024-thread workgroup allows (they spilled there): 256 threads.
+template <class F, int K>
+constexpr int tail_block() {
+  return (F::D == 3 && K >= 2) ? 256 : kTailBlock;
+}
 
 // All remaining rounds in one workgroup.  On entry the K tables have pow2
 // length L (>= 2); they are in `in` (field FI) when from_input, else in
@@ -234,7 +240,7 @@ constexpr int kTailBlock = 1024;
 // challenge folds the tables once more and the final value
 // (prod_i table_i[0] when the length reached 1, else 0) goes to fin.
 template <class FI, class F, int K>
-__global__ __launch_bounds__(kTailBlock) void k_tail(Views<FI, K> in, Views<F, K> work, int from_input, size_t L,
+__global__ __launch_bounds__((tail_block<F, K>())) void k_tail(Views<FI, K> in, Views<F, K> work, int from_input, size_t L,
                                                      int rounds, int have_chal, DevFs* t, int ext, uint64_t* polys,
                                                      uint64_t* chals, uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
   __shared__ uint64_t chal_sh[3];

@@ -110,6 +110,52 @@ __device__ __forceinline__ void serve_chunk(const SInst* ip, uint32_t j, uint64_
   // j = 1, else the work table, in place) and evaluate the folded pairs
   const uint64_t q = (uint64_t)1 << (v - j - 1);
   const FW c = gl::load_words<FW>(ip->cur);
+  if constexpr (FW::D == 3 && K >= 3) {
+    // one table at a time with running products (as k_fused_seq): the
+    // all-tables-at-once form needs more registers than two waves allow
+    for (uint64_t k = begin + threadIdx.x; k < end; k += blockDim.x) {
+      FW p[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        View<FW> d{ip->work[i], ip->ws};
+        FW lo, hi;
+        if (j == 1) {
+          const uint64_t* s = ip->in[i];
+          Fp x0 = {s[k]}, x1 = {s[k + q]}, x2 = {s[k + 2 * q]}, x3 = {s[k + 3 * q]};
+          lo = fold1<FW>(x0, x2, c, w);
+          hi = fold1<FW>(x1, x3, c, w);
+        } else {
+          FW x0 = d.ld(k), x1 = d.ld(k + q), x2 = d.ld(k + 2 * q), x3 = d.ld(k + 3 * q);
+          lo = fold1<FW>(x0, x2, c, w);
+          hi = fold1<FW>(x1, x3, c, w);
+        }
+        d.st(k, lo);
+        d.st(k + q, hi);
+        const FW dd = gl::sub(hi, lo);
+        FW x = hi;
+        if (i == 0) {
+          p[0] = lo;
+          p[1] = dd;
+#pragma unroll
+          for (int t = 2; t < K; t++) {
+            x = gl::add(x, dd);
+            p[t] = x;
+          }
+        } else {
+          p[0] = gl::mul(p[0], lo, w);
+          p[1] = gl::mul(p[1], dd, w);
+#pragma unroll
+          for (int t = 2; t < K; t++) {
+            x = gl::add(x, dd);
+            p[t] = gl::mul(p[t], x, w);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < K; t++) acc[t] = gl::add(acc[t], p[t]);
+    }
+    return;
+  }
   for (uint64_t k = begin + threadIdx.x; k < end; k += blockDim.x) {
     FW lo[K], hi[K];
 #pragma unroll
@@ -170,7 +216,8 @@ __global__ __launch_bounds__(kBlock) void k_serve_round_base(const SInst* insts,
     serve_block<2, Fp>(ip, it, gb - it.chunk0, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_serve_round_ext(const SInst* insts, const RItem* items, uint32_t nitems,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_serve_round_ext(
+    const SInst* insts, const RItem* items, uint32_t nitems,
                                                             uint64_t block0, uint64_t* partials) {
   const uint64_t gb = block0 + blockIdx.x;
   const RItem it = items[find_item(items, nitems, gb)];

@@ -132,6 +132,27 @@ def test_eq_factored_parity(ext, v):
     check_proof("eq", ext, ext, 1 << v, v, 900 + v)
 
 
+@pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 17), (1, 20), (3, 11), (3, 12), (3, 15)])
+def test_eq_factored_device_loop(ext, v, monkeypatch):
+    # large p: multi-workgroup rounds with in-reduction eq folds, then the
+    # one-workgroup tail; against the oracle and the per-round host loop
+    check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    monkeypatch.setenv("EFFSC_HIP_HOST_LOOP", "1")
+    check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    monkeypatch.delenv("EFFSC_HIP_HOST_LOOP")
+    # a hook forces the host loop too; a partial run is per round
+    wpt = O.uniform(v * ext, 960 + v)
+    tab = O.uniform((1 << v) * ext, 961 + v)
+    g = EqFactoredProver(wpt, tab, ext=ext)
+    part = sumcheck(g, 3, Transcript(b"eq-part", ext=ext))
+    rest = sumcheck(g, v - 3, Transcript(b"eq-part-2", ext=ext))
+    o = O.Prover.eq_factored(wpt, tab, ext=ext)
+    po = o.sumcheck(3, O.Transcript.shake(b"eq-part"))
+    assert tuple(as_oracle(part, ext)) == tuple(po)
+    po2 = o.sumcheck(v - 3, O.Transcript.shake(b"eq-part-2"))
+    assert tuple(as_oracle(rest, ext)) == tuple(po2)
+
+
 def test_eq_factored_short_input_is_zero_padded():
     check_proof("eq", 1, 1, 37, 7, 4242)
 
