@@ -232,9 +232,15 @@ def bench_single(args, workload):
 
     valu = spec["kind"] == "product"
     profiled = any(launches[i] for i in range(4))
+    names = list(CLASS_NAMES)
+    if spec["kind"] == "eq":
+        names[0] = "k_eq_round (fold p with the challenge + evaluate with eq weights)"
+        names[1] = "k_eq_round (round 0: evaluate with eq weights)"
+    elif spec["kind"] == "product" and ext == 3:
+        names[0] = "k_fused_seq (fold + evaluate, one F_p^3 table at a time)"
     if profiled:
         dom = max(range(4), key=lambda i: ms[i])
-        kernel = CLASS_NAMES[dom]
+        kernel = names[dom]
     if valu:
         peak = C.c_double()
         N.check(L.effsc_hip_mul_rate(dev, C.byref(peak)))
@@ -254,8 +260,8 @@ def bench_single(args, workload):
         roof = {"bound": "hbm", "achieved": alg / T / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": alg / T / 1e9 / HBM_PEAK_GBS, "traffic": None,
                 "kernel": "whole proof (algorithmic bytes / proof time; latency-bound at this size)"}
-    roof["per_class"] = {CLASS_NAMES[i].split(" ")[0]: {"launches": int(launches[i]), "ms": ms[i], "gbs": gbs[i],
-                                                        "gmul_s": gmuls[i]} for i in range(4) if launches[i]}
+    roof["per_class"] = {names[i]: {"launches": int(launches[i]), "ms": ms[i], "gbs": gbs[i], "gmul_s": gmuls[i]}
+                         for i in range(4) if launches[i]}
 
     cpu = None
     if not args.no_cpu:

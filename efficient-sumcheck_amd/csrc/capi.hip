@@ -1544,6 +1544,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     const unsigned g = blocks_for(h, units);
     const View<FI> pin{h->in_work ? nullptr : h->in[0], h->in_stride};
     const View<F> pw{h->work[0], h->work_stride};
+    TRY(prof_begin(h));
     if (have && h->in_work) {
       hipLaunchKernelGGL((k_eq_round<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, h->d_cur, eql,
                          h->eq_l_len, eqr, m_log, left, h->w, h->red);
@@ -1552,6 +1553,16 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
                          h->d_cur, eql, h->eq_l_len, eqr, m_log, left, h->w, h->red);
     }
     HIPCHK(hipGetLastError());
+    {
+      // algorithmic bytes: p read once (and its folded half written); the
+      // eq tables are L2-resident.  Products per evaluated pair: 2, plus 2
+      // for the left-phase weights; 2 folds per pair when folding.
+      const int din = (have && h->in_work) ? F::D : FI::D;
+      const double pairs = (double)units;
+      const double bytes = have ? 8.0 * (4.0 * pairs * din + 2.0 * pairs * F::D) : 8.0 * 2.0 * pairs * din;
+      const double muls = pairs * ((left ? 4.0 : 2.0) + (have ? 2.0 : 0.0)) * mul_cost<F>();
+      TRY(prof_end(h, have ? 0 : 1, bytes, muls));
+    }
     if (have) {
       L /= 2;
       h->in_work = true;
@@ -1616,6 +1627,7 @@ int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript
   HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * 2 * E, round_polys);
   std::copy(hc, hc + num_rounds * E, challenges);
   std::copy(hf, hf + E, final_value);
