@@ -1,21 +1,23 @@
 // Batched proofs: many independent multilinear / inner-product sumcheck
-// instances of uneven size advance round by round together.
+// instances of uneven size advance together, two rounds per step.
 //
-// Per round two launches serve every active instance:
-//   k_batch_round  one workgroup per chunk of one instance's work (static
-//                  chunk tables built at setup): round 0 evaluates, round
-//                  j >= 1 folds with the instance's challenge and evaluates
-//                  (the fused quadruple pass of kernels.hpp), round j = v_i
-//                  applies the last fold and writes the final value.
-//   k_batch_fs     one workgroup per active instance: sums its chunks'
-//                  partials, writes the round polynomial and runs the
-//                  Fiat-Shamir step on its own device transcript.
-// So the latency of the transcript step (one Keccak permutation per round) is
-// paid once per round for the whole batch instead of once per instance.  The
-// whole round sequence is captured once into a hipGraph and replayed for
-// every prove call; every kernel reads its challenges from device memory.
-// Each instance's proof is the one runner::sumcheck (src/runner.rs:29-69)
-// produces for that instance alone.
+// Instance i with v_i variables runs floor(v_i / 2) pass steps and one final
+// step.  Step s (two-round passes, pass2.hpp):
+//   k_bpass   one workgroup per chunk of one instance's pass units: step 0
+//             reads the input once (round 0 values and round 1
+//             coefficients); step s >= 1 folds the stored level twice on the
+//             fly with the instance's two pending challenges, stores the
+//             level-2s table and accumulates rounds 2s and 2s+1
+//   k_bfs     one workgroup per instance: after a pass, sums its chunks'
+//             partials and runs two transcript steps on the instance's device
+//             sponge (emit_two_rounds); at an instance's final step it folds
+//             the last (at most 8-element) table, runs the odd last round if
+//             there is one, and writes the final value
+// Levels after the input are materialised every other round, so the batch
+// moves about two thirds of the bytes of one fold per round.  The static
+// chunk tables are built once; the whole step sequence is captured into a
+// hipGraph and replayed for every prove call.  Each instance's proof is the
+// one runner::sumcheck (src/runner.rs:29-69) produces for that instance alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +27,7 @@
 #include "common.hpp"
 #include "effsc_hip.h"
 #include "kernels.hpp"
+#include "pass2.hpp"
 #include "prove.hpp"
 
 using namespace effsc_hip;
@@ -33,109 +36,164 @@ using gl::Fp;
 namespace {
 
 struct BInst {
-  const uint64_t* in[2];  // input tables (borrowed, canonical)
-  uint64_t* work[2];      // work tables, n/2 elements
+  const uint64_t* in[2];  // input tables (borrowed, canonical), n = 2^v words
+  uint64_t* work[2];      // level-2 (then deeper levels, in place) tables, n/4 words
   uint32_t v;             // rounds
   uint32_t first_round;   // offset of this instance in the round/challenge arrays
 };
 
 struct BChunk {
   uint32_t inst;
-  uint32_t begin;  // first unit (pair in round 0, quadruple otherwise)
+  uint32_t begin;  // first pass unit
   uint32_t count;
   uint32_t pad;
 };
 
-// Units an instance processes in round j (j <= v): round 0 evaluates n/2
-// pairs, round j in [1, v) folds and evaluates L/4 quadruples (L = 2^(v-j+1)),
-// round v applies the last fold (one unit).
-inline size_t units(uint32_t v, uint32_t j) {
-  if (j == v) return 1;
-  if (j == 0) return (size_t)1 << (v - 1);
-  return (size_t)1 << (v - j - 1);
-}
+// Per-instance item of a step: kind 0 = after a pass (range of this step's
+// partials), 1 = the instance's final step.
+struct BItem {
+  uint32_t inst, kind, part0, nparts;
+};
+
+inline uint32_t pass_steps(uint32_t v) { return v / 2; }
+
+// Pass units of an instance with v variables at step s < pass_steps(v): the
+// evaluated level has L = 2^(v - 2s) elements, L / 4 units.
+inline size_t pass_units(uint32_t v, uint32_t s) { return (size_t)1 << (v - 2 * s - 2); }
 
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_batch_round(const BInst* insts, const BChunk* chunks, uint32_t j,
-                                                        const uint64_t* cur, uint64_t* finals, uint64_t* partials) {
+__global__ __launch_bounds__(kBlock) void k_bpass(const BInst* insts, const BChunk* chunks, uint32_t s,
+                                                  const uint64_t* cur, uint64_t* partials) {
   const BChunk ch = chunks[blockIdx.x];
-  const BInst in = insts[ch.inst];
-  Fp acc[K];
+  const BInst& in = insts[ch.inst];
+  const size_t L = (size_t)1 << (in.v - 2 * s), q = L / 4;
+  const uint64_t c1 = s ? cur[2 * ch.inst] : 0, c2 = s ? cur[2 * ch.inst + 1] : 0;
+  // the stored level: the input for steps 0 and 1, else level 2s - 2
+  const uint64_t* src[2] = {s <= 1 ? in.in[0] : in.work[0], s <= 1 ? in.in[1] : in.work[1]};
+  PassAcc<K> acc;
 #pragma unroll
-  for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp>();
-  const size_t n = (size_t)1 << in.v;
-  if (j == in.v) {
-    // finalize (src/runner.rs:57-60) and final value
-    if (threadIdx.x == 0) {
-      uint64_t val;
-      if (in.v == 0) {
-        val = in.in[0][0];
-        if (K == 2) val = gl::mul(val, in.in[1][0]);
-      } else {
-        const uint64_t* src[2] = {in.v == 1 ? in.in[0] : in.work[0], in.v == 1 ? in.in[1] : in.work[1]};
-        uint64_t c = cur[ch.inst];
-        uint64_t f[2];
+  for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
+  const size_t end = (size_t)ch.begin + ch.count;
+  if (q >= 2) {
+    // pairs of units with 16-byte loads (q and the chunk bounds are even)
+    for (size_t u = ch.begin + 2 * threadIdx.x; u < end; u += 2 * blockDim.x) {
+      uint64_t E0[K][4], E1[K][4];
 #pragma unroll
-        for (int i = 0; i < K; i++) f[i] = gl::add(src[i][0], gl::mul(c, gl::sub(src[i][1], src[i][0])));
-        val = K == 2 ? gl::mul(f[0], f[1]) : f[0];
+      for (int i = 0; i < K; i++)
 #pragma unroll
-        for (int i = 0; i < K; i++) in.work[i][0] = f[i];
+        for (int j = 0; j < 4; j++) {
+          if (s == 0)
+            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+          else
+            fold_chain2<2, false>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+        }
+      if (s > 0) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            *reinterpret_cast<ulonglong2*>(in.work[i] + u + j * q) = make_ulonglong2(E0[i][j], E1[i][j]);
       }
-      finals[ch.inst] = val;
-    }
-  } else if (j == 0) {
-    const size_t half = n / 2;
-    for (size_t k = ch.begin + threadIdx.x; k < (size_t)ch.begin + ch.count; k += blockDim.x) {
-      Fp lo[K], hi[K];
-#pragma unroll
-      for (int i = 0; i < K; i++) {
-        lo[i] = {in.in[i][k]};
-        hi[i] = {in.in[i][k + half]};
-      }
-      eval_pair<Fp, K>(lo, hi, acc, 0);
+      pass_accumulate<K>(E0, acc);
+      pass_accumulate<K>(E1, acc);
     }
   } else {
-    const size_t q = n >> (j + 1);  // L / 4 with L = 2^(v - j + 1)
-    const Fp c = {cur[ch.inst]};
-    for (size_t k = ch.begin + threadIdx.x; k < (size_t)ch.begin + ch.count; k += blockDim.x) {
-      Fp lo[K], hi[K];
+    for (size_t u = ch.begin + threadIdx.x; u < end; u += blockDim.x) {
+      uint64_t E[K][4];
 #pragma unroll
-      for (int i = 0; i < K; i++) {
-        const uint64_t* s = j == 1 ? in.in[i] : in.work[i];
-        Fp x0 = {s[k]}, x1 = {s[k + q]}, x2 = {s[k + 2 * q]}, x3 = {s[k + 3 * q]};
-        lo[i] = fold1<Fp>(x0, x2, c, 0);
-        hi[i] = fold1<Fp>(x1, x3, c, 0);
-        in.work[i][k] = lo[i].v;
-        in.work[i][k + q] = hi[i].v;
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2) : fold_chain<2>(src[i], u + j * q, L, c1, c2);
+      if (s > 0) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) in.work[i][u + j * q] = E[i][j];
       }
-      eval_pair<Fp, K>(lo, hi, acc, 0);
+      pass_accumulate<K>(E, acc);
     }
   }
-  block_reduce<Fp, K>(acc);
-  if (threadIdx.x == 0)
+  Fp raw[pass_raw<K>()], vals[pass_values<K>()];
 #pragma unroll
-    for (int i = 0; i < K; i++) partials[(size_t)blockIdx.x * K + i] = acc[i].v;
+  for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
+  finish<Fp, pass_values<K>()>(vals, Reduce{partials, nullptr});
 }
 
-// One workgroup per active instance of round j: its chunks are
-// [chunk_begin[w], chunk_begin[w] + chunk_count[w]) of this round's table.
+// Final step of an instance (the whole workgroup calls it).  The stored level
+// (the input at step 0 or 1, else the work table) has M = 2^(v - 2s + 2)
+// elements (M = 2^v at s = 0) and, for s >= 1, two pending challenges.
+// Folding them leaves one element per table (v even) or two (v odd: that
+// pair is round v - 1, then the final fold with its challenge).
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_batch_fs(const BInst* insts, const uint32_t* act_inst,
-                                                     const uint32_t* chunk_begin, const uint32_t* chunk_count,
-                                                     uint32_t j, const uint64_t* partials, DevFs* fs,
-                                                     uint64_t* polys, uint64_t* chals, uint64_t* cur) {
-  const uint32_t inst = act_inst[blockIdx.x];
-  const uint32_t b0 = chunk_begin[blockIdx.x], nb = chunk_count[blockIdx.x];
-  Fp s[K];
+__device__ __forceinline__ void batch_final(const BInst& in, uint32_t inst, uint32_t s, const uint64_t* cur, DevFs* fs,
+                                            uint64_t* polys, uint64_t* chals, uint64_t* finals) {
+  __shared__ uint64_t chal_sh[3];
+  const uint32_t v = in.v;
+  if (v == 0) {
+    if (threadIdx.x == 0) finals[inst] = K == 2 ? gl::mul(in.in[0][0], in.in[1][0]) : in.in[0][0];
+    return;
+  }
+  const size_t M = s == 0 ? ((size_t)1 << v) : ((size_t)1 << (v - 2 * s + 2));
+  const uint64_t* src[2] = {s <= 1 ? in.in[0] : in.work[0], s <= 1 ? in.in[1] : in.work[1]};
+  const uint64_t x = s ? cur[2 * inst] : 0, y = s ? cur[2 * inst + 1] : 0;
+  const size_t n2 = s ? M / 4 : M;  // 1 or 2 elements per table after the pending folds
+  uint64_t t[2][2] = {{0, 0}, {0, 0}};
 #pragma unroll
-  for (int i = 0; i < K; i++) s[i] = gl::zero<Fp>();
-  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+  for (int i = 0; i < K; i++)
+    for (size_t e = 0; e < n2; e++) {
+      if (s == 0) {
+        t[i][e] = src[i][e];
+      } else {
+        // level M -> M/2 with x (pairs (k, k + M/2)), then M/2 -> M/4 with y
+        const size_t Q = M / 4;
+        const uint64_t a = gl::add(src[i][e], gl::mul(x, gl::sub(src[i][e + 2 * Q], src[i][e])));
+        const uint64_t b = gl::add(src[i][e + Q], gl::mul(x, gl::sub(src[i][e + 3 * Q], src[i][e + Q])));
+        t[i][e] = gl::add(a, gl::mul(y, gl::sub(b, a)));
+      }
+    }
+  uint64_t f[2] = {t[0][0], t[1][0]};
+  if (n2 == 2) {
+    // round v - 1 over the pair (t[.][0], t[.][1]): ML [h(0)], IP [h(0), h(inf)]
+    Fp vals[K];
+    if constexpr (K == 1) {
+      vals[0] = {t[0][0]};
+    } else {
+      vals[0] = {gl::mul(t[0][0], t[1][0])};
+      vals[1] = {gl::mul(gl::sub(t[0][1], t[0][0]), gl::sub(t[1][1], t[1][0]))};
+    }
+    const size_t r = in.first_round + v - 1;
+    if (threadIdx.x < 64) emit_round<Fp, K>(vals, 1, fs + inst, polys + r * K, chals + r, chal_sh);
+    __syncthreads();
+    const uint64_t z = chal_sh[0];
 #pragma unroll
-    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], Fp{partials[(size_t)(b0 + b) * K + i]});
-  block_reduce<Fp, K>(s);
+    for (int i = 0; i < K; i++) f[i] = gl::add(t[i][0], gl::mul(z, gl::sub(t[i][1], t[i][0])));
+  }
+  if (threadIdx.x == 0) finals[inst] = K == 2 ? gl::mul(f[0], f[1]) : f[0];
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_bfs(const BInst* insts, const BItem* items, uint32_t s,
+                                                const uint64_t* partials, DevFs* fs, uint64_t* polys, uint64_t* chals,
+                                                uint64_t* cur, uint64_t* finals) {
+  const BItem it = items[blockIdx.x];
+  const BInst& in = insts[it.inst];
+  if (it.kind == 1) {
+    batch_final<K>(in, it.inst, s, cur, fs, polys, chals, finals);
+    return;
+  }
+  constexpr int NV = pass_values<K>();
+  Fp sum[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) sum[i] = gl::zero<Fp>();
+  for (uint32_t b = threadIdx.x; b < it.nparts; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; i++) sum[i] = gl::add(sum[i], Fp{partials[(size_t)(it.part0 + b) * NV + i]});
+  block_reduce<Fp, NV>(sum);
   if (threadIdx.x < 64) {
-    const uint32_t r = insts[inst].first_round + j;
-    emit_round<Fp, K>(s, 1, fs + inst, polys + (size_t)r * K, chals + r, cur + inst);
+    const size_t r = in.first_round + 2 * s;
+    emit_two_rounds<K>(sum, fs + it.inst, polys + r * K, chals + r, cur + 2 * it.inst);
   }
 }
 
@@ -144,21 +202,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_fs(const BInst* insts, const u
 struct effsc_hip_batch {
   int dev = 0, K = 2;
   size_t count = 0, total_rounds = 0;
-  uint32_t max_v = 0;
+  uint32_t steps = 0;
   std::vector<uint32_t> v, first_round;
   hipStream_t stream = nullptr;
   BInst* d_insts = nullptr;
   uint64_t* d_work = nullptr;
   BChunk* d_chunks = nullptr;
-  uint32_t* d_act = nullptr;      // per round: active instances
-  uint32_t* d_cbegin = nullptr;   // per round: chunk range per active instance
-  uint32_t* d_ccount = nullptr;
-  std::vector<size_t> chunk_off, nchunks, act_off, nact;  // per round offsets
+  BItem* d_items = nullptr;
+  std::vector<size_t> chunk_off, nchunks, item_off, nitems;  // per step
   uint64_t* d_partials = nullptr;
   DevFs* d_fs = nullptr;
   uint64_t* d_polys = nullptr;
   uint64_t* d_chals = nullptr;
-  uint64_t* d_cur = nullptr;
+  uint64_t* d_cur = nullptr;  // two pending challenges per instance
   uint64_t* d_finals = nullptr;
   DevFs* h_fs = nullptr;  // pinned staging
   hipGraph_t graph = nullptr;
@@ -181,45 +237,43 @@ void bdestroy(effsc_hip_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (void* p : {(void*)b->d_insts, (void*)b->d_work, (void*)b->d_chunks, (void*)b->d_act, (void*)b->d_cbegin,
-                  (void*)b->d_ccount, (void*)b->d_partials, (void*)b->d_fs, (void*)b->d_polys, (void*)b->d_chals,
-                  (void*)b->d_cur, (void*)b->d_finals})
+  for (void* p : {(void*)b->d_insts, (void*)b->d_work, (void*)b->d_chunks, (void*)b->d_items, (void*)b->d_partials,
+                  (void*)b->d_fs, (void*)b->d_polys, (void*)b->d_chals, (void*)b->d_cur, (void*)b->d_finals})
     if (p) (void)hipFree(p);
   if (b->h_fs) (void)hipHostFree(b->h_fs);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
-// Enqueues every round of the batch on the stream.
-int enqueue_rounds(effsc_hip_batch* b) {
-  for (uint32_t j = 0; j <= b->max_v; j++) {
-    if (b->nchunks[j] == 0) continue;
-    const BChunk* ch = b->d_chunks + b->chunk_off[j];
-    uint64_t* part = b->d_partials;
-    if (b->K == 1)
-      hipLaunchKernelGGL(k_batch_round<1>, dim3((unsigned)b->nchunks[j]), dim3(kBlock), 0, b->stream, b->d_insts, ch,
-                         j, b->d_cur, b->d_finals, part);
-    else
-      hipLaunchKernelGGL(k_batch_round<2>, dim3((unsigned)b->nchunks[j]), dim3(kBlock), 0, b->stream, b->d_insts, ch,
-                         j, b->d_cur, b->d_finals, part);
-    HIPCHK(hipGetLastError());
-    if (b->nact[j] == 0) continue;
-    const uint32_t* act = b->d_act + b->act_off[j];
-    const uint32_t* cb = b->d_cbegin + b->act_off[j];
-    const uint32_t* cc = b->d_ccount + b->act_off[j];
-    if (b->K == 1)
-      hipLaunchKernelGGL(k_batch_fs<1>, dim3((unsigned)b->nact[j]), dim3(kBlock), 0, b->stream, b->d_insts, act, cb,
-                         cc, j, part, b->d_fs, b->d_polys, b->d_chals, b->d_cur);
-    else
-      hipLaunchKernelGGL(k_batch_fs<2>, dim3((unsigned)b->nact[j]), dim3(kBlock), 0, b->stream, b->d_insts, act, cb,
-                         cc, j, part, b->d_fs, b->d_polys, b->d_chals, b->d_cur);
-    HIPCHK(hipGetLastError());
+// Enqueues every step of the batch on the stream.
+int enqueue_steps(effsc_hip_batch* b) {
+  for (uint32_t s = 0; s < b->steps; s++) {
+    if (b->nchunks[s]) {
+      const BChunk* ch = b->d_chunks + b->chunk_off[s];
+      if (b->K == 1)
+        hipLaunchKernelGGL(k_bpass<1>, dim3((unsigned)b->nchunks[s]), dim3(kBlock), 0, b->stream, b->d_insts, ch, s,
+                           b->d_cur, b->d_partials);
+      else
+        hipLaunchKernelGGL(k_bpass<2>, dim3((unsigned)b->nchunks[s]), dim3(kBlock), 0, b->stream, b->d_insts, ch, s,
+                           b->d_cur, b->d_partials);
+      HIPCHK(hipGetLastError());
+    }
+    if (b->nitems[s]) {
+      const BItem* it = b->d_items + b->item_off[s];
+      if (b->K == 1)
+        hipLaunchKernelGGL(k_bfs<1>, dim3((unsigned)b->nitems[s]), dim3(kBlock), 0, b->stream, b->d_insts, it, s,
+                           b->d_partials, b->d_fs, b->d_polys, b->d_chals, b->d_cur, b->d_finals);
+      else
+        hipLaunchKernelGGL(k_bfs<2>, dim3((unsigned)b->nitems[s]), dim3(kBlock), 0, b->stream, b->d_insts, it, s,
+                           b->d_partials, b->d_fs, b->d_polys, b->d_chals, b->d_cur, b->d_finals);
+      HIPCHK(hipGetLastError());
+    }
   }
   return EFFSC_HIP_OK;
 }
 
-// (Re)builds the per-round chunk tables for b->chunk_units units per
-// workgroup and, when graphs are on, captures the round sequence.
+// (Re)builds the per-step chunk and item tables for b->chunk_units units per
+// workgroup and, when graphs are on, captures the step sequence.
 int build_plan(effsc_hip_batch* b) {
   const size_t count = b->count;
   if (b->stream) HIPCHK(hipStreamSynchronize(b->stream));  // no launch may still use the old tables
@@ -227,55 +281,52 @@ int build_plan(effsc_hip_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  for (void** p : {(void**)&b->d_chunks, (void**)&b->d_act, (void**)&b->d_cbegin, (void**)&b->d_ccount,
-                   (void**)&b->d_partials}) {
+  for (void** p : {(void**)&b->d_chunks, (void**)&b->d_items, (void**)&b->d_partials}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
   b->chunk_off.clear();
-  b->act_off.clear();
   b->nchunks.clear();
-  b->nact.clear();
+  b->item_off.clear();
+  b->nitems.clear();
+  // even chunk sizes: a pair of units never straddles two chunks
+  const size_t cu = std::max<size_t>(2, b->chunk_units & ~(size_t)1);
   std::vector<BChunk> chunks;
-  std::vector<uint32_t> act, cb, cc;
-  for (uint32_t j = 0; j <= b->max_v; j++) {
+  std::vector<BItem> items;
+  for (uint32_t s = 0; s < b->steps; s++) {
     b->chunk_off.push_back(chunks.size());
-    b->act_off.push_back(act.size());
+    b->item_off.push_back(items.size());
     for (size_t i = 0; i < count; i++) {
-      if (j > b->v[i]) continue;
-      size_t u = units(b->v[i], j);
-      // partials of round j are indexed by the chunk's position in that
-      // round's launch (blockIdx.x), i.e. relative to chunk_off[j]
-      uint32_t first = (uint32_t)(chunks.size() - b->chunk_off[j]);
-      for (size_t s = 0; s < u; s += b->chunk_units)
-        chunks.push_back({(uint32_t)i, (uint32_t)s, (uint32_t)std::min(b->chunk_units, u - s), 0});
-      if (j < b->v[i]) {
-        act.push_back((uint32_t)i);
-        cb.push_back(first);
-        cc.push_back((uint32_t)(chunks.size() - b->chunk_off[j] - first));
+      const uint32_t np = pass_steps(b->v[i]);
+      if (s < np) {
+        const size_t u = pass_units(b->v[i], s);
+        // partials are indexed by the chunk's position in this step's launch
+        const uint32_t first = (uint32_t)(chunks.size() - b->chunk_off[s]);
+        for (size_t x = 0; x < u; x += cu)
+          chunks.push_back({(uint32_t)i, (uint32_t)x, (uint32_t)std::min(cu, u - x), 0});
+        items.push_back({(uint32_t)i, 0u, first, (uint32_t)(chunks.size() - b->chunk_off[s] - first)});
+      } else if (s == np) {
+        items.push_back({(uint32_t)i, 1u, 0u, 0u});
       }
     }
     b->nchunks.push_back(chunks.size() - b->chunk_off.back());
-    b->nact.push_back(act.size() - b->act_off.back());
+    b->nitems.push_back(items.size() - b->item_off.back());
   }
   size_t max_chunks = 0;
   for (size_t x : b->nchunks) max_chunks = std::max(max_chunks, x);
   TRY(balloc((void**)&b->d_chunks, chunks.size() * sizeof(BChunk)));
-  TRY(balloc((void**)&b->d_act, act.size() * 4));
-  TRY(balloc((void**)&b->d_cbegin, act.size() * 4));
-  TRY(balloc((void**)&b->d_ccount, act.size() * 4));
-  TRY(balloc((void**)&b->d_partials, max_chunks * b->K * 8));
+  TRY(balloc((void**)&b->d_items, items.size() * sizeof(BItem)));
+  TRY(balloc((void**)&b->d_partials, max_chunks * pass_values<2>() * 8));
   hipError_t e = hipSuccess;
   if (!chunks.empty()) e = hipMemcpy(b->d_chunks, chunks.data(), chunks.size() * sizeof(BChunk), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_act, act.data(), act.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_cbegin, cb.data(), cb.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess && !act.empty()) e = hipMemcpy(b->d_ccount, cc.data(), cc.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice);
   if (e != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, hipGetErrorString(e));
   if (!b->use_graph) return EFFSC_HIP_OK;
-  // capture the round sequence once
+  // capture the step sequence once
   e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return set_error(EFFSC_HIP_EDEVICE, std::string("begin capture: ") + hipGetErrorString(e));
-  int rr = enqueue_rounds(b);
+  int rr = enqueue_steps(b);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->stream, &g);
   b->graph = g;
@@ -292,7 +343,7 @@ int run_once(effsc_hip_batch* b) {
   if (b->use_graph)
     HIPCHK(hipGraphLaunch(b->exec, b->stream));
   else
-    TRY(enqueue_rounds(b));
+    TRY(enqueue_steps(b));
   return EFFSC_HIP_OK;
 }
 
@@ -362,6 +413,9 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
   }
   int rc = EFFSC_HIP_OK;
   std::vector<BInst> hi(count);
+  // work tables hold level 2 (n/4 elements) and deeper levels in place; an
+  // even number of words each keeps every table 16-byte aligned
+  auto work_len = [](size_t n) { return (std::max<size_t>(n / 4, 1) + 1) & ~(size_t)1; };
   size_t work_words = 0;
   for (size_t i = 0; i < count && rc == EFFSC_HIP_OK; i++) {
     size_t n = lens[i];
@@ -371,8 +425,8 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
     b->v.push_back(v);
     b->first_round.push_back((uint32_t)b->total_rounds);
     b->total_rounds += v;
-    b->max_v = std::max(b->max_v, v);
-    work_words += kind * std::max<size_t>(n / 2, 1);
+    b->steps = std::max(b->steps, pass_steps(v) + 1);
+    work_words += kind * work_len(n);
   }
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_work, work_words * 8);
   size_t off = 0;
@@ -382,11 +436,13 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
       hi[i].in[t] = t < kind ? (t == 0 ? a[i] : b_tabs[i]) : nullptr;
       if (t < kind) {
         hi[i].work[t] = b->d_work + off;
-        off += std::max<size_t>(n / 2, 1);
+        off += work_len(n);
       } else {
         hi[i].work[t] = nullptr;
       }
       if (t < kind && !hi[i].in[t]) rc = set_error(EFFSC_HIP_EINVAL, "null table");
+      if (t < kind && hi[i].in[t] && reinterpret_cast<uintptr_t>(hi[i].in[t]) % 16)
+        rc = set_error(EFFSC_HIP_EINVAL, "batch tables must be 16-byte aligned");
     }
     hi[i].v = b->v[i];
     hi[i].first_round = b->first_round[i];
@@ -395,7 +451,7 @@ int effsc_hip_batch_new(int device, int kind, size_t count, const size_t* lens, 
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_fs, count * sizeof(DevFs));
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_polys, std::max<size_t>(b->total_rounds, 1) * kind * 8);
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_chals, std::max<size_t>(b->total_rounds, 1) * 8);
-  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_cur, count * 8);
+  if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_cur, count * 2 * 8);
   if (rc == EFFSC_HIP_OK) rc = balloc((void**)&b->d_finals, count * 8);
   if (rc == EFFSC_HIP_OK) {
     hipError_t e = hipHostMalloc((void**)&b->h_fs, count * sizeof(DevFs), hipHostMallocDefault);

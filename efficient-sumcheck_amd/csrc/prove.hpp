@@ -176,24 +176,15 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs(Reduce r, unsigned n
   if (threadIdx.x < 64) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
 }
 
-// Two-round pass (pass2.hpp) over base-field tables: sums the pass values,
-// then runs two transcript steps: round s values, challenge x, round s+1
-// values (the pass polynomials at x), challenge y.  cur receives (x, y), the
-// pending folds of the next pass.
+// Two transcript steps of a two-round pass, by the 64 lanes of one wave
+// whose lane 0 holds the reduced pass values: round s values, challenge x,
+// round s+1 values (the pass polynomials at x), challenge y.  poly receives
+// 2K words, chals (x, y), cur (x, y): the pending folds of the next pass.
 template <int K>
-__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
-                                                       uint64_t* chals, uint64_t* cur) {
+__device__ __forceinline__ void emit_two_rounds(const Fp* s, DevFs* t, uint64_t* poly, uint64_t* chals,
+                                                uint64_t* cur) {
   constexpr int NV = pass_values<K>();
-  Fp s[NV];
-#pragma unroll
-  for (int i = 0; i < NV; i++) s[i] = gl::zero<Fp>();
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) s[i] = gl::add(s[i], gl::load_words<Fp>(r.partials + (size_t)b * NV + i));
-  }
-  block_reduce<Fp, NV>(s);
-  if (threadIdx.x >= 64) return;
-  const bool lane0 = threadIdx.x == 0;
+  const bool lane0 = (threadIdx.x & 63) == 0;
   uint64_t v[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) v[i] = uniform64(s[i].v);
@@ -222,6 +213,24 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
     cur[1] = y[0];
   }
   fs.store(t);
+}
+
+// Two-round pass (pass2.hpp) over base-field tables: sums the pass values,
+// then runs the two transcript steps (emit_two_rounds).
+template <int K>
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
+                                                       uint64_t* chals, uint64_t* cur) {
+  constexpr int NV = pass_values<K>();
+  Fp s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) s[i] = gl::zero<Fp>();
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) s[i] = gl::add(s[i], gl::load_words<Fp>(r.partials + (size_t)b * NV + i));
+  }
+  block_reduce<Fp, NV>(s);
+  if (threadIdx.x >= 64) return;
+  emit_two_rounds<K>(s, t, poly, chals, cur);
 }
 
 constexpr int kTailBlock = 1024;
