@@ -276,6 +276,103 @@ class EqFactoredProver : public GpuProver<F> {
   }
 };
 
+// GkrProver (src/provers/gkr.rs:38-202): add and mult over (b, c) in
+// {0,1}^{2k} (n * n elements, index b n + c), w the n witness values.
+template <class F>
+class GkrProver : public GpuProver<F> {
+ public:
+  GkrProver(const std::vector<F>& add, const std::vector<F>& mult, const std::vector<F>& w, int device = 0) {
+    if (add.size() != w.size() * w.size()) throw Error(EFFSC_HIP_EINVAL, "add_evals must have len w^2");
+    if (mult.size() != w.size() * w.size()) throw Error(EFFSC_HIP_EINVAL, "mult_evals must have len w^2");
+    auto cfg = this->config(F::EXT, device);
+    auto wa = flatten(add), wm = flatten(mult), ww = flatten(w);
+    effsc_hip_prover* h = nullptr;
+    check(effsc_hip_gkr_new(&cfg, wa.data(), wm.data(), ww.data(), w.size(), &h));
+    this->adopt(h);
+  }
+  // claimed_w_values (src/provers/gkr.rs:89-97): (W(b*), W(c*))
+  std::pair<F, F> claimed_w_values() const {
+    uint64_t out[6];
+    check(effsc_hip_gkr_claimed_w(this->h_, out));
+    return {F::from_words(out), F::from_words(out + F::EXT)};
+  }
+};
+
+// LSB (pair-split) provers (src/provers/{multilinear,inner_product}_lsb.rs):
+// power-of-two tables, pairs (2k, 2k+1).
+template <class F>
+class MultilinearProverLSB : public GpuProver<F> {
+ public:
+  explicit MultilinearProverLSB(const std::vector<F>& evals, int device = 0) {
+    auto cfg = this->config(F::EXT, device);
+    auto w = flatten(evals);
+    effsc_hip_prover* h = nullptr;
+    check(effsc_hip_multilinear_lsb_new(&cfg, w.data(), evals.size(), &h));
+    this->adopt(h);
+  }
+  std::vector<F> evals() const { return this->table(0); }
+};
+
+template <class F>
+class InnerProductProverLSB : public GpuProver<F> {
+ public:
+  InnerProductProverLSB(const std::vector<F>& a, const std::vector<F>& b, int device = 0) {
+    if (a.size() != b.size()) throw Error(EFFSC_HIP_EINVAL, "a and b must have equal length");
+    auto cfg = this->config(F::EXT, device);
+    auto wa = flatten(a), wb = flatten(b);
+    effsc_hip_prover* h = nullptr;
+    check(effsc_hip_inner_product_lsb_new(&cfg, wa.data(), wb.data(), a.size(), &h));
+    this->adopt(h);
+  }
+};
+
+// FileStream (src/streams/file/core.rs:13-76) of base-field elements.
+class FileStream {
+ public:
+  explicit FileStream(const std::string& path, int ext = 1) { check(effsc_hip_stream_open(path.c_str(), ext, &s_)); }
+  FileStream(FileStream&& o) noexcept : s_(std::exchange(o.s_, nullptr)) {}
+  FileStream(const FileStream&) = delete;
+  ~FileStream() {
+    if (s_) effsc_hip_stream_close(s_);
+  }
+  size_t num_variables() const {
+    size_t v = 0;
+    check(effsc_hip_stream_num_variables(s_, &v));
+    return v;
+  }
+  F64 evaluation(size_t point) const {
+    uint64_t w = 0;
+    check(effsc_hip_stream_read(s_, point, 1, &w));
+    return F64{w};
+  }
+  effsc_hip_stream* raw() const { return s_; }
+
+ private:
+  effsc_hip_stream* s_ = nullptr;
+};
+
+// multivariate_claim (src/streams/stream.rs:3-14), on the device.
+inline F64 multivariate_claim(const FileStream& s, int device = 0) {
+  effsc_hip_stream* h[1] = {s.raw()};
+  uint64_t out[3] = {0, 0, 0};
+  check(effsc_hip_stream_claim(device, h, 1, EFFSC_HIP_W_FROM_RAW7, out));
+  return F64{out[0]};
+}
+
+// An InnerProductProver over two streams' tables: the upload overlaps the
+// first pass over the data.
+template <class F>
+class StreamInnerProductProver : public GpuProver<F> {
+ public:
+  StreamInnerProductProver(const FileStream& a, const FileStream& b, int device = 0) {
+    auto cfg = this->config(1, device);
+    effsc_hip_stream* hs[2] = {a.raw(), b.raw()};
+    effsc_hip_prover* h = nullptr;
+    check(effsc_hip_stream_prover_new(&cfg, 2, hs, &h));
+    this->adopt(h);
+  }
+};
+
 // ------------------------------------------------------------------- hooks
 struct NoopHook {
   template <class T>

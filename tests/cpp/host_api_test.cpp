@@ -134,6 +134,78 @@ static int gpu_tests() {
   auto proof4 = sumcheck(p4, v, t4);
   EXPECT(proof4.round_polys.size() == v && proof4.round_polys[0].size() == 3);
   print_proof("ip", proof2);
+
+  // GKR layer with k = 4 (src/provers/gkr.rs:208-266): claim chain through the
+  // verifier, claimed W values = the witness MLE at the two challenge halves
+  {
+    const size_t k = 4, m = size_t(1) << k;
+    const unsigned __int128 P = 0xFFFFFFFF00000001ull;
+    std::vector<F64> add(m * m), mult(m * m), w(m);
+    for (size_t i = 0; i < m * m; i++) {
+      add[i] = F64(uniform(0xC0FFEE20, i));
+      mult[i] = F64(uniform(0xC0FFEE21, i));
+    }
+    for (size_t i = 0; i < m; i++) w[i] = F64(uniform(0xC0FFEE22, i));
+    unsigned __int128 gc = 0;
+    for (size_t bb = 0; bb < m; bb++)
+      for (size_t cc = 0; cc < m; cc++) {
+        const unsigned __int128 wb = w[bb].v, wc = w[cc].v;
+        gc = (gc + (unsigned __int128)add[bb * m + cc].v * ((wb + wc) % P) % P +
+              (unsigned __int128)mult[bb * m + cc].v * (wb * wc % P) % P) % P;
+      }
+    GkrProver<F64> g(add, mult, w);
+    auto tg = Transcript<F64>::fiat_shamir("effsc-hip/cpp-gkr");
+    auto pg = sumcheck(g, 2 * k, tg);
+    auto tgv = Transcript<F64>::fiat_shamir("effsc-hip/cpp-gkr");
+    tgv.attach(pg);
+    auto rg = sumcheck_verify(F64((uint64_t)gc), 2, 2 * k, tgv);
+    EXPECT(rg.final_claim == pg.final_value);
+    auto mle = [&](size_t lo) {
+      std::vector<unsigned __int128> t(m);
+      for (size_t i = 0; i < m; i++) t[i] = w[i].v;
+      for (size_t r = 0; r < k; r++) {
+        const unsigned __int128 c = pg.challenges[lo + r].v;
+        const size_t h = t.size() / 2;
+        for (size_t j = 0; j < h; j++) t[j] = (t[j] + c * ((t[j + h] + P - t[j]) % P)) % P;
+        t.resize(h);
+      }
+      return (uint64_t)t[0];
+    };
+    auto cw = g.claimed_w_values();
+    EXPECT(cw.first.v == mle(0));
+    EXPECT(cw.second.v == mle(k));
+  }
+  // LSB inner product over the same tables, printed for the oracle
+  InnerProductProverLSB<F64> pl(a, b);
+  auto tl = Transcript<F64>::fiat_shamir("effsc-hip/cpp");
+  auto proofl = sumcheck(pl, v, tl);
+  print_proof("ip_lsb", proofl);
+  // the same inner product from FileStreams equals the in-memory proof
+  {
+    const std::string fa = "/tmp/effsc_cpp_stream_a.bin", fb = "/tmp/effsc_cpp_stream_b.bin";
+    for (auto pr : {std::make_pair(fa, &a), std::make_pair(fb, &b)}) {
+      FILE* f = std::fopen(pr.first.c_str(), "wb");
+      EXPECT(f != nullptr);
+      if (!f) break;
+      for (auto& x : *pr.second) std::fwrite(&x.v, 8, 1, f);
+      std::fclose(f);
+    }
+    {
+      FileStream sa(fa), sb(fb);
+      EXPECT(sa.num_variables() == v);
+      EXPECT(sa.evaluation(5) == a[5]);
+      unsigned __int128 sum = 0;
+      for (auto& x : a) sum = (sum + x.v) % 0xFFFFFFFF00000001ull;
+      EXPECT(multivariate_claim(sa).v == (uint64_t)sum);
+      StreamInnerProductProver<F64> ps(sa, sb);
+      auto ts = Transcript<F64>::fiat_shamir("effsc-hip/cpp");
+      auto proofs = sumcheck(ps, v, ts);
+      EXPECT(proofs.round_polys == proof2.round_polys);
+      EXPECT(proofs.final_value == proof2.final_value);
+    }
+    std::remove(fa.c_str());
+    std::remove(fb.c_str());
+  }
   return failures;
 }
 

@@ -38,3 +38,8 @@ def test_cpp_gpu_provers_match_oracle():
     assert out["ip"]["round_polys"] == [[x[0] for x in r] for r in rp]
     assert out["ip"]["challenges"] == [c[0] for c in ch]
     assert out["ip"]["final"] == fin[0]
+
+    # the LSB inner product printed by the program, against the oracle's LSB restatement
+    rl, cl, fl = O.Prover.inner_product_lsb(a, b).sumcheck(v, O.Transcript.shake(b"effsc-hip/cpp"))
+    assert out["ip_lsb"]["round_polys"] == [[x[0] for x in r] for r in rl]
+    assert out["ip_lsb"]["final"] == fl[0]
