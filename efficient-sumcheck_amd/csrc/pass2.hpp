@@ -176,7 +176,7 @@ __device__ __forceinline__ void fold_chain2(const uint64_t* src, size_t e, size_
 // One pass: P pending folds from the level-M tables `src` to level L = 4 q,
 // written to `dst` when P > 0; accumulates the pass values.  Challenges come
 // from `cptr` (device words c1, c2) when non-null, else from c1, c2.
-template <int K, int P, int VEC>
+template <int K, int P, int VEC, int NTM = 0>
 __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> dst, uint64_t c1, uint64_t c2,
                                                  const uint64_t* cptr, size_t q, Reduce r) {
   if (cptr) {
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
       for (int j = 0; j < 4; j++) {
         if constexpr (VEC == 2) {
           // a pass without folds reads each element exactly once: stream it
-          fold_chain2<P, P == 0>(src.t[i].p, u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+          fold_chain2<P, P == 0 || (NTM & 1)>(src.t[i].p, u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
         } else {
           E0[i][j] = fold_chain<P>(src.t[i].p, u + j * q, L, c1, c2);
         }
@@ -207,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
       for (int i = 0; i < K; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          if constexpr (VEC == 2)
+          if constexpr (VEC == 2 && (NTM & 2))
+            __builtin_nontemporal_store(v2u64{E0[i][j], E1[i][j]}, reinterpret_cast<v2u64*>(dst.t[i].p + u + j * q));
+          else if constexpr (VEC == 2)
             *reinterpret_cast<ulonglong2*>(dst.t[i].p + u + j * q) = make_ulonglong2(E0[i][j], E1[i][j]);
           else
             dst.t[i].p[u + j * q] = E0[i][j];
