@@ -367,6 +367,27 @@ struct DevOps {
                   bool have, size_t first_round, const uint64_t* cur) {
     Views<FI, K> in = views<FI, K>(src, sst);
     Views<F, K> work = views<F, K>(h->work, h->work_stride);
+    if constexpr (std::is_same<FI, Fp>::value && std::is_same<F, Fp>::value && K <= 2) {
+      // base-field tables of 2^11 .. 2^14 elements (after the pending fold):
+      // the register-resident tail
+      const size_t L1 = have ? L / 2 : L;
+      if (rounds > 0 && L1 >= 2048 && L1 <= 16384) {
+        auto go = [&](auto m0) {
+          constexpr int M0 = decltype(m0)::value;
+          hipLaunchKernelGGL((k_tail_reg<K, M0>), dim3(1), dim3(kTailRegThreads), 0, h->stream, in, work,
+                             from_input ? 1 : 0, rounds, have ? 1 : 0, h->d_fs, h->d_polys + first_round * K,
+                             h->d_chals + first_round, cur, h->d_fin);
+        };
+        switch (L1 / 1024) {
+          case 2: go(std::integral_constant<int, 2>{}); break;
+          case 4: go(std::integral_constant<int, 4>{}); break;
+          case 8: go(std::integral_constant<int, 8>{}); break;
+          default: go(std::integral_constant<int, 16>{}); break;
+        }
+        HIPCHK(hipGetLastError());
+        return EFFSC_HIP_OK;
+      }
+    }
     hipLaunchKernelGGL((k_tail<FI, F, K>), dim3(1), dim3(tail_block<F, K>()), 0, h->stream, in, work, from_input ? 1 : 0, L,
                        rounds, have ? 1 : 0, h->d_fs, h->ext, h->d_polys + first_round * K * h->ext,
                        h->d_chals + first_round * h->ext, const_cast<uint64_t*>(cur), h->d_fin, h->w);

@@ -249,16 +249,38 @@ __device__ __forceinline__ void finish(F* acc, Reduce r) {
   }
 }
 
+// Adds this thread's share of `nblocks` partials of K values into s.  Four
+// blocks per step with every load issued before the adds: the partials come
+// from HBM, and one load-add chain per block left the single reducing
+// workgroup latency-bound (8192 blocks of 8 values took about 8 us).
+template <class F, int K>
+__device__ __forceinline__ void sum_partials(const uint64_t* partials, unsigned nblocks, F* s) {
+  const unsigned st = blockDim.x;
+  unsigned b = threadIdx.x;
+  for (; b + 3 * st < nblocks; b += 4 * st) {
+    F v[4][K];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int i = 0; i < K; i++) v[u][i] = gl::load_words<F>(partials + ((size_t)(b + u * st) * K + i) * F::D);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int i = 0; i < K; i++) s[i] = gl::add(s[i], v[u][i]);
+  }
+  for (; b < nblocks; b += st) {
+#pragma unroll
+    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(partials + ((size_t)b * K + i) * F::D));
+  }
+}
+
 // Sums `nblocks` partials of K values each into r.result (one workgroup).
 template <class F, int K>
 __global__ __launch_bounds__(1024) void k_reduce(Reduce r, unsigned nblocks) {
   F s[K];
 #pragma unroll
   for (int i = 0; i < K; i++) s[i] = gl::zero<F>();
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * K + i) * F::D));
-  }
+  sum_partials<F, K>(r.partials, nblocks, s);
   block_reduce<F, K>(s);
   if (threadIdx.x == 0) {
 #pragma unroll

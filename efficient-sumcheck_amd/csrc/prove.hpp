@@ -214,10 +214,7 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs(Reduce r, unsigned n
   F s[K];
 #pragma unroll
   for (int i = 0; i < K; i++) s[i] = gl::zero<F>();
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < K; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * K + i) * F::D));
-  }
+  sum_partials<F, K>(r.partials, nblocks, s);
   block_reduce<F, K>(s);
   if (threadIdx.x < 64) emit_round<F, K>(s, ext, t, poly, chal_out, chal_cur);
 }
@@ -270,10 +267,7 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
   Fp s[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) s[i] = gl::zero<Fp>();
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) s[i] = gl::add(s[i], gl::load_words<Fp>(r.partials + (size_t)b * NV + i));
-  }
+  sum_partials<Fp, NV>(r.partials, nblocks, s);
   block_reduce<Fp, NV>(s);
   if (threadIdx.x >= 64) return;
   emit_two_rounds<K>(s, t, poly, chals, cur);
@@ -386,6 +380,209 @@ __global__ __launch_bounds__((tail_block<F, K>())) void k_tail(Views<FI, K> in, 
     gl::store_words(v, wd);
     for (int c = 0; c < ext; c++) fin[c] = wd[c];
   }
+}
+
+// Transcript step of one round with the sponge held in registers by the 64
+// lanes of one wave (lane 0 holds the K base-field values): writes poly
+// (K words) and chal_out, returns the challenge (wave-uniform).
+template <int K>
+__device__ __forceinline__ uint64_t emit_round_regs(LaneFs& fs, const Fp* s, uint64_t* poly, uint64_t* chal_out) {
+  const bool lane0 = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t w[3] = {uniform64(s[i].v), 0, 0};
+    if (lane0) poly[i] = w[0];
+    fs.send(1, w);
+  }
+  uint64_t ch[3];
+  fs.challenge(1, ch);
+  if (lane0) chal_out[0] = ch[0];
+  return ch[0];
+}
+
+// Sum of K values over one wave, valid in every lane.
+template <int K>
+__device__ __forceinline__ void wave_sum(Fp* acc) {
+#pragma unroll
+  for (int i = 0; i < K; i++)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc[i] = gl::add(acc[i], shfl_xor(acc[i], m));
+}
+
+constexpr int kTailRegThreads = 1024;
+
+// k_tail for base-field tables with K <= 2 and 2^11 <= L1 <= 2^14 elements
+// after the pending fold (L1 = M0 * 1024), with the tables in registers.
+//
+// Thread t holds elements t + 1024 m (m < Mc) of every table: while Mc >= 2
+// the pairs (k, k + L/2) of a round are (m, m + Mc/2) of one thread, so a
+// round is an in-register evaluation, a block reduction, the transcript step
+// by wave 0 (sponge kept in its registers for the whole tail) and an
+// in-register fold.  At 1024 elements the table moves through LDS to wave 0,
+// lane l holding l + 64 m, and the remaining rounds run in that wave alone:
+// in-lane pairs down to 64 elements, then pairs (l, l + h) by shuffles; no
+// workgroup barrier after the hand-over.  The results are those of k_tail:
+// same round polynomials, challenges, final tables (written back to `work`
+// when the rounds stop early) and final value.
+template <int K, int M0>
+__global__ __launch_bounds__(kTailRegThreads) void k_tail_reg(Views<Fp, K> in, Views<Fp, K> work, int from_input,
+                                                             int rounds, int have_chal, DevFs* t, uint64_t* polys,
+                                                             uint64_t* chals, const uint64_t* chal_cur, uint64_t* fin) {
+  static_assert(M0 >= 2 && M0 <= 16 && (M0 & (M0 - 1)) == 0, "M0: 2, 4, 8 or 16");
+  constexpr int LG0 = ilog2c(M0);
+  // loops below count log2 sizes so that they unroll and index registers
+  __shared__ uint64_t tab[K][kTailRegThreads];
+  __shared__ uint64_t chal_sh;
+  const unsigned tid = threadIdx.x;
+  const int lane = tid & 63;
+  const uint64_t w = 0;
+  const Views<Fp, K>& src = from_input ? in : work;
+  Fp x[K][M0];
+  if (have_chal) {
+    // the pending fold: length 2 L1 -> L1, pairs (m, m + M0) of this thread
+    const Fp c{chal_cur[0]};
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int m = 0; m < M0; m++)
+        x[i][m] = fold1<Fp>(src.t[i].ld(tid + 1024 * m), src.t[i].ld(tid + 1024 * (m + M0)), c, w);
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int m = 0; m < M0; m++) x[i][m] = src.t[i].ld(tid + 1024 * m);
+  }
+  LaneFs fs;
+  if (tid < 64) fs.load(t);
+  int r = 0;
+  int Lc = 1024 * M0;  // current table length
+  // ---- all waves: rounds with in-thread pairs
+#pragma unroll
+  for (int lg = LG0; lg >= 1; lg--) {
+    const int Mc = 1 << lg;
+    if (r < rounds) {
+      Fp acc[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp>();
+#pragma unroll
+      for (int m = 0; m < Mc / 2; m++) {
+        Fp lo[K], hi[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          lo[i] = x[i][m];
+          hi[i] = x[i][m + Mc / 2];
+        }
+        eval_pair<Fp, K>(lo, hi, acc, w);
+      }
+      block_reduce<Fp, K>(acc);
+      if (tid < 64) {
+        const uint64_t c = emit_round_regs<K>(fs, acc, polys + (size_t)r * K, chals + r);
+        if (tid == 0) chal_sh = c;
+      }
+      __syncthreads();
+      const Fp c{chal_sh};
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int m = 0; m < Mc / 2; m++) x[i][m] = fold1<Fp>(x[i][m], x[i][m + Mc / 2], c, w);
+      Lc /= 2;
+      r++;
+    }
+  }
+  if (r < rounds) {
+    // ---- hand-over: 1024 elements to wave 0 through LDS; lane l owns
+    // positions l + 64 m of `tab` (no other lane touches them, so the wave
+    // needs no barrier)
+#pragma unroll
+    for (int i = 0; i < K; i++) tab[i][tid] = x[i][0].v;
+    __syncthreads();
+    if (tid >= 64) return;
+#pragma unroll
+    for (int lg = 4; lg >= 1; lg--) {
+      const int Mw = 1 << lg;
+      if (r < rounds) {
+        Fp acc[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp>();
+#pragma unroll
+        for (int m = 0; m < Mw / 2; m++) {
+          Fp lo[K], hi[K];
+#pragma unroll
+          for (int i = 0; i < K; i++) {
+            lo[i] = Fp{tab[i][lane + 64 * m]};
+            hi[i] = Fp{tab[i][lane + 64 * (m + Mw / 2)]};
+          }
+          eval_pair<Fp, K>(lo, hi, acc, w);
+        }
+        wave_sum<K>(acc);
+        const Fp c{emit_round_regs<K>(fs, acc, polys + (size_t)r * K, chals + r)};
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int m = 0; m < Mw / 2; m++)
+            tab[i][lane + 64 * m] = fold1<Fp>(Fp{tab[i][lane + 64 * m]}, Fp{tab[i][lane + 64 * (m + Mw / 2)]}, c, w).v;
+        Lc /= 2;
+        r++;
+      }
+    }
+    // 64 elements, one per lane: pairs (l, l + h)
+    Fp z[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) z[i] = Fp{tab[i][lane]};
+#pragma unroll
+    for (int lg = 5; lg >= 0; lg--) {
+      const int h = 1 << lg;
+      if (r < rounds) {
+        Fp lo[K], hi[K], acc[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          lo[i] = z[i];
+          hi[i] = shfl_xor(z[i], h);  // lanes l < h read l + h
+          acc[i] = gl::zero<Fp>();
+        }
+        if (lane < h) eval_pair<Fp, K>(lo, hi, acc, w);
+        wave_sum<K>(acc);
+        const Fp c{emit_round_regs<K>(fs, acc, polys + (size_t)r * K, chals + r)};
+#pragma unroll
+        for (int i = 0; i < K; i++) z[i] = fold1<Fp>(lo[i], hi[i], c, w);
+        Lc /= 2;
+        r++;
+      }
+    }
+    // write back the tables (length Lc; lanes >= Lc hold no element)
+    if (Lc >= 64) {
+      const int Mw = Lc / 64;
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int m = 0; m < 16; m++)
+          if (m < Mw) work.t[i].st(lane + 64 * m, Fp{tab[i][lane + 64 * m]});
+    } else {
+#pragma unroll
+      for (int i = 0; i < K; i++)
+        if (lane < Lc) work.t[i].st(lane, z[i]);
+    }
+    if (lane == 0) {
+      Fp v = gl::zero<Fp>();
+      if (Lc == 1) {
+        v = z[0];
+#pragma unroll
+        for (int i = 1; i < K; i++) v = gl::mul(v, z[i], w);
+      }
+      fin[0] = v.v;
+    }
+    fs.store(t);
+    return;
+  }
+  // rounds ended with the tables still spread over the workgroup
+  const int Mc = Lc / 1024;
+#pragma unroll
+  for (int i = 0; i < K; i++)
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+      if (m < Mc) work.t[i].st(tid + 1024 * m, x[i][m]);
+  if (tid == 0) fin[0] = 0;  // more than one element left
+  if (tid < 64) fs.store(t);
 }
 
 }  // namespace effsc_hip

@@ -371,6 +371,20 @@ def test_device_loop_matches_oracle(kind, ext, input_ext, v, tail, monkeypatch):
             assert tuple(as_oracle(dev, ext)) == tuple(po)
 
 
+@pytest.mark.parametrize("tail", [None, 1 << 12, 1 << 13])
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("v", [11, 12, 13, 14, 15, 16, 17])
+def test_register_tail_parity(kind, v, tail, monkeypatch):
+    # base-field tails of 2^11..2^14 elements run in registers: stop in every
+    # phase (all waves, one wave with in-lane pairs, one wave with shuffles)
+    if tail is not None:
+        monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
+    n = 1 << v
+    for rounds in sorted({1, 2, 3, max(v - 11, 1), v - 10, v - 9, v - 6, v - 5, v - 3, v - 1, v}):
+        for tr in ("shake", "rng"):
+            check_proof(kind, 1, 1, n, v, 4000 + v, transcript=tr, rounds=rounds)
+
+
 @pytest.mark.parametrize("tail", [2, 1 << 14])
 def test_device_loop_partial_then_continue(tail, monkeypatch):
     monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
