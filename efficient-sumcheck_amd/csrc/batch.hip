@@ -188,16 +188,17 @@ __global__ __launch_bounds__(kBlock) void k_bfs(const BInst* insts, const BItem*
     return;
   }
   constexpr int NV = pass_values<K>();
+  // the sponge load overlaps the partial sums
+  LaneFs lf;
+  if (threadIdx.x < 64) lf.load(fs + it.inst);
   Fp sum[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) sum[i] = gl::zero<Fp>();
-  for (uint32_t b = threadIdx.x; b < it.nparts; b += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < NV; i++) sum[i] = gl::add(sum[i], Fp{partials[(size_t)(it.part0 + b) * NV + i]});
+  sum_partials<Fp, NV>(partials + (size_t)it.part0 * NV, it.nparts, sum);
   block_reduce<Fp, NV>(sum);
   if (threadIdx.x < 64) {
     const size_t r = in.first_round + 2 * s;
-    emit_two_rounds<K>(sum, fs + it.inst, polys + r * K, chals + r, cur + 2 * it.inst);
+    emit_two_rounds<K>(sum, lf, fs + it.inst, polys + r * K, chals + r, cur + 2 * it.inst);
   }
 }
 

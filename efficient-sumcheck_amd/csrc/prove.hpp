@@ -40,7 +40,7 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
 
 // Cross-lane moves on the VALU.  DPP row shifts (row_shl:n: lane l reads
 // l + n, row_shr:n: l - n, row_ror:n: rotate, all within 16-lane rows) and
-// the gfx950 permlane swaps, used as "value of lane l ^ 32" and "of l ^ 16".
+// the gfx950 permlane swaps, used for sums over lanes l, l ^ 32 and l, l ^ 16.
 // A DPP read from a lane that is off in EXEC returns zero, so these are only
 // used where the whole wave is active.
 template <int CTRL>
@@ -49,19 +49,20 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, true);
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint64_t lane_xor32(uint64_t t, bool low_half) {
-  // permlane32_swap(a, a): element 0 holds a's rows 0,1 in rows 2,3, element
-  // 1 holds a's rows 2,3 in rows 0,1
+// t[l] ^ t[l ^ 32] in every lane.  permlane32_swap(a, b) leaves a with b's
+// rows 0,1 in its rows 2,3 and b with a's rows 2,3 in its rows 0,1; for
+// a = b = t the xor of the two results is the sum in every row.
+__device__ __forceinline__ uint64_t xor_lane32(uint64_t t) {
   auto l = __builtin_amdgcn_permlane32_swap((uint32_t)t, (uint32_t)t, false, false);
   auto h = __builtin_amdgcn_permlane32_swap((uint32_t)(t >> 32), (uint32_t)(t >> 32), false, false);
-  return low_half ? ((uint64_t)h[1] << 32) | l[1] : ((uint64_t)h[0] << 32) | l[0];
+  return ((uint64_t)(h[0] ^ h[1]) << 32) | (l[0] ^ l[1]);
 }
-__device__ __forceinline__ uint64_t lane_xor16(uint64_t t, bool even_row) {
-  // permlane16_swap(a, a): element 0 holds even rows in the odd ones,
-  // element 1 odd rows in the even ones
+// t[l] ^ t[l ^ 16]: permlane16_swap exchanges odd rows of a with even rows
+// of b.
+__device__ __forceinline__ uint64_t xor_lane16(uint64_t t) {
   auto l = __builtin_amdgcn_permlane16_swap((uint32_t)t, (uint32_t)t, false, false);
   auto h = __builtin_amdgcn_permlane16_swap((uint32_t)(t >> 32), (uint32_t)(t >> 32), false, false);
-  return even_row ? ((uint64_t)h[1] << 32) | l[1] : ((uint64_t)h[0] << 32) | l[0];
+  return ((uint64_t)(h[0] ^ h[1]) << 32) | (l[0] ^ l[1]);
 }
 __device__ __forceinline__ uint64_t rotl64v(uint64_t x, uint32_t r) { return (x << r) | (x >> ((64 - r) & 63)); }
 
@@ -84,7 +85,6 @@ __device__ __forceinline__ uint64_t keccak_lanes(uint64_t a_in) {
   const int g = lane >> 3, j = lane & 7;
   const bool active = g < 5;
   const int x = j < 5 ? j : j - 5, y = active ? g : 0;
-  const bool low = lane < 32, even = ((lane >> 4) & 1) == 0;
   const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
   // pi: destination (x, y) reads source (3 (y - 3x) mod 5, x)
   const int sx = (3 * (y - 3 * x) % 5 + 25) % 5, sy = x;
@@ -95,9 +95,7 @@ __device__ __forceinline__ uint64_t keccak_lanes(uint64_t a_in) {
   const uint64_t m_copy = j >= 5 ? ~0ull : 0ull, m_act = active ? ~0ull : 0ull;
   uint64_t a = shfl64(a_in, x + 5 * y) & m_act;
   for (int rnd = 0; rnd < 24; rnd++) {
-    uint64_t t = a ^ dpp64<0x128>(a);  // row_ror:8
-    t ^= lane_xor32(t, low);
-    const uint64_t c = t ^ lane_xor16(t, even);
+    const uint64_t c = xor_lane16(xor_lane32(a ^ dpp64<0x128>(a)));  // row_ror:8, rows r ^ 2, rows r ^ 1
     const uint64_t cp = dpp64<0x101>(c);                           // row_shl:1
     const uint64_t c4 = dpp64<0x104>(c), c1 = dpp64<0x111>(c);     // row_shl:4, row_shr:1
     a ^= ((c4 & m_x0) | (c1 & ~m_x0)) ^ rotl64(cp, 1);
@@ -223,16 +221,15 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs(Reduce r, unsigned n
 // whose lane 0 holds the reduced pass values: round s values, challenge x,
 // round s+1 values (the pass polynomials at x), challenge y.  poly receives
 // 2K words, chals (x, y), cur (x, y): the pending folds of the next pass.
+// `fs` holds the sponge loaded from t (callers issue that load early).
 template <int K>
-__device__ __forceinline__ void emit_two_rounds(const Fp* s, DevFs* t, uint64_t* poly, uint64_t* chals,
+__device__ __forceinline__ void emit_two_rounds(const Fp* s, LaneFs& fs, DevFs* t, uint64_t* poly, uint64_t* chals,
                                                 uint64_t* cur) {
   constexpr int NV = pass_values<K>();
   const bool lane0 = (threadIdx.x & 63) == 0;
   uint64_t v[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) v[i] = uniform64(s[i].v);
-  LaneFs fs;
-  fs.load(t);
 #pragma unroll
   for (int i = 0; i < K; i++) {
     uint64_t w[3] = {v[i], 0, 0};
@@ -264,13 +261,16 @@ template <int K>
 __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
                                                        uint64_t* chals, uint64_t* cur) {
   constexpr int NV = pass_values<K>();
+  // the sponge load overlaps the partial sums
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
   Fp s[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) s[i] = gl::zero<Fp>();
   sum_partials<Fp, NV>(r.partials, nblocks, s);
   block_reduce<Fp, NV>(s);
   if (threadIdx.x >= 64) return;
-  emit_two_rounds<K>(s, t, poly, chals, cur);
+  emit_two_rounds<K>(s, fs, t, poly, chals, cur);
 }
 
 constexpr int kTailBlock = 1024;
