@@ -567,12 +567,22 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   // reused before it would be evicted; measured 2.75 against 2.85 ms for the
   // v = 28 inner product, no gain below that size
   const bool nt = vec && P > 0 && 8.0 * (double)(L << P) >= kNtMinBytes;
+  // the read pass (P = 0) gives each workgroup one contiguous chunk of units;
+  // the fold passes keep the grid-stride order.  Measured for v = 28: the
+  // chunked read pass took the proof from 2.52-2.62 to 2.38 ms, chunked fold
+  // passes were slower (2.66 ms)
+  size_t chunk = 0;
+  if (vec && P == 0) {
+    const size_t per_step = (size_t)kBlock * 2;
+    chunk = std::max<size_t>(1, (q + per_step * g - 1) / (per_step * g)) * per_step;
+    g = (unsigned)((q + chunk - 1) / chunk);
+  }
   if (nt)
     hipLaunchKernelGGL((k_pass<K, P, 2, 3>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1],
-                       cptr, q, h->red);
+                       cptr, q, h->red, chunk);
   else if (vec)
     hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
-                       q, h->red);
+                       q, h->red, chunk);
   else
     hipLaunchKernelGGL((k_pass<K, P, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
                        q, h->red);
