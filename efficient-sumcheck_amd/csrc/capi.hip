@@ -572,17 +572,24 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   // chunked read pass took the proof from 2.52-2.62 to 2.38 ms, chunked fold
   // passes were slower (2.66 ms)
   size_t chunk = 0;
-  if (vec && P == 0) {
-    const size_t per_step = (size_t)kBlock * 2;
-    chunk = std::max<size_t>(1, (q + per_step * g - 1) / (per_step * g)) * per_step;
-    g = (unsigned)((q + chunk - 1) / chunk);
+  unsigned gpb = 1;
+  // groups of 8..2048 workgroups per chunk for the read pass, or 2..128 for
+  // the fold passes, measured within noise of these two choices
+  const unsigned G = P == 0 ? g : 1;
+  if (vec && G > 1) {
+    // G groups of gpb workgroups; each group strides over its own chunk
+    const unsigned groups = std::min<unsigned>((unsigned)G, g);
+    gpb = std::max<unsigned>(1, g / groups);
+    const size_t per_step = (size_t)kBlock * 2 * gpb;
+    chunk = std::max<size_t>(1, (q + per_step * groups - 1) / (per_step * groups)) * per_step;
+    g = (unsigned)((q + chunk - 1) / chunk) * gpb;
   }
   if (nt)
     hipLaunchKernelGGL((k_pass<K, P, 2, 3>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1],
-                       cptr, q, h->red, chunk);
+                       cptr, q, h->red, chunk, gpb);
   else if (vec)
     hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
-                       q, h->red, chunk);
+                       q, h->red, chunk, gpb);
   else
     hipLaunchKernelGGL((k_pass<K, P, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
                        q, h->red);

@@ -178,7 +178,8 @@ __device__ __forceinline__ void fold_chain2(const uint64_t* src, size_t e, size_
 // from `cptr` (device words c1, c2) when non-null, else from c1, c2.
 template <int K, int P, int VEC, int NTM = 0>
 __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> dst, uint64_t c1, uint64_t c2,
-                                                 const uint64_t* cptr, size_t q, Reduce r, size_t chunk = 0) {
+                                                 const uint64_t* cptr, size_t q, Reduce r, size_t chunk = 0,
+                                                 unsigned gpb = 1) {
   if (cptr) {
     c1 = cptr[0];
     c2 = cptr[1];
@@ -187,12 +188,14 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
   PassAcc<K> acc;
 #pragma unroll
   for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
-  // grid-stride (chunk == 0) or one contiguous chunk of units per workgroup
+  // grid-stride (chunk == 0), or groups of gpb workgroups each striding over
+  // one contiguous chunk of units
   size_t u0, uend, step;
   if (chunk) {
-    u0 = (size_t)blockIdx.x * chunk + threadIdx.x * VEC;
-    uend = min(q, (size_t)(blockIdx.x + 1) * chunk);
-    step = (size_t)blockDim.x * VEC;
+    const size_t grp = blockIdx.x / gpb, k = blockIdx.x % gpb;
+    u0 = grp * chunk + (k * blockDim.x + threadIdx.x) * VEC;
+    uend = min(q, (grp + 1) * chunk);
+    step = (size_t)gpb * blockDim.x * VEC;
   } else {
     u0 = gtid() * VEC;
     uend = q;
