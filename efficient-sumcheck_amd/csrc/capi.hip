@@ -226,8 +226,11 @@ int prof_end(effsc_hip_prover* h, int slot, double bytes, double muls) {
   return EFFSC_HIP_OK;
 }
 
-// Goldilocks products per field product (Karatsuba F_p^3: 6 + 2 by w) and
-// per fold product c * (hi - lo) with c in FO, the difference in FI.
+// Algorithmic Goldilocks products per field product and per fold product
+// c * (hi - lo) with c in FO, the difference in FI.  An F_p^3 product counts
+// as Karatsuba's 8 full products (6 + 2 by w) whatever the kernels execute:
+// they run 9 or 11 unreduced 64x64 products with one reduction per
+// coefficient (gl::mul(Fp3, Fp3Pre)), so the count is a fixed unit of work.
 template <class F>
 constexpr double mul_cost() {
   return F::D == 1 ? 1.0 : 8.0;
