@@ -156,6 +156,14 @@ struct effsc_hip_prover {
   // they stay valid across resets)
   uint64_t* d_pass0 = nullptr;
   bool pass0_ready = false;
+  // Out-of-core stream provers: the input stays in the host streams; the two
+  // passes that read it (rounds 0-1 at construction, rounds 2-3 in the proof)
+  // move it through pinned and device staging chunk by chunk, and only the
+  // level-2 tables (n/4 elements each) are ever resident
+  bool ooc = false;
+  const effsc_hip_stream* ooc_src[kMaxK] = {};
+  uint64_t* ooc_parts = nullptr;  // partials of the streamed fold pass
+  size_t ooc_nparts = 0;
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
   unsigned last_grid = 0;
@@ -542,6 +550,7 @@ void to_wire(const effsc_hip_prover* h, const uint64_t* vals, uint64_t* out) {
 }
 
 int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
+int ooc_fold_pass(effsc_hip_prover* h, size_t q, unsigned* nblocks);
 int eq_finalize(effsc_hip_prover* h, const uint64_t* chal);
 int eq_final_value(effsc_hip_prover* h, uint64_t* out);
 int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
@@ -761,6 +770,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->d_pass0) (void)hipFree(h->d_pass0);
+  if (h->ooc_parts) (void)hipFree(h->ooc_parts);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
@@ -1460,6 +1470,11 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
         // the pass over the input was accumulated while it streamed in
         red = Reduce{h->d_pass0, h->red.result};
         g = 1;
+      } else if (h->ooc && !h->in_work) {
+        // out-of-core input: the fold pass streams it from the host again
+        if (p != 2) return fail(EFFSC_HIP_ESTATE, "out-of-core pass order");
+        TRY(ooc_fold_pass(h, L / 4, &g));
+        red = Reduce{h->ooc_parts, h->red.result};
       } else {
         TRY(launch_pass(h, p, L / 4, p ? h->d_cur : nullptr));
         g = h->last_grid;
@@ -1986,8 +2001,18 @@ int effsc_hip_ext(const effsc_hip_prover* h, size_t* out) {
   return EFFSC_HIP_OK;
 }
 
+// An out-of-core prover's input is only read by whole device-resident proofs.
+static int ooc_guard(const effsc_hip_prover* h) {
+  if (h->ooc && !h->in_work)
+    return fail(EFFSC_HIP_ESTATE,
+                "out-of-core stream prover: the input stays in the streams; run effsc_hip_sumcheck over all rounds "
+                "without a hook first");
+  return EFFSC_HIP_OK;
+}
+
 int effsc_hip_round(effsc_hip_prover* h, const uint64_t* challenge, uint64_t* out) {
   if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  TRY(ooc_guard(h));
   HIPCHK(hipSetDevice(h->dev));
   uint64_t c[3];
   if (challenge) words_in(h, challenge, c, h->ext);
@@ -2000,6 +2025,7 @@ int effsc_hip_round(effsc_hip_prover* h, const uint64_t* challenge, uint64_t* ou
 
 int effsc_hip_finalize(effsc_hip_prover* h, const uint64_t* challenge) {
   if (!h || !challenge) return fail(EFFSC_HIP_EINVAL, "null argument");
+  TRY(ooc_guard(h));
   HIPCHK(hipSetDevice(h->dev));
   uint64_t c[3];
   words_in(h, challenge, c, h->ext);
@@ -2011,6 +2037,7 @@ int effsc_hip_finalize(effsc_hip_prover* h, const uint64_t* challenge) {
 
 int effsc_hip_final_value(effsc_hip_prover* h, uint64_t* out) {
   if (!h || !out) return fail(EFFSC_HIP_EINVAL, "null argument");
+  TRY(ooc_guard(h));
   HIPCHK(hipSetDevice(h->dev));
   uint64_t v[3];
   TRY(final_value_impl(h, v));
@@ -2044,6 +2071,7 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
   const int E = h->ext;
   if (!dst) return cap_words == 0 ? EFFSC_HIP_OK : fail(EFFSC_HIP_EINVAL, "null destination");
   if (cap_words < L * E) return fail(EFFSC_HIP_EINVAL, "destination too small");
+  TRY(ooc_guard(h));
   HIPCHK(hipSetDevice(h->dev));
   TRY(materialize(h));
   const uint64_t* planes;
@@ -2195,6 +2223,12 @@ int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcr
   if (!h || !t || !final_value || (num_rounds && (!round_polys || !challenges)))
     return fail(EFFSC_HIP_EINVAL, "null argument");
   HIPCHK(hipSetDevice(h->dev));
+  if (h->ooc && !h->in_work) {
+    // the input is read by the two streamed passes of a whole device proof
+    const bool whole = num_rounds == h->v && !hook && device_loop_applies(h, num_rounds) &&
+                       !std::getenv("EFFSC_HIP_HOST_LOOP") && h->len == h->n && h->n / 4 > tail_max(h);
+    if (!whole) TRY(ooc_guard(h));
+  }
   if (!hook && device_loop_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
     return prove_device(h, num_rounds, t, round_polys, challenges, final_value);
   if (!hook && eq_device_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
@@ -2465,22 +2499,24 @@ size_t stream_chunk_units() {
 // Round 0 values and round 1 coefficients (the P = 0 two-round pass) of the
 // units [u0, u0 + count) of resident base tables, one partial per block;
 // flags any word >= p (a serialized element must be canonical).
+// Unit u's element j of table i is t.t[i].p[(u - base) + j * stride]: the
+// resident table (base 0, stride q) or a staged chunk (base u0, stride B).
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_stream_pass(Views<Fp, K> t, size_t q, size_t u0, size_t count,
-                                                        uint64_t* partials, unsigned* bad) {
+__global__ __launch_bounds__(kBlock) void k_stream_pass(Views<Fp, K> t, size_t stride, size_t base, size_t u0,
+                                                        size_t count, uint64_t* partials, unsigned* bad) {
   PassAcc<K> acc;
 #pragma unroll
   for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
   unsigned b = 0;
   const size_t step = (size_t)gridDim.x * blockDim.x;
   for (size_t k = gtid(); k < count; k += step) {
-    const size_t u = u0 + k;
+    const size_t u = u0 + k - base;
     uint64_t E[K][4];
 #pragma unroll
     for (int i = 0; i < K; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        E[i][j] = t.t[i].p[u + j * q];
+        E[i][j] = t.t[i].p[u + j * stride];
         b |= E[i][j] >= gl::P;
       }
     pass_accumulate<K>(E, acc);
@@ -2491,6 +2527,188 @@ __global__ __launch_bounds__(kBlock) void k_stream_pass(Views<Fp, K> t, size_t q
   for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
   pass_values_from_raw<K>(raw, vals);
   finish<Fp, pass_values<K>()>(vals, Reduce{partials, nullptr});
+}
+
+// The streamed fold pass of an out-of-core prover (the P = 2 two-round pass
+// from the input, pass2.hpp): the staged chunk holds, per table, the 16
+// regions m q + [u0, u0 + count) of the input (M = 16 q elements) at m B;
+// unit u folds input elements u + (j + 4t) q (t < 4) with c1 = cptr[0],
+// c2 = cptr[1] into the level-4q element u + j q of `work` and accumulates
+// the pass values, one partial per block.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_stream_fold(Views<Fp, K> stage, size_t B, size_t u0, size_t count,
+                                                        Views<Fp, K> work, size_t q, const uint64_t* cptr,
+                                                        uint64_t* partials) {
+  const uint64_t c1 = cptr[0], c2 = cptr[1];
+  PassAcc<K> acc;
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < count; k += step) {
+    const size_t u = u0 + k;
+    uint64_t E[K][4];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint64_t* x = stage.t[i].p + k;
+        const uint64_t x0 = x[j * B], x1 = x[(j + 4) * B], x2 = x[(j + 8) * B], x3 = x[(j + 12) * B];
+        const uint64_t a = gl::add(x0, gl::mul(c1, gl::sub(x2, x0)));
+        const uint64_t b = gl::add(x1, gl::mul(c1, gl::sub(x3, x1)));
+        E[i][j] = gl::add(a, gl::mul(c2, gl::sub(b, a)));
+        work.t[i].p[u + j * q] = E[i][j];
+      }
+    pass_accumulate<K>(E, acc);
+  }
+  Fp raw[pass_raw<K>()], vals[pass_values<K>()];
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
+  finish<Fp, pass_values<K>()>(vals, Reduce{partials, nullptr});
+}
+
+// Chunked transfer of an out-of-core prover's input: the n elements of each
+// of the K streams are cut into nreg regions of R = n / nreg; chunk c moves
+// elements [c B, c B + cnt) of every region (host pages -> pinned buffer ->
+// device staging, double-buffered, on a copy stream) and then `consume(c, u0,
+// cnt, staged)` enqueues the chunk's kernel on the prover's stream, which
+// sees table i region m at staged + (i nreg + m) B.  Returns after the last
+// kernel has finished reading the staging.
+struct OocChunks {
+  size_t B = 0, nch = 0;
+  unsigned G = 0;  // workgroups per chunk kernel
+};
+OocChunks ooc_chunks(size_t R) {
+  OocChunks c;
+  c.B = std::max<size_t>(1, std::min(R, stream_chunk_units() >> 2));
+  c.nch = (R + c.B - 1) / c.B;
+  c.G = (unsigned)std::min<size_t>((c.B + kBlock - 1) / kBlock, 1024);
+  return c;
+}
+
+template <class Consume>
+int ooc_stream(effsc_hip_prover* h, size_t nreg, Consume&& consume) {
+  const int K = h->K;
+  const size_t R = h->n / nreg;
+  const OocChunks ch = ooc_chunks(R);
+  const size_t words = (size_t)K * nreg * ch.B;
+  int rc = EFFSC_HIP_OK;
+  hipStream_t cs = nullptr;
+  uint8_t* pin[2] = {};
+  uint64_t* stage[2] = {};
+  hipEvent_t copied[2] = {}, used[2] = {};
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == EFFSC_HIP_OK) rc = fail(EFFSC_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return rc == EFFSC_HIP_OK;
+  };
+  chk(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream");
+  for (int b = 0; b < 2 && rc == EFFSC_HIP_OK; b++) {
+    chk(hipHostMalloc((void**)&pin[b], words * 8, hipHostMallocDefault), "pinned staging");
+    if (rc == EFFSC_HIP_OK) rc = alloc_dev(&stage[b], words);
+    chk(hipEventCreateWithFlags(&copied[b], hipEventDisableTiming), "event");
+    chk(hipEventCreateWithFlags(&used[b], hipEventDisableTiming), "event");
+  }
+  for (size_t c = 0; c < ch.nch && rc == EFFSC_HIP_OK; c++) {
+    const int b = (int)(c & 1);
+    const size_t u0 = c * ch.B, cnt = std::min(ch.B, R - u0);
+    // pinned buffer b was last read by the copy of chunk c - 2
+    if (c >= 2 && !chk(hipEventSynchronize(copied[b]), "wait")) break;
+    for (int i = 0; i < K; i++)
+      for (size_t m = 0; m < nreg; m++)
+        par_copy(pin[b] + ((size_t)i * nreg + m) * ch.B * 8, h->ooc_src[i]->map + (m * R + u0) * 8, cnt * 8);
+    // device staging b was last read by the kernel of chunk c - 2
+    if (c >= 2) chk(hipStreamWaitEvent(cs, used[b], 0), "wait");
+    chk(hipMemcpyAsync(stage[b], pin[b], words * 8, hipMemcpyHostToDevice, cs), "copy");
+    chk(hipEventRecord(copied[b], cs), "event");
+    chk(hipStreamWaitEvent(h->stream, copied[b], 0), "wait");
+    if (rc != EFFSC_HIP_OK) break;
+    TRY(consume(c, u0, cnt, stage[b], ch));
+    chk(hipEventRecord(used[b], h->stream), "event");
+  }
+  if (cs) chk(hipStreamSynchronize(cs), "sync");
+  chk(hipStreamSynchronize(h->stream), "sync");
+  for (int b = 0; b < 2; b++) {
+    if (pin[b]) (void)hipHostFree(pin[b]);
+    if (stage[b]) (void)hipFree(stage[b]);
+    if (copied[b]) (void)hipEventDestroy(copied[b]);
+    if (used[b]) (void)hipEventDestroy(used[b]);
+  }
+  if (cs) (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+template <int K>
+Views<Fp, K> staged_views(uint64_t* staged, size_t nreg, size_t B) {
+  Views<Fp, K> v{};
+  for (int i = 0; i < K; i++) v.t[i] = View<Fp>{staged + (size_t)i * nreg * B, B};
+  return v;
+}
+
+// Rounds 0-1 of an out-of-core prover: round 0 values and round 1
+// coefficients of the input into d_pass0, with the canonical-word check.
+int ooc_pass0(effsc_hip_prover* h) {
+  const size_t q = h->n / 4;
+  const OocChunks ch = ooc_chunks(q);
+  constexpr int NVmax = kMaxK + kMaxK * (kMaxK + 1);
+  uint64_t* parts = nullptr;
+  unsigned* bad = nullptr;
+  TRY(alloc_dev(&parts, ch.nch * ch.G * NVmax));
+  int rc = alloc_dev(reinterpret_cast<uint64_t**>(&bad), 1);
+  if (rc == EFFSC_HIP_OK && hipMemsetAsync(bad, 0, 4, h->stream) != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, "flag");
+  if (rc == EFFSC_HIP_OK)
+    rc = ooc_stream(h, 4, [&](size_t c, size_t u0, size_t cnt, uint64_t* st, const OocChunks& cc) -> int {
+      if (h->K == 1)
+        hipLaunchKernelGGL(k_stream_pass<1>, dim3(cc.G), dim3(kBlock), 0, h->stream, staged_views<1>(st, 4, cc.B), cc.B,
+                           u0, u0, cnt, parts + c * cc.G * pass_values<1>(), bad);
+      else
+        hipLaunchKernelGGL(k_stream_pass<2>, dim3(cc.G), dim3(kBlock), 0, h->stream, staged_views<2>(st, 4, cc.B), cc.B,
+                           u0, u0, cnt, parts + c * cc.G * pass_values<2>(), bad);
+      HIPCHK(hipGetLastError());
+      return EFFSC_HIP_OK;
+    });
+  if (rc == EFFSC_HIP_OK) {
+    if (h->K == 1)
+      hipLaunchKernelGGL((k_reduce<Fp, pass_values<1>()>), dim3(1), dim3(1024), 0, h->stream,
+                         Reduce{parts, h->d_pass0}, (unsigned)(ch.nch * ch.G));
+    else
+      hipLaunchKernelGGL((k_reduce<Fp, pass_values<2>()>), dim3(1), dim3(1024), 0, h->stream,
+                         Reduce{parts, h->d_pass0}, (unsigned)(ch.nch * ch.G));
+    if (hipGetLastError() != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, "reduce");
+  }
+  unsigned hbad = 0;
+  if (rc == EFFSC_HIP_OK && hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    rc = fail(EFFSC_HIP_EDEVICE, "copy back");
+  if (rc == EFFSC_HIP_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(EFFSC_HIP_EDEVICE, "sync");
+  if (rc == EFFSC_HIP_OK && hbad) rc = fail(EFFSC_HIP_EINVAL, "non-canonical serialized element");
+  (void)hipStreamSynchronize(h->stream);
+  if (parts) (void)hipFree(parts);
+  if (bad) (void)hipFree(bad);
+  return rc;
+}
+
+// Rounds 2-3 of an out-of-core prover: the P = 2 pass over the streamed
+// input with the two challenges in d_cur; writes the level-n/4 tables into
+// `work` and leaves nblocks partials in ooc_parts.
+int ooc_fold_pass(effsc_hip_prover* h, size_t q, unsigned* nblocks) {
+  const OocChunks ch = ooc_chunks(q);
+  if (ch.nch * ch.G > h->ooc_nparts) return fail(EFFSC_HIP_ESTATE, "out-of-core partials too small");
+  Views<Fp, 2> wv{};
+  for (int i = 0; i < h->K; i++) wv.t[i] = View<Fp>{h->work[i], h->work_stride};
+  TRY(ooc_stream(h, 16, [&](size_t c, size_t u0, size_t cnt, uint64_t* st, const OocChunks& cc) -> int {
+    if (h->K == 1) {
+      Views<Fp, 1> w1{};
+      w1.t[0] = wv.t[0];
+      hipLaunchKernelGGL(k_stream_fold<1>, dim3(cc.G), dim3(kBlock), 0, h->stream, staged_views<1>(st, 16, cc.B), cc.B,
+                         u0, cnt, w1, q, h->d_cur, h->ooc_parts + c * cc.G * pass_values<1>());
+    } else {
+      hipLaunchKernelGGL(k_stream_fold<2>, dim3(cc.G), dim3(kBlock), 0, h->stream, staged_views<2>(st, 16, cc.B), cc.B,
+                         u0, cnt, wv, q, h->d_cur, h->ooc_parts + c * cc.G * pass_values<2>());
+    }
+    HIPCHK(hipGetLastError());
+    return EFFSC_HIP_OK;
+  }));
+  *nblocks = (unsigned)(ch.nch * ch.G);
+  return EFFSC_HIP_OK;
 }
 
 // Flags non-canonical words of a resident base table.
@@ -2677,6 +2895,68 @@ int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, 
   return rc;
 }
 
+int effsc_hip_stream_prover_new_ex(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams, int mode,
+                                   effsc_hip_prover** out) {
+  if (mode != EFFSC_HIP_STREAM_RESIDENT && mode != EFFSC_HIP_STREAM_OUT_OF_CORE && mode != EFFSC_HIP_STREAM_AUTO)
+    return fail(EFFSC_HIP_EINVAL, "stream prover mode: resident (0), out of core (1) or auto (2)");
+  if (mode == EFFSC_HIP_STREAM_RESIDENT) return effsc_hip_stream_prover_new(cfg, kind, streams, out);
+  if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
+  *out = nullptr;
+  TRY(check_cfg(cfg));
+  if (kind != K_ML && kind != K_IP) return fail(EFFSC_HIP_EINVAL, "stream provers: multilinear (1) or inner product (2)");
+  if (cfg->input_ext != 1 || cfg->ext != 1)
+    return fail(EFFSC_HIP_EINVAL, "out-of-core stream provers: base-field streams and challenges");
+  const int K = kind;
+  if (!streams) return fail(EFFSC_HIP_EINVAL, "null streams");
+  for (int i = 0; i < K; i++)
+    if (!streams[i] || streams[i]->ext != 1) return fail(EFFSC_HIP_EINVAL, "stream provers read base-field streams");
+  const size_t n = streams[0]->n;
+  if (K == 2 && streams[1]->n != n) return fail(EFFSC_HIP_EINVAL, "a and b must have equal length");
+  HIPCHK(hipSetDevice(cfg->device));
+  if (mode == EFFSC_HIP_STREAM_AUTO) {
+    // resident when the input, its level-1 work tables and a margin fit
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)K * (double)n * 8.0 * 1.5 + (double)(1 << 30);
+    if (need < (double)free_b) return effsc_hip_stream_prover_new(cfg, kind, streams, out);
+  }
+  auto* h = new effsc_hip_prover();
+  h->kind = kind;
+  h->ext = 1;
+  h->in_ext = 1;
+  h->K = K;
+  h->deg = K;
+  h->repr = cfg->repr;
+  h->dev = cfg->device;
+  h->n = n;
+  h->len = n;
+  h->twostep = !std::getenv("EFFSC_HIP_ONE_ROUND");
+  while (((size_t)1 << h->v) < n) h->v++;
+  h->ooc = true;
+  h->own_in = false;
+  for (int i = 0; i < K; i++) h->ooc_src[i] = streams[i];
+  int rc = EFFSC_HIP_OK;
+  // the two streamed passes end at level n/4, which must still be above the
+  // one-workgroup tail
+  if (!h->twostep || n / 4 <= tail_max(h) || n < 64)
+    rc = fail(EFFSC_HIP_EINVAL, "out-of-core stream provers need the two-round passes and n/4 above the tail size");
+  if (rc == EFFSC_HIP_OK) rc = setup_common(h, n / 4);
+  if (rc == EFFSC_HIP_OK) rc = alloc_dev(&h->d_pass0, kMaxK + kMaxK * (kMaxK + 1));
+  if (rc == EFFSC_HIP_OK) {
+    const OocChunks ch = ooc_chunks(n / 16);
+    h->ooc_nparts = ch.nch * ch.G;
+    rc = alloc_dev(&h->ooc_parts, h->ooc_nparts * (kMaxK + kMaxK * (kMaxK + 1)));
+  }
+  if (rc == EFFSC_HIP_OK) rc = ooc_pass0(h);
+  if (rc != EFFSC_HIP_OK) {
+    destroy(h);
+    return rc;
+  }
+  h->pass0_ready = true;
+  *out = h;
+  return EFFSC_HIP_OK;
+}
+
 int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams,
                                 effsc_hip_prover** out) {
   // MultilinearProver / InnerProductProver over tables read from streams
@@ -2759,10 +3039,10 @@ int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip
       if (K == 1) {
         Views<Fp, 1> t1{};
         t1.t[0] = tv.t[0];
-        hipLaunchKernelGGL(k_stream_pass<1>, dim3(G), dim3(kBlock), 0, h->stream, t1, q, u0, cnt,
+        hipLaunchKernelGGL(k_stream_pass<1>, dim3(G), dim3(kBlock), 0, h->stream, t1, q, (size_t)0, u0, cnt,
                            parts + c * G * pass_values<1>(), bad);
       } else {
-        hipLaunchKernelGGL(k_stream_pass<2>, dim3(G), dim3(kBlock), 0, h->stream, tv, q, u0, cnt,
+        hipLaunchKernelGGL(k_stream_pass<2>, dim3(G), dim3(kBlock), 0, h->stream, tv, q, (size_t)0, u0, cnt,
                            parts + c * G * pass_values<2>(), bad);
       }
       chk(hipGetLastError(), "stream pass");

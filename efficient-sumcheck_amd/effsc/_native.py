@@ -101,6 +101,7 @@ _SIGS = {
     "effsc_hip_stream_close": (None, [vp]),
     "effsc_hip_stream_claim": (C.c_int, [C.c_int, C.POINTER(vp), C.c_int, u64, u64p]),
     "effsc_hip_stream_prover_new": (C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+    "effsc_hip_stream_prover_new_ex": (C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp)]),
     "effsc_hip_server_new": (C.c_int, [C.c_int, sz, sz, C.POINTER(vp)]),
     "effsc_hip_server_footprint": (C.c_int, [C.POINTER(Request), szp]),
     "effsc_hip_server_submit": (C.c_int, [vp, C.POINTER(Request), u64p]),

@@ -81,21 +81,28 @@ def multivariate_claim(stream, w=W_FROM_RAW7, device=0):
     return multivariate_product_claim([stream], w, device)
 
 
-def _stream_prover(cls, kind, streams, ext, repr, w, device):
+RESIDENT, OUT_OF_CORE, AUTO = 0, 1, 2
+
+
+def _stream_prover(cls, kind, streams, ext, repr, w, device, mode=RESIDENT):
     p = cls.__new__(cls)
     NativeProver.__init__(p, ext, repr)
     cfg = _config(ext, 1, repr, w, device, 0)
     h = C.c_void_p()
-    N.check(N.lib().effsc_hip_stream_prover_new(C.byref(cfg), kind, _handles(streams), C.byref(h)))
+    N.check(N.lib().effsc_hip_stream_prover_new_ex(C.byref(cfg), kind, _handles(streams), mode, C.byref(h)))
     p._finish(h)
+    # an out-of-core prover reads the streams again during its proof
+    p._streams = list(streams)
     return p
 
 
-def multilinear_from_stream(stream, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0):
-    """MultilinearProver over a stream's table."""
-    return _stream_prover(MultilinearProver, 1, [stream], ext, repr, w, device)
+def multilinear_from_stream(stream, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0, mode=RESIDENT):
+    """MultilinearProver over a stream's table.  mode OUT_OF_CORE keeps the
+    table in the stream (whole proofs only); AUTO picks by device memory."""
+    return _stream_prover(MultilinearProver, 1, [stream], ext, repr, w, device, mode)
 
 
-def inner_product_from_stream(a, b, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0):
-    """InnerProductProver over two streams' tables."""
-    return _stream_prover(InnerProductProver, 2, [a, b], ext, repr, w, device)
+def inner_product_from_stream(a, b, *, ext=1, repr=CANONICAL, w=W_FROM_RAW7, device=0, mode=RESIDENT):
+    """InnerProductProver over two streams' tables (mode as for
+    multilinear_from_stream)."""
+    return _stream_prover(InnerProductProver, 2, [a, b], ext, repr, w, device, mode)

@@ -286,6 +286,23 @@ int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, 
  * stream; effsc_hip_sumcheck then starts from that pass.  cfg.input_ext = 1. */
 int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams,
                                 effsc_hip_prover** out);
+/* The same provers for inputs larger than device memory (FileStream's
+ * purpose, src/streams/file/core.rs:13-76; the reference reads such a stream
+ * on the CPU).  mode EFFSC_HIP_STREAM_RESIDENT is effsc_hip_stream_prover_new;
+ * EFFSC_HIP_STREAM_OUT_OF_CORE keeps the input in the streams: rounds 0-1 are
+ * accumulated here while it passes through pinned and device staging chunk by
+ * chunk, and the proof streams it once more for rounds 2-3, folding it on the
+ * fly into level-n/4 tables, the only resident ones (K n/4 elements of device
+ * memory instead of K (n + n/2)).  The streams must stay open for the
+ * prover's lifetime.  Such a prover runs whole proofs only (effsc_hip_sumcheck
+ * over all rounds, no hook; ESTATE otherwise until that proof has run) and
+ * needs n/4 above the one-workgroup tail (n >= 2^17 by default).
+ * EFFSC_HIP_STREAM_AUTO picks resident when the input fits with a margin. */
+#define EFFSC_HIP_STREAM_RESIDENT 0
+#define EFFSC_HIP_STREAM_OUT_OF_CORE 1
+#define EFFSC_HIP_STREAM_AUTO 2
+int effsc_hip_stream_prover_new_ex(const effsc_hip_config* cfg, int kind, effsc_hip_stream* const* streams, int mode,
+                                   effsc_hip_prover** out);
 
 /* ---------------------------------------------------------------------------
  * Served proofs: a request loop with round-level continuous batching.  No

@@ -8,8 +8,8 @@ import pytest
 import oracle as O
 from effsc import TestTranscript, Transcript, sumcheck
 from effsc import _native as N
-from effsc.streams import (FileStream, inner_product_from_stream, multilinear_from_stream, multivariate_claim,
-                           multivariate_product_claim)
+from effsc.streams import (AUTO, OUT_OF_CORE, FileStream, inner_product_from_stream, multilinear_from_stream,
+                           multivariate_claim, multivariate_product_claim)
 
 pytestmark = pytest.mark.gpu
 P = O.P
@@ -102,3 +102,53 @@ def test_stream_prover_rejects_non_canonical(tmp_path, monkeypatch):
         multilinear_from_stream(FileStream(p))
     with pytest.raises(N.NativeError):
         multivariate_claim(FileStream(p))
+
+
+def _make(kind, s, mode):
+    return multilinear_from_stream(s[0], mode=mode) if kind == "ml" else inner_product_from_stream(s[0], s[1], mode=mode)
+
+
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("v,chunk,tail", [(17, None, None), (18, "4096", None), (19, "64", None), (16, "1024", "4096")])
+def test_out_of_core_prover_matches_oracle(tmp_path, monkeypatch, kind, v, chunk, tail):
+    # the input stays in the files: rounds 0-1 while it streams through the
+    # staging at construction, rounds 2-3 while it streams again in the proof
+    for name, val in (("EFFSC_HIP_STREAM_CHUNK", chunk), ("EFFSC_HIP_TAIL", tail)):
+        if val:
+            monkeypatch.setenv(name, val)
+        else:
+            monkeypatch.delenv(name, raising=False)
+    k = 1 if kind == "ml" else 2
+    fs = files(tmp_path, k, 1 << v, 900 + v)
+    say(f"out-of-core prover kind={kind} v={v} chunk={chunk} tail={tail}")
+    s = [FileStream(p) for p, _ in fs]
+    g = _make(kind, s, OUT_OF_CORE)
+    # before its whole proof the prover has no resident input
+    with pytest.raises(N.NativeError):
+        g.table_words(0)
+    with pytest.raises(N.NativeError):
+        sumcheck(g, v - 1, Transcript(b"ooc"))
+    with pytest.raises(N.NativeError):
+        sumcheck(g, v, Transcript(b"ooc"), lambda r, t: None)
+    o = O.Prover.multilinear(fs[0][1]) if k == 1 else O.Prover.inner_product(fs[0][1], fs[1][1])
+    po = o.sumcheck(v, O.Transcript.shake(b"ooc"))
+    assert tuple(as_oracle(sumcheck(g, v, Transcript(b"ooc")), 1)) == tuple(po)
+    for i in range(k):
+        assert np.array_equal(g.table_words(i), o.table(i)), f"table {i}"
+    # reset streams the input again; the test transcript gives other folds
+    g.reset()
+    o2 = O.Prover.multilinear(fs[0][1]) if k == 1 else O.Prover.inner_product(fs[0][1], fs[1][1])
+    assert tuple(as_oracle(sumcheck(g, v, TestTranscript(5)), 1)) == tuple(o2.sumcheck(v, O.Transcript.rng(5)))
+    g.close()
+
+
+def test_out_of_core_rejects_small_and_auto_picks_resident(tmp_path, monkeypatch):
+    monkeypatch.delenv("EFFSC_HIP_TAIL", raising=False)
+    fs = files(tmp_path, 2, 1 << 15, 31)
+    s = [FileStream(p) for p, _ in fs]
+    with pytest.raises(N.NativeError):
+        inner_product_from_stream(s[0], s[1], mode=OUT_OF_CORE)  # n/4 is not above the tail
+    g = inner_product_from_stream(s[0], s[1], mode=AUTO)  # fits: resident, per-round calls allowed
+    o = O.Prover.inner_product(fs[0][1], fs[1][1])
+    hooked = sumcheck(g, 15, TestTranscript(3), lambda r, t: None)
+    assert tuple(as_oracle(hooked, 1)) == tuple(o.sumcheck(15, O.Transcript.rng(3)))
