@@ -13,7 +13,12 @@ Other workloads (--workload):
   product3  CoefficientProver, product of 3 multilinears (degree 3), v = 24,
             base-field inputs, challenges and folded tables in F_p^3
   eq, eq3   EqFactoredProver, v = 24, random point, base field / F_p^3
+  lsb       InnerProductProverLSB, v = 28 (tables bit-reversed once into the
+            MSB path at construction)
+  gkr       GkrProver, k = 12: a 2^24-point layer, 24 rounds over four tables
   batch     512 inner-product instances, v ~ U[12, 24], batched vs one by one
+  serve     served request mix, saturated and Poisson, learned admission
+  stream    the v = 28 inner product from host files, resident and out of core
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline     - the dominant kernel class's achieved rate: its algorithmic
@@ -39,6 +44,7 @@ SEED_A, SEED_B = 0x5EED0002, 0x5EED0003
 SEED_ML = 0x5EED0001
 SEEDS_P3 = (0x5EED0031, 0x5EED0032, 0x5EED0033)
 SEED_EQ_P, SEED_EQ_W = 0x5EED0034, 0x5EED0035
+SEEDS_GKR = (0x5EED0041, 0x5EED0042, 0x5EED0043)  # add, mult, witness
 DOMSEP = b"effsc-hip/bench/ip"
 
 CLASS_NAMES = ["k_fused (fold + evaluate)", "k_eval (evaluate)", "k_fold (fold)",
@@ -55,6 +61,10 @@ WORKLOADS = {
                metric="Goldilocks sumcheck evals/s (EqFactoredProver, base field, v=24)"),
     "eq3": dict(kind="eq", v=24, ext=3, input_ext=3, tables=1,
                 metric="Goldilocks sumcheck evals/s (EqFactoredProver, F_p^3, v=24)"),
+    "lsb": dict(kind="ip_lsb", v=28, ext=1, input_ext=1, tables=2,
+                metric="Goldilocks sumcheck evals/s (InnerProductProverLSB, v=28)"),
+    "gkr": dict(kind="gkr", v=24, ext=1, input_ext=1, tables=4,
+                metric="Goldilocks sumcheck evals/s (GkrProver, k=12: 2^24-point layer, 24 rounds)"),
 }
 
 
@@ -102,7 +112,13 @@ def cpu_baseline(workload, v_sample, threads):
         t, _, _, _ = O.bench_ip(a, b, v_sample, threads)
         desc = "two parallel folds per round, serial evaluate, scalar Montgomery arithmetic"
         return n / t, t, threads, desc
-    if workload == "ml":
+    if workload == "lsb":
+        p = O.Prover.inner_product_lsb(O.uniform(n, SEED_A), O.uniform(n, SEED_B))
+    elif workload == "gkr":
+        k = v_sample // 2
+        m = 1 << k
+        p = O.Prover.gkr(O.uniform(m * m, SEEDS_GKR[0]), O.uniform(m * m, SEEDS_GKR[1]), O.uniform(m, SEEDS_GKR[2]))
+    elif workload == "ml":
         p = O.Prover.multilinear(O.uniform(n, SEED_ML))
     elif workload == "product3":
         p = O.Prover.product([O.uniform(n, s) for s in SEEDS_P3], ext=3)
@@ -129,8 +145,11 @@ def device_tables(L, N, dev, spec):
         made.append(p)
         return C.cast(p, C.c_void_p).value
 
-    if kind == "ip":
+    if kind in ("ip", "ip_lsb"):
         addrs = [table(n, SEED_A), table(n, SEED_B)]
+    elif kind == "gkr":
+        m = 1 << (v // 2)
+        addrs = [table(n, SEEDS_GKR[0]), table(n, SEEDS_GKR[1]), table(m, SEEDS_GKR[2])]
     elif kind == "ml":
         addrs = [table(n, SEED_ML)]
     elif kind == "product":
@@ -157,6 +176,19 @@ def make_prover(effsc, L, N, dev, spec, addrs):
     kind = spec["kind"]
     if kind == "ip":
         return effsc.InnerProductProver(addrs[0], addrs[1], borrow=True, n=n, device=dev)
+    if kind == "ip_lsb":
+        # the prover bit-reverses the tables into its own planes once
+        return effsc.InnerProductProverLSB(addrs[0], addrs[1], borrow=True, n=n, device=dev)
+    if kind == "gkr":
+        # GkrProver takes host tables (it builds the broadcast witness tables)
+        import numpy as np
+        m = 1 << (v // 2)
+        host = []
+        for a, cnt in zip(addrs, (n, n, m)):
+            h = np.zeros(cnt, dtype=np.uint64)
+            N.check(L.effsc_hip_copy(dev, h.ctypes.data_as(N.u64p), C.cast(a, N.u64p), cnt))
+            host.append(h)
+        return effsc.GkrProver(host[0], host[1], host[2], device=dev)
     if kind == "ml":
         return effsc.MultilinearProver(addrs[0], borrow=True, n=n, device=dev)
     if kind == "product":
@@ -167,6 +199,10 @@ def make_prover(effsc, L, N, dev, spec, addrs):
 
 def proof_bytes(spec):
     v, kind, ext = spec["v"], spec["kind"], spec["ext"]
+    if kind == "gkr":
+        # four tables of 2^{2k} elements, folded every round (the witness
+        # broadcasts included, as the reference materialises them)
+        return algorithmic_bytes(v, tables=4, in_words=1, work_words=1)
     if kind == "product":
         return algorithmic_bytes(v, tables=3, in_words=1, work_words=3)
     return algorithmic_bytes(v, tables=spec["tables"], in_words=ext, work_words=ext)
@@ -267,7 +303,8 @@ def bench_single(args, workload):
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         # bounded samples: a few seconds of the serial oracle at most
-        default_cpu_v = {"ip": v, "ml": v, "product3": min(v, 22), "eq": v, "eq3": min(v, 22)}[workload]
+        default_cpu_v = {"ip": v, "ml": v, "product3": min(v, 22), "eq": v, "eq3": min(v, 22), "lsb": min(v, 24),
+                         "gkr": min(v, 20)}[workload]
         cv = args.cpu_v if args.cpu_v is not None else default_cpu_v
         log(f"bench: CPU baseline (oracle port of the reference prover), v={cv}")
         rate, secs, cores, desc = cpu_baseline(workload, cv, threads)
@@ -275,7 +312,8 @@ def bench_single(args, workload):
                "sample": f"one full proof at v={cv} ({secs:.2f} s): {desc}"}
 
     inputs = {"ip": "seeds 0x5EED0002/3", "ml": "seed 0x5EED0001", "product3": "seeds 0x5EED0031/32/33",
-              "eq": "p seed 0x5EED0034, point seed 0x5EED0035", "eq3": "p seed 0x5EED0034, point seed 0x5EED0035"}
+              "eq": "p seed 0x5EED0034, point seed 0x5EED0035", "eq3": "p seed 0x5EED0034, point seed 0x5EED0035",
+              "lsb": "seeds 0x5EED0002/3 (LSB order)", "gkr": "add, mult, witness seeds 0x5EED0041/42/43"}
     out = {
         "metric": spec["metric"] if v == WORKLOADS[workload]["v"] else spec["metric"].rsplit("v=", 1)[0] + f"v={v})",
         "value": n / T,
@@ -289,7 +327,7 @@ def bench_single(args, workload):
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{workload}_msb_v{v}", "v": v, "field": "goldilocks" if ext == 1 else "goldilocks^3",
+        "config": {"workload": f"{workload}_v{v}" if workload in ("lsb", "gkr") else f"{workload}_msb_v{v}", "v": v, "field": "goldilocks" if ext == 1 else "goldilocks^3",
                    "tables": spec["tables"], "transcript": "shake256-fs",
                    "input": f"uniform [0,p) counter-based, {inputs[workload]}"},
         # the reference algorithm's bytes (one fold per round) over the proof time
