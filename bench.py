@@ -274,6 +274,9 @@ def bench_single(args, workload):
         names[1] = "k_eq_round (round 0: evaluate with eq weights)"
     elif spec["kind"] == "product" and ext == 3:
         names[0] = "k_fused_seq (fold + evaluate, one F_p^3 table at a time)"
+    elif spec["kind"] == "gkr":
+        names[0] = "k_gkr_fused (fold the four tables + evaluate q(0), q(1), q(2))"
+        names[1] = "k_gkr_eval (round 0: evaluate q(0), q(1), q(2))"
     if profiled:
         dom = max(range(4), key=lambda i: ms[i])
         kernel = names[dom]

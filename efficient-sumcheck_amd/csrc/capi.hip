@@ -959,6 +959,7 @@ int eq_fold_table(effsc_hip_prover* h, uint64_t* planes, size_t stride, size_t* 
 
 // Reads element 0 of an ext-plane table.
 // ---------------------------------------------------------------------------
+
 // Device-resident EqFactored proof.  A round is the inner-product round of
 // (p, E) with E = eq_l (x) eq_r while the left variables are bound and
 // E = eq_l[0] eq_r afterwards (round_poly_left / round_poly_right,
@@ -1380,6 +1381,146 @@ int gkr_eval(effsc_hip_prover* h, uint64_t* out) {
   return EFFSC_HIP_OK;
 }
 
+// Device-resident GKR rounds.  k_gkr_fused folds the four tables with the
+// device challenge (quadruples k, k + q, k + 2q, k + 3q, q = L/4, as
+// k_fused) and evaluates q(0), q(1), q(2) of the folded pairs;
+// k_gkr_reduce_fs sums the partials, writes the wire [q(0), q(inf)] and runs
+// the transcript step; k_gkr_tail runs the remaining rounds, the final fold
+// and f at the last element in one workgroup.
+template <class FI, class F>
+__global__ __launch_bounds__(kBlock) void k_gkr_fused(Views<FI, 4> in, Views<F, 4> out, const uint64_t* cptr, size_t q,
+                                                      uint64_t w, Reduce r) {
+  const F c = gl::load_words<F>(cptr);
+  F acc[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = gtid(); k < q; k += step) {
+    F lo[4], hi[4], x2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const FI x0 = in.t[j].ld(k), x1 = in.t[j].ld(k + q), y0 = in.t[j].ld(k + 2 * q), y1 = in.t[j].ld(k + 3 * q);
+      lo[j] = fold1<F>(x0, y0, c, w);
+      hi[j] = fold1<F>(x1, y1, c, w);
+      out.t[j].st(k, lo[j]);
+      out.t[j].st(k + q, hi[j]);
+      x2[j] = gl::sub(gl::add(hi[j], hi[j]), lo[j]);
+    }
+    acc[0] = gl::add(acc[0], gkr_f(lo[0], lo[1], lo[2], lo[3], w));
+    acc[1] = gl::add(acc[1], gkr_f(hi[0], hi[1], hi[2], hi[3], w));
+    acc[2] = gl::add(acc[2], gkr_f(x2[0], x2[1], x2[2], x2[3], w));
+  }
+  finish<F, 3>(acc, r);
+}
+
+// q(0), q(1), q(2) -> the wire [q(0), q(inf)], q(inf) = (q(0) + q(2) - 2 q(1)) / 2
+template <class F>
+__device__ __forceinline__ void gkr_wire(const F* q, F* wire) {
+  wire[0] = q[0];
+  wire[1] = gl::mul(gl::sub(gl::add(q[0], q[2]), gl::add(q[1], q[1])), Fp{(gl::P + 1) / 2}, 0);
+}
+
+template <class F>
+__global__ __launch_bounds__(kReduceBlock) void k_gkr_reduce_fs(Reduce r, unsigned nblocks, DevFs* t, int ext,
+                                                                uint64_t* poly, uint64_t* chal_out, uint64_t* chal_cur) {
+  F s[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
+  sum_partials<F, 3>(r.partials, nblocks, s);
+  block_reduce<F, 3>(s);
+  if (threadIdx.x >= 64) return;
+  F wire[2];
+  gkr_wire<F>(s, wire);
+  emit_round<F, 2>(wire, ext, t, poly, chal_out, chal_cur);
+}
+
+template <class FI, class F>
+__global__ __launch_bounds__((tail_block<F, 4>())) void k_gkr_tail(Views<FI, 4> in, Views<F, 4> work, int from_input,
+                                                                  size_t L, int rounds, int have_chal, DevFs* t,
+                                                                  int ext, uint64_t* polys, uint64_t* chals,
+                                                                  const uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
+  __shared__ uint64_t chal_sh[3];
+  if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal ? chal_cur[threadIdx.x] : 0;
+  __syncthreads();
+  const unsigned tid = threadIdx.x, nt = blockDim.x;
+  bool src_in = from_input != 0;
+  for (int r = 0; r < rounds; r++) {
+    F acc[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
+    F lo[4], hi[4], x2[4];
+    if (have_chal) {
+      const F c = gl::load_words<F>(chal_sh);
+      const size_t q = L / 4;
+      for (size_t k = tid; k < q; k += nt) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (src_in) {
+            lo[j] = fold1<F>(in.t[j].ld(k), in.t[j].ld(k + 2 * q), c, w);
+            hi[j] = fold1<F>(in.t[j].ld(k + q), in.t[j].ld(k + 3 * q), c, w);
+          } else {
+            lo[j] = fold1<F>(work.t[j].ld(k), work.t[j].ld(k + 2 * q), c, w);
+            hi[j] = fold1<F>(work.t[j].ld(k + q), work.t[j].ld(k + 3 * q), c, w);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          work.t[j].st(k, lo[j]);
+          work.t[j].st(k + q, hi[j]);
+          x2[j] = gl::sub(gl::add(hi[j], hi[j]), lo[j]);
+        }
+        acc[0] = gl::add(acc[0], gkr_f(lo[0], lo[1], lo[2], lo[3], w));
+        acc[1] = gl::add(acc[1], gkr_f(hi[0], hi[1], hi[2], hi[3], w));
+        acc[2] = gl::add(acc[2], gkr_f(x2[0], x2[1], x2[2], x2[3], w));
+      }
+      L /= 2;
+      src_in = false;
+    } else {
+      const size_t half = L / 2;
+      for (size_t k = tid; k < half; k += nt) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          lo[j] = src_in ? gl::lift_to<F>(in.t[j].ld(k)) : work.t[j].ld(k);
+          hi[j] = src_in ? gl::lift_to<F>(in.t[j].ld(k + half)) : work.t[j].ld(k + half);
+          x2[j] = gl::sub(gl::add(hi[j], hi[j]), lo[j]);
+        }
+        acc[0] = gl::add(acc[0], gkr_f(lo[0], lo[1], lo[2], lo[3], w));
+        acc[1] = gl::add(acc[1], gkr_f(hi[0], hi[1], hi[2], hi[3], w));
+        acc[2] = gl::add(acc[2], gkr_f(x2[0], x2[1], x2[2], x2[3], w));
+      }
+    }
+    block_reduce<F, 3>(acc);
+    if (tid < 64) {
+      F wire[2];
+      gkr_wire<F>(acc, wire);
+      emit_round<F, 2>(wire, ext, t, polys + (size_t)r * 2 * ext, chals + (size_t)r * ext, chal_sh);
+    }
+    __syncthreads();
+    have_chal = 1;
+  }
+  // finalize: fold with the last challenge (src/runner.rs:57-60)
+  if (have_chal && L >= 2) {
+    const F c = gl::load_words<F>(chal_sh);
+    const size_t half = L / 2;
+    for (size_t k = tid; k < half; k += nt) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const F v = src_in ? fold1<F>(in.t[j].ld(k), in.t[j].ld(k + half), c, w)
+                           : fold1<F>(work.t[j].ld(k), work.t[j].ld(k + half), c, w);
+        work.t[j].st(k, v);
+      }
+    }
+    L = half;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    // src/provers/gkr.rs:192-201: f at the single remaining point, else 0
+    F v = gl::zero<F>();
+    if (L == 1 && src_in)
+      v = gkr_f(gl::lift_to<F>(in.t[0].ld(0)), gl::lift_to<F>(in.t[1].ld(0)), gl::lift_to<F>(in.t[2].ld(0)),
+                gl::lift_to<F>(in.t[3].ld(0)), w);
+    else if (L == 1)
+      v = gkr_f(work.t[0].ld(0), work.t[1].ld(0), work.t[2].ld(0), work.t[3].ld(0), w);
+    uint64_t wd[3] = {0, 0, 0};
+    gl::store_words(v, wd);
+    for (int c = 0; c < ext; c++) fin[c] = wd[c];
+  }
+}
+
 int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   // src/provers/gkr.rs:93-101: fold all four tables with the challenge
   if (chal) {
@@ -1665,6 +1806,138 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
   h->eq_l_len = 1;
   h->eq_r_len = 1;
   h->rounds_elapsed = v;
+  return EFFSC_HIP_OK;
+}
+
+// GkrProver proofs on the device (no hook, power-of-two tables): round
+// launches while the tables are large, then one-workgroup rounds; the same
+// wire, transcript and post-state as the per-round path.
+bool gkr_device_applies(const effsc_hip_prover* h, size_t num_rounds) {
+  if (h->kind != K_GKR || num_rounds == 0 || num_rounds > kMaxRounds || !is_pow2(h->len)) return false;
+  size_t logl = 0;
+  while (((size_t)1 << logl) < h->len) logl++;
+  return num_rounds <= logl;
+}
+
+template <class FI, class F>
+int gkr_device_run(effsc_hip_prover* h, size_t num_rounds, bool* tail_used) {
+  const int E = h->ext;
+  const size_t tail = F::D == 3 ? ((size_t)1 << 10) : ((size_t)1 << 12);
+  size_t r = 0;
+  bool have = false;
+  while (r < num_rounds && h->len > tail && h->len >= 4) {
+    const size_t M = h->len;
+    const Views<FI, 4> in = views<FI, 4>(h->in, h->in_stride);
+    const Views<F, 4> wv = views<F, 4>(h->work, h->work_stride);
+    unsigned g;
+    if (have) {
+      g = blocks_for(h, M / 4);
+      TRY(prof_begin(h));
+      if (h->in_work)
+        hipLaunchKernelGGL((k_gkr_fused<F, F>), dim3(g), dim3(kBlock), 0, h->stream, wv, wv, h->d_cur, M / 4, h->w,
+                           h->red);
+      else
+        hipLaunchKernelGGL((k_gkr_fused<FI, F>), dim3(g), dim3(kBlock), 0, h->stream, in, wv, h->d_cur, M / 4, h->w,
+                           h->red);
+      HIPCHK(hipGetLastError());
+      const int din = h->in_work ? F::D : FI::D;
+      TRY(prof_end(h, 0, 8.0 * 4 * ((double)M * din + (double)(M / 2) * F::D), 0.0));
+      h->len = M / 2;
+      h->in_work = true;
+    } else {
+      g = blocks_for(h, M / 2);
+      TRY(prof_begin(h));
+      if (h->in_work)
+        hipLaunchKernelGGL(k_gkr_eval<F>, dim3(g), dim3(kBlock), 0, h->stream, wv, M / 2, M / 2, h->w, h->red);
+      else
+        hipLaunchKernelGGL(k_gkr_eval<FI>, dim3(g), dim3(kBlock), 0, h->stream, in, M / 2, M / 2, h->w, h->red);
+      HIPCHK(hipGetLastError());
+      TRY(prof_end(h, 1, 8.0 * 4 * (double)M * (h->in_work ? F::D : FI::D), 0.0));
+    }
+    if (have || h->in_work)
+      hipLaunchKernelGGL(k_gkr_reduce_fs<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
+                         h->d_polys + r * 2 * E, h->d_chals + r * E, h->d_cur);
+    else
+      hipLaunchKernelGGL(k_gkr_reduce_fs<FI>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
+                         h->d_polys + r * 2 * E, h->d_chals + r * E, h->d_cur);
+    HIPCHK(hipGetLastError());
+    r++;
+    have = true;
+  }
+  const size_t rounds_left = num_rounds - r;
+  const size_t M = h->len;
+  const Views<FI, 4> in = views<FI, 4>(h->in, h->in_stride);
+  const Views<F, 4> wv = views<F, 4>(h->work, h->work_stride);
+  if (rounds_left > 0 || M <= tail) {
+    hipLaunchKernelGGL((k_gkr_tail<FI, F>), dim3(1), dim3(tail_block<F, 4>()), 0, h->stream, in, wv,
+                       h->in_work ? 0 : 1, M, (int)rounds_left, have ? 1 : 0, h->d_fs, E, h->d_polys + r * 2 * E,
+                       h->d_chals + r * E, h->d_cur, h->d_fin, h->w);
+    HIPCHK(hipGetLastError());
+    const size_t folds = rounds_left == 0 ? 0 : (have ? rounds_left : rounds_left - 1);
+    size_t Lt = M >> folds;
+    if ((have || rounds_left > 0) && Lt >= 2) Lt /= 2;
+    h->len = Lt;
+    h->in_work = true;
+    *tail_used = true;
+  } else if (have) {
+    // partial run ending on a large table: finalize with the multi-block fold
+    // (of the input when no round has folded yet)
+    if (h->in_work)
+      hipLaunchKernelGGL((k_fold<F, F, 4>), dim3(blocks_for(h, M / 2)), dim3(kBlock), 0, h->stream, wv, wv,
+                         gl::zero<F>(), h->d_cur, M / 2, M / 2, h->w);
+    else
+      hipLaunchKernelGGL((k_fold<FI, F, 4>), dim3(blocks_for(h, M / 2)), dim3(kBlock), 0, h->stream, in, wv,
+                         gl::zero<F>(), h->d_cur, M / 2, M / 2, h->w);
+    HIPCHK(hipGetLastError());
+    h->len = M / 2;
+    h->in_work = true;
+  }
+  return EFFSC_HIP_OK;
+}
+
+int prove_device_gkr(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, uint64_t* round_polys,
+                     uint64_t* challenges, uint64_t* final_value) {
+  TRY(ensure_dev_bufs(h));
+  DevModeGuard guard(h);
+  const int E = h->ext;
+  DevFs* hs = reinterpret_cast<DevFs*>(h->h_proof + kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4);
+  std::memset(hs, 0, sizeof(DevFs));
+  hs->kind = (uint32_t)t->kind;
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) hs->st[i] = t->fs.sp.st[i];
+    hs->pos = t->fs.sp.pos;
+  }
+  hs->rng = t->rng;
+  HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+  bool tail_used = false;
+  const int fi = src_field(h);
+  if (fi == 1 && E == 1)
+    TRY((gkr_device_run<Fp, Fp>(h, num_rounds, &tail_used)));
+  else if (fi == 1)
+    TRY((gkr_device_run<Fp, Fp3>(h, num_rounds, &tail_used)));
+  else
+    TRY((gkr_device_run<Fp3, Fp3>(h, num_rounds, &tail_used)));
+  uint64_t* hp = h->h_proof;
+  uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
+  uint64_t* hf = hc + kMaxRounds * 3;
+  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * 2 * E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
+  if (tail_used) HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  TRY(prof_collect(h));
+  std::copy(hp, hp + num_rounds * 2 * E, round_polys);
+  std::copy(hc, hc + num_rounds * E, challenges);
+  if (t->kind == 0) {
+    for (int i = 0; i < 25; i++) t->fs.sp.st[i] = hs->st[i];
+    t->fs.sp.pos = hs->pos;
+  }
+  t->rng = hs->rng;
+  if (tail_used) {
+    std::copy(hf, hf + E, final_value);
+  } else {
+    TRY(final_value_impl(h, final_value));
+  }
   return EFFSC_HIP_OK;
 }
 
@@ -2233,6 +2506,8 @@ int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcr
     return prove_device(h, num_rounds, t, round_polys, challenges, final_value);
   if (!hook && eq_device_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
     return prove_device_eq(h, num_rounds, t, round_polys, challenges, final_value);
+  if (!hook && gkr_device_applies(h, num_rounds) && !std::getenv("EFFSC_HIP_HOST_LOOP"))
+    return prove_device_gkr(h, num_rounds, t, round_polys, challenges, final_value);
   const int E = h->ext;
   const size_t nw = (size_t)std::max(h->deg, 1);
   uint64_t prev[3];

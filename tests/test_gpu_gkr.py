@@ -99,3 +99,34 @@ def test_gkr_length_errors():
         GkrProver(O.uniform(15, 1), O.uniform(16, 2), O.uniform(4, 3))
     with pytest.raises(ValueError):
         GkrProver(O.uniform(16, 1), O.uniform(9, 2), O.uniform(4, 3))
+
+
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
+@pytest.mark.parametrize("k", [6, 7, 8])
+def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
+    # hookless proofs of power-of-two layers run on the device (round
+    # launches, then one workgroup); partial runs end on large tables
+    n = 1 << k
+    seed = 50 * k + ext + input_ext
+    add, mult, wit = (O.uniform(n * n * input_ext, seed), O.uniform(n * n * input_ext, seed + 1),
+                      O.uniform(n * input_ext, seed + 2))
+    lifted = [lift(x, ext // input_ext) for x in (add, mult, wit)]
+    for rounds in sorted({1, 3, 2 * k - 1, 2 * k}):
+        for tr in ("shake", "rng"):
+            say(f"gkr device loop k={k} ext={ext}/{input_ext} rounds={rounds} tr={tr}")
+            mk = (lambda: Transcript(b"gkr-dev", ext=ext)) if tr == "shake" else (lambda: TestTranscript(21, ext=ext))
+            g = GkrProver(add, mult, wit, ext=ext, input_ext=input_ext)
+            pg = sumcheck(g, rounds, mk())
+            o = O.Prover.gkr(*lifted, ext=ext)
+            po = o.sumcheck(rounds, O.Transcript.shake(b"gkr-dev") if tr == "shake" else O.Transcript.rng(21))
+            assert tuple(as_oracle(pg, ext)) == tuple(po)
+            for i in range(4):
+                assert np.array_equal(g.table_words(i), o.table(i)), f"table {i}"
+            # the per-round path gives the same proof
+            monkeypatch.setenv("EFFSC_HIP_HOST_LOOP", "1")
+            h = GkrProver(add, mult, wit, ext=ext, input_ext=input_ext)
+            ph = sumcheck(h, rounds, mk())
+            monkeypatch.delenv("EFFSC_HIP_HOST_LOOP")
+            assert ph.round_polys == pg.round_polys and ph.final_value == pg.final_value
+            if rounds == 2 * k:
+                assert g.claimed_w_values() == h.claimed_w_values()
