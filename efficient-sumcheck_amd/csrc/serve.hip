@@ -4,14 +4,21 @@
 // F_p^3, each with its own transcript) arrive at any time.  A scheduler thread
 // admits them, in priority order, while a slot and room in a fixed HBM arena
 // are free: a request's footprint is its input tables (unless borrowed), its
-// work tables and its proof words.  Every admitted request then advances one
-// round per tick, whatever its size and whenever it arrived:
+// work tables and its proof words.  Every admitted request then advances at
+// every tick, whatever its size and whenever it arrived: ML and IP requests
+// two rounds (the two-round passes of pass2.hpp), F_p^3 requests one:
 //
-//   k_serve_round_base  one workgroup per chunk of base-field round work (ML
-//                       and IP requests: round 0 evaluates, round j >= 1 folds
-//                       with the request's challenge and evaluates)
-//   k_serve_round_ext   the same for product-of-3 requests over F_p^3
-//   k_serve_fs          one workgroup per request: sums its chunks' partials,
+//   k_serve_pass        one workgroup per chunk of a base request's two-round
+//                       pass (step 0 reads the input, later steps fold twice
+//                       with the request's two challenges and evaluate)
+//   k_serve_fs2         one workgroup per base request: sums its partials and
+//                       runs two transcript rounds, or at the final step folds
+//                       what is left, runs the odd last round and writes the
+//                       final value
+//   k_serve_round_ext   one workgroup per chunk of a product-of-3 request's
+//                       round over F_p^3 (round 0 evaluates the base inputs,
+//                       round j >= 1 folds with the challenge and evaluates)
+//   k_serve_fs          one workgroup per F_p^3 request: sums its partials,
 //                       writes the round polynomial, runs the Fiat-Shamir step
 //                       on the request's device transcript, and after the last
 //                       round folds the tables once more into the final value
@@ -52,6 +59,7 @@ constexpr int kPartWords = 9;              // K * ext <= 9 words per partial
 constexpr uint64_t kChunkMin = 2048;       // units per workgroup, at least
 constexpr uint64_t kBlocksTarget = 32768;  // workgroups per tick, about
 constexpr uint32_t kFinalOnly = 0xFFFFFFFFu;
+constexpr uint32_t kFinalStep = 0x80000000u;  // base requests: the final step flag on FItem::j
 
 // A request's device record.
 struct SInst {
@@ -205,15 +213,153 @@ __device__ __forceinline__ uint32_t find_item(const RItem* items, uint32_t nitem
   return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_serve_round_base(const SInst* insts, const RItem* items, uint32_t nitems,
-                                                             uint64_t block0, uint64_t* partials) {
+// Base-field requests (ML, IP): two rounds per tick with the two-round
+// passes of pass2.hpp.  Step s < v/2 is a pass over the level-2^(v-2s) table
+// (units q = 2^(v-2s-2)): step 0 reads the input, step 1 folds the input
+// twice with the step-0 challenges in cur and stores level 2^(v-2), later
+// steps fold the stored level in place; k_serve_fs2 sums the partials and
+// runs two transcript rounds.  The final step (s = v/2) folds what is left,
+// runs the last round when v is odd, and writes the final value.
+template <int K>
+__device__ __forceinline__ void serve_pass_block(const SInst* ip, const RItem& it, uint64_t b, uint64_t* partials) {
+  const uint32_t s = it.j, v = ip->v;
+  const size_t L = (size_t)1 << (v - 2 * s), q = L / 4;
+  const uint64_t c1 = s ? ip->cur[0] : 0, c2 = s ? ip->cur[1] : 0;
+  const uint64_t* src[2] = {s <= 1 ? ip->in[0] : ip->work[0], s <= 1 ? ip->in[1] : ip->work[1]};
+  const uint64_t begin = b * it.per, end = std::min<uint64_t>(begin + it.per, it.units);
+  PassAcc<K> acc;
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
+  if (q >= 2) {
+    // pairs of units with 16-byte loads (q, per and the planes are even)
+    for (uint64_t u = begin + 2 * threadIdx.x; u < end; u += 2 * blockDim.x) {
+      uint64_t E0[K][4], E1[K][4];
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (s == 0)
+            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+          else
+            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+        }
+      if (s > 0) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            __builtin_nontemporal_store(v2u64{E0[i][j], E1[i][j]}, reinterpret_cast<v2u64*>(ip->work[i] + u + j * q));
+      }
+      pass_accumulate<K>(E0, acc);
+      pass_accumulate<K>(E1, acc);
+    }
+  } else {
+    for (uint64_t u = begin + threadIdx.x; u < end; u += blockDim.x) {
+      uint64_t E[K][4];
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2) : fold_chain<2>(src[i], u + j * q, L, c1, c2);
+      if (s > 0) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) ip->work[i][u + j * q] = E[i][j];
+      }
+      pass_accumulate<K>(E, acc);
+    }
+  }
+  constexpr int NV = pass_values<K>();
+  Fp raw[pass_raw<K>()], vals[NV];
+#pragma unroll
+  for (int i = 0; i < pass_raw<K>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
+  block_reduce<Fp, NV>(vals);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; i++) partials[(it.chunk0 + b) * kPartWords + i] = vals[i].v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_serve_pass(const SInst* insts, const RItem* items, uint32_t nitems,
+                                                       uint64_t block0, uint64_t* partials) {
   const uint64_t gb = block0 + blockIdx.x;
   const RItem it = items[find_item(items, nitems, gb)];
   const SInst* ip = insts + it.slot;
   if (ip->K == 1)
-    serve_block<1, Fp>(ip, it, gb - it.chunk0, partials);
+    serve_pass_block<1>(ip, it, gb - it.chunk0, partials);
   else
-    serve_block<2, Fp>(ip, it, gb - it.chunk0, partials);
+    serve_pass_block<2>(ip, it, gb - it.chunk0, partials);
+}
+
+// The final step s of a base request: the stored level (the input when
+// s <= 1, else the work table) with the step's two pending folds, at most 8
+// elements per table, is folded to 1 or 2; the odd last round, the last fold
+// and the product follow.
+template <int K>
+__device__ __forceinline__ void serve_final2(SInst* ip, uint32_t s) {
+  __shared__ uint64_t chal_sh[3];
+  const uint32_t v = ip->v;
+  if (v == 0) {
+    if (threadIdx.x == 0) ip->fin[0] = K == 2 ? gl::mul(ip->in[0][0], ip->in[1][0]) : ip->in[0][0];
+    return;
+  }
+  const size_t M = s == 0 ? ((size_t)1 << v) : ((size_t)1 << (v - 2 * s + 2));
+  const uint64_t* src[2] = {s <= 1 ? ip->in[0] : ip->work[0], s <= 1 ? ip->in[1] : ip->work[1]};
+  const uint64_t x = s ? ip->cur[0] : 0, y = s ? ip->cur[1] : 0;
+  const size_t n2 = s ? M / 4 : M;  // 1 or 2 elements per table after the pending folds
+  uint64_t t[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+  for (int i = 0; i < K; i++)
+    for (size_t e = 0; e < n2; e++) t[i][e] = s == 0 ? src[i][e] : fold_chain<2>(src[i], e, M / 4, x, y);
+  uint64_t f[2] = {t[0][0], t[1][0]};
+  if (n2 == 2) {
+    // round v - 1 over the pair (t[.][0], t[.][1]): ML [h(0)], IP [h(0), h(inf)]
+    Fp vals[K];
+    if constexpr (K == 1) {
+      vals[0] = {t[0][0]};
+    } else {
+      vals[0] = {gl::mul(t[0][0], t[1][0])};
+      vals[1] = {gl::mul(gl::sub(t[0][1], t[0][0]), gl::sub(t[1][1], t[1][0]))};
+    }
+    if (threadIdx.x < 64) emit_round<Fp, K>(vals, 1, &ip->fs, ip->polys + (size_t)(v - 1) * K, ip->chals + (v - 1), chal_sh);
+    __syncthreads();
+    const uint64_t z = chal_sh[0];
+#pragma unroll
+    for (int i = 0; i < K; i++) f[i] = gl::add(t[i][0], gl::mul(z, gl::sub(t[i][1], t[i][0])));
+  }
+  if (threadIdx.x == 0) ip->fin[0] = K == 2 ? gl::mul(f[0], f[1]) : f[0];
+}
+
+template <int K>
+__device__ __forceinline__ void serve_fs2(SInst* ip, const FItem& it, const uint64_t* partials) {
+  const uint32_t s = it.j & ~kFinalStep;
+  if (it.j & kFinalStep) {
+    serve_final2<K>(ip, s);
+    return;
+  }
+  constexpr int NV = pass_values<K>();
+  // the sponge load overlaps the partial sums
+  LaneFs lf;
+  if (threadIdx.x < 64) lf.load(&ip->fs);
+  Fp sum[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) sum[i] = gl::zero<Fp>();
+  for (uint64_t b = threadIdx.x; b < it.nparts; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; i++) sum[i] = gl::add(sum[i], Fp{partials[(it.part0 + b) * kPartWords + i]});
+  block_reduce<Fp, NV>(sum);
+  if (threadIdx.x < 64)
+    emit_two_rounds<K>(sum, lf, &ip->fs, ip->polys + (size_t)2 * s * K, ip->chals + 2 * s, ip->cur);
+}
+
+__global__ __launch_bounds__(kBlock) void k_serve_fs2(SInst* insts, const FItem* items, const uint64_t* partials) {
+  const FItem it = items[blockIdx.x];
+  SInst* ip = insts + it.slot;
+  if (ip->K == 1)
+    serve_fs2<1>(ip, it, partials);
+  else
+    serve_fs2<2>(ip, it, partials);
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_serve_round_ext(
@@ -268,13 +414,7 @@ __device__ __forceinline__ void serve_fs(SInst* ip, const FItem& it, const uint6
 __global__ __launch_bounds__(kBlock) void k_serve_fs(SInst* insts, const FItem* items, const uint64_t* partials) {
   const FItem it = items[blockIdx.x];
   SInst* ip = insts + it.slot;
-  const uint32_t K = ip->K;
-  if (K == 1)
-    serve_fs<1, Fp>(ip, it, partials);
-  else if (K == 2)
-    serve_fs<2, Fp>(ip, it, partials);
-  else
-    serve_fs<3, Fp3>(ip, it, partials);
+  serve_fs<3, Fp3>(ip, it, partials);  // F_p^3 requests; base requests use k_serve_fs2
 }
 
 __global__ __launch_bounds__(kBlock) void k_serve_fill(uint64_t* dst, uint64_t n, uint64_t key) {
@@ -342,7 +482,10 @@ Layout layout(int K, int ext, uint32_t v, const uint64_t* const* tables) {
     L.in_off[i] = o;
     if (!tables[i]) o += align256(n * 8);
   }
-  L.ws = (std::max<size_t>(n / 2, 1) + 31) & ~(size_t)31;
+  // base-field requests (ML, IP) run two rounds per tick and store every
+  // other level, from n/4 down; F_p^3 requests store each level from n/2
+  const bool two_round = K <= 2 && ext == 1;
+  L.ws = (std::max<size_t>(two_round ? n / 4 : n / 2, 2) + 31) & ~(size_t)31;
   L.work_off = o;
   o += align256((size_t)K * ext * L.ws * 8);
   L.proof_off = o;
@@ -498,16 +641,41 @@ int tick(effsc_hip_server* s, const std::vector<std::shared_ptr<Req>>& admitted)
   }
   // round and transcript items
   std::vector<RItem> base_items, ext_items;
-  std::vector<FItem> fs_items;
+  std::vector<FItem> fs_items, fs2_items;
+  // base requests: step s = j / 2 is a two-round pass while s < v / 2, then
+  // the final step; F_p^3 requests: one round per tick
+  auto base_units = [](const Req& r) -> uint64_t {
+    const uint32_t st = r.j / 2;
+    return st < r.v / 2 ? (uint64_t)1 << (r.v - 2 * st - 2) : 0;
+  };
   uint64_t total_units = 0;
-  for (auto& r : s->active)
-    if (r->v > 0) total_units += r->j == 0 ? (uint64_t)1 << (r->v - 1) : (uint64_t)1 << (r->v - r->j - 1);
-  const uint64_t per = std::max<uint64_t>(kChunkMin, (total_units + kBlocksTarget - 1) / kBlocksTarget);
+  for (auto& r : s->active) {
+    if (r->K != 3)
+      total_units += base_units(*r);
+    else if (r->v > 0)
+      total_units += r->j == 0 ? (uint64_t)1 << (r->v - 1) : (uint64_t)1 << (r->v - r->j - 1);
+  }
+  // even: a pass chunk takes pairs of units
+  uint64_t per = std::max<uint64_t>(kChunkMin, (total_units + kBlocksTarget - 1) / kBlocksTarget);
+  per += per & 1;
   uint64_t gb = 0;
   for (int pass = 0; pass < 2; pass++) {
     for (auto& r : s->active) {
       const bool ext_class = r->K == 3;
       if (ext_class != (pass == 1)) continue;
+      if (!ext_class) {
+        const uint32_t st = r->j / 2;
+        const uint64_t units = base_units(*r);
+        if (units == 0) {
+          fs2_items.push_back({(uint32_t)r->slot, st | kFinalStep, 0, 0});
+          continue;
+        }
+        const uint64_t nb = (units + per - 1) / per;
+        base_items.push_back({(uint32_t)r->slot, st, units, gb, per});
+        fs2_items.push_back({(uint32_t)r->slot, st, gb, nb});
+        gb += nb;
+        continue;
+      }
       if (r->v == 0) {
         fs_items.push_back({(uint32_t)r->slot, kFinalOnly, 0, 0});
         continue;
@@ -536,6 +704,7 @@ int tick(effsc_hip_server* s, const std::vector<std::shared_ptr<Req>>& admitted)
   size_t base_off = put(base_items.data(), base_items.size() * sizeof(RItem)) - items_start;
   size_t ext_off = put(ext_items.data(), ext_items.size() * sizeof(RItem)) - items_start;
   size_t fs_off = put(fs_items.data(), fs_items.size() * sizeof(FItem)) - items_start;
+  size_t fs2_off = put(fs2_items.data(), fs2_items.size() * sizeof(FItem)) - items_start;
   if (off > s->stage_bytes || off - items_start > s->items_bytes) return record_fatal(s, "serve: staging overflow");
   for (size_t i = 0; i < admitted.size(); i++)
     SCHK(hipMemcpyAsync(s->d_insts + admitted[i]->slot, stage + rec_off + i * sizeof(SInst), sizeof(SInst),
@@ -557,7 +726,7 @@ int tick(effsc_hip_server* s, const std::vector<std::shared_ptr<Req>>& admitted)
     }
   }
   if (!base_items.empty()) {
-    hipLaunchKernelGGL(k_serve_round_base, dim3((unsigned)nbase), dim3(kBlock), 0, s->stream, s->d_insts,
+    hipLaunchKernelGGL(k_serve_pass, dim3((unsigned)nbase), dim3(kBlock), 0, s->stream, s->d_insts,
                        reinterpret_cast<const RItem*>(s->d_items + base_off), (uint32_t)base_items.size(),
                        (uint64_t)0, s->d_partials);
     SCHK(hipGetLastError());
@@ -573,14 +742,28 @@ int tick(effsc_hip_server* s, const std::vector<std::shared_ptr<Req>>& admitted)
                        reinterpret_cast<const FItem*>(s->d_items + fs_off), s->d_partials);
     SCHK(hipGetLastError());
   }
+  if (!fs2_items.empty()) {
+    hipLaunchKernelGGL(k_serve_fs2, dim3((unsigned)fs2_items.size()), dim3(kBlock), 0, s->stream, s->d_insts,
+                       reinterpret_cast<const FItem*>(s->d_items + fs2_off), s->d_partials);
+    SCHK(hipGetLastError());
+  }
   s->stat_ticks++;
   // advance; finished requests copy their proof words back
   std::vector<std::shared_ptr<Req>> still;
   for (auto& r : s->active) {
-    r->j++;
-    if (r->v > 0 && r->j < r->v) {
-      still.push_back(r);
-      continue;
+    if (r->K != 3) {
+      // a base request retires after its final step
+      if (r->j / 2 < r->v / 2) {
+        r->j += 2;
+        still.push_back(r);
+        continue;
+      }
+    } else {
+      r->j++;
+      if (r->v > 0 && r->j < r->v) {
+        still.push_back(r);
+        continue;
+      }
     }
     const size_t words = (size_t)r->v * r->K * r->ext + (size_t)r->v * r->ext + r->ext;
     SCHK(hipMemcpyAsync(s->h_res + (size_t)r->slot * s->res_words, s->arena_base + r->arena_off + r->lay.proof_off,
@@ -743,6 +926,9 @@ int effsc_hip_server_submit(effsc_hip_server* s, const effsc_hip_request* req, u
   for (int i = 0; i < 3; i++) {
     r->seeds[i] = req->seeds[i];
     r->tables[i] = i < K ? req->tables[i] : nullptr;
+    // the base-field passes load pairs of words
+    if (r->tables[i] && reinterpret_cast<uintptr_t>(r->tables[i]) % 16)
+      return set_error(EFFSC_HIP_EINVAL, "borrowed request table not 16-byte aligned");
   }
   r->lay = layout(K, ext, r->v, r->tables);
   if (align256(r->lay.bytes) > s->arena.cap)

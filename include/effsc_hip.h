@@ -327,8 +327,9 @@ typedef struct {
   int kind;                         /* EFFSC_HIP_SERVE_* */
   unsigned v;                       /* variables: tables of 2^v elements */
   uint64_t seeds[3];                /* table i = uniform(seeds[i]) made on the device when tables[i] is NULL */
-  const uint64_t* tables[3];        /* or caller-owned canonical device tables of 2^v words (not counted
-                                       in the footprint, kept alive until the request completes) */
+  const uint64_t* tables[3];        /* or caller-owned canonical device tables of 2^v words, 16-byte
+                                       aligned (not counted in the footprint, kept alive until the
+                                       request completes) */
   const effsc_hip_transcript* transcript; /* its state is copied at submit */
   double priority;                  /* admission order: lower first; ties in submission order */
 } effsc_hip_request;
