@@ -1041,6 +1041,77 @@ __global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs(Reduce r, unsigne
   for (size_t k = threadIdx.x; k < fold_half; k += blockDim.x) ft.st(k, fold1<F>(ft.ld(k), ft.ld(k + fold_half), c, w));
 }
 
+// EqFactored over the base field in two-round passes: the proof is the inner
+// product of p with the eq table EQ(x) = eq_l[x >> m] eq_r[x & (2^m - 1)]
+// (m = log2 of the current eq_r length; eq_l has length 1 once its
+// variables are bound), so the two-round pass of pass2.hpp applies with the
+// second table computed from the two small eq tables instead of read.
+// k_eq_pass<P> folds p with P pending challenges on the fly (writing the
+// level when P > 0) and accumulates the pass values against EQ;
+// k_eq_reduce_fs2 runs the two transcript rounds and folds the eq tables of
+// both variables (the current eq tables are EQ at the next level).
+template <int P, bool NT>
+__global__ __launch_bounds__(kBlock) void k_eq_pass(const uint64_t* src, uint64_t* dst, const uint64_t* cptr, size_t q,
+                                                    const uint64_t* eql, const uint64_t* eqr, unsigned m_log,
+                                                    Reduce r) {
+  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0;
+  const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
+  PassAcc<2> acc;
+#pragma unroll
+  for (int i = 0; i < pass_raw<2>(); i++) acc.a[i].zero();
+  const size_t step = (size_t)gridDim.x * blockDim.x * 2;
+  for (size_t u = gtid() * 2; u < q; u += step) {
+    uint64_t E0[2][4], E1[2][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, E0[0][j], E1[0][j]);
+      const size_t x = u + j * q;
+      E0[1][j] = gl::mul(eql[x >> m_log], eqr[x & mask]);
+      E1[1][j] = gl::mul(eql[(x + 1) >> m_log], eqr[(x + 1) & mask]);
+    }
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if constexpr (NT)
+          __builtin_nontemporal_store(v2u64{E0[0][j], E1[0][j]}, reinterpret_cast<v2u64*>(dst + u + j * q));
+        else
+          *reinterpret_cast<ulonglong2*>(dst + u + j * q) = make_ulonglong2(E0[0][j], E1[0][j]);
+      }
+    }
+    pass_accumulate<2>(E0, acc);
+    pass_accumulate<2>(E1, acc);
+  }
+  Fp raw[pass_raw<2>()], vals[pass_values<2>()];
+#pragma unroll
+  for (int i = 0; i < pass_raw<2>(); i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<2>(raw, vals);
+  finish<Fp, pass_values<2>()>(vals, r);
+}
+
+// Two transcript rounds from the pass values, then the MSB folds of the eq
+// table of round s's variable (t0, half h0) with x and of round s+1's (t1,
+// half h1) with y; t0 and t1 may be the same table (h1 = h0 / 2 then).
+__global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
+                                                                uint64_t* chals, uint64_t* cur, uint64_t* t0,
+                                                                size_t h0, uint64_t* t1, size_t h1) {
+  constexpr int NV = pass_values<2>();
+  __shared__ uint64_t xy[3];
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
+  Fp sv[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) sv[i] = gl::zero<Fp>();
+  sum_partials<Fp, NV>(r.partials, nblocks, sv);
+  block_reduce<Fp, NV>(sv);
+  if (threadIdx.x < 64) emit_two_rounds<2>(sv, fs, t, poly, chals, xy);
+  __syncthreads();
+  const uint64_t x = xy[0], y = xy[1];
+  if (threadIdx.x < 2) cur[threadIdx.x] = xy[threadIdx.x];
+  for (size_t k = threadIdx.x; k < h0; k += blockDim.x) t0[k] = gl::add(t0[k], gl::mul(x, gl::sub(t0[k + h0], t0[k])));
+  __syncthreads();
+  for (size_t k = threadIdx.x; k < h1; k += blockDim.x) t1[k] = gl::add(t1[k], gl::mul(y, gl::sub(t1[k + h1], t1[k])));
+}
+
 template <class F>
 constexpr int eq_tail_block() {
   return F::D == 3 ? 256 : 1024;
@@ -1743,6 +1814,70 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
   size_t L = h->len, j = 0;
   bool have = false;
   View<F> eql{h->eq_l, h->eq_l_stride}, eqr{h->eq_r, h->eq_r_stride};
+  const uint64_t* cur_in = h->d_cur;  // the pending challenge when entering the tail
+  if constexpr (std::is_same<FI, Fp>::value && std::is_same<F, Fp>::value) {
+    if (!std::getenv("EFFSC_HIP_ONE_ROUND")) {
+      // two rounds per pass while the level evaluated is above the tail
+      int pend = 0;
+      while (j + 2 <= v) {
+        const size_t Lv = L >> pend;
+        if (Lv <= tail || Lv < 8) break;
+        size_t m_log = 0;
+        while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+        const size_t q = Lv / 4;
+        const unsigned g = (unsigned)std::max<size_t>(
+            1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, (size_t)h->grid_max * kPassGridScale));
+        const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
+        const bool nt = 8.0 * (double)L >= kNtMinBytes;
+        TRY(prof_begin(h));
+        if (pend == 0)
+          hipLaunchKernelGGL((k_eq_pass<0, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur, q,
+                             h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        else if (nt)
+          hipLaunchKernelGGL((k_eq_pass<2, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur, q,
+                             h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        else
+          hipLaunchKernelGGL((k_eq_pass<2, false>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur,
+                             q, h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        HIPCHK(hipGetLastError());
+        TRY(prof_end(h, 3, 8.0 * ((double)L + (pend ? (double)Lv : 0.0)), (double)q * (8.0 + 8.0 + (pend ? 12.0 : 0.0))));
+        if (pend) {
+          L = Lv;
+          h->len = Lv;
+          h->in_work = true;
+        }
+        // the eq tables of variables j and j + 1, folded after the two rounds
+        size_t* l0 = j < v_l ? &h->eq_l_len : &h->eq_r_len;
+        uint64_t* t0 = j < v_l ? h->eq_l : h->eq_r;
+        const size_t h0 = *l0 >> 1;
+        *l0 = h0;
+        size_t* l1 = j + 1 < v_l ? &h->eq_l_len : &h->eq_r_len;
+        uint64_t* t1 = j + 1 < v_l ? h->eq_l : h->eq_r;
+        const size_t h1 = *l1 >> 1;
+        *l1 = h1;
+        hipLaunchKernelGGL(k_eq_reduce_fs2, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
+                           h->d_polys + j * 2, h->d_chals + j, h->d_cur, t0, h0, t1, h1);
+        HIPCHK(hipGetLastError());
+        j += 2;
+        pend = 2;
+      }
+      if (pend == 2) {
+        // the older pending challenge folds p with a multi-block fold; the
+        // newer one stays pending for the rounds below
+        Views<Fp, 1> a{}, b{};
+        a.t[0] = View<Fp>{h->in_work ? h->work[0] : h->in[0], 0};
+        b.t[0] = View<Fp>{h->work[0], 0};
+        hipLaunchKernelGGL((k_fold<Fp, Fp, 1>), dim3(blocks_for(h, L / 2)), dim3(kBlock), 0, h->stream, a, b,
+                           gl::zero<Fp>(), h->d_cur, L / 2, L / 2, h->w);
+        HIPCHK(hipGetLastError());
+        L /= 2;
+        h->len = L;
+        h->in_work = true;
+        have = true;
+        cur_in = h->d_cur + 1;
+      }
+    }
+  }
   while (j < v && L > tail && L >= 4) {
     const int left = j < v_l;
     size_t m_log = 0;
@@ -1753,11 +1888,11 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     const View<F> pw{h->work[0], h->work_stride};
     TRY(prof_begin(h));
     if (have && h->in_work) {
-      hipLaunchKernelGGL((k_eq_round<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, h->d_cur, eql,
+      hipLaunchKernelGGL((k_eq_round<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, cur_in, eql,
                          h->eq_l_len, eqr, m_log, left, h->w, h->red);
     } else {
       hipLaunchKernelGGL((k_eq_round<FI, F>), dim3(g), dim3(kBlock), 0, h->stream, pin, pw, L, have ? 1 : 0,
-                         h->d_cur, eql, h->eq_l_len, eqr, m_log, left, h->w, h->red);
+                         cur_in, eql, h->eq_l_len, eqr, m_log, left, h->w, h->red);
     }
     HIPCHK(hipGetLastError());
     {
@@ -1790,6 +1925,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     }
     *tl = half;
     have = true;
+    cur_in = h->d_cur;  // k_eq_reduce_fs wrote this round's challenge there
     j++;
   }
   // remaining rounds, finalize and final value
@@ -1797,7 +1933,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
   const View<F> pw{h->work[0], h->work_stride};
   hipLaunchKernelGGL((k_eq_tail<FI, F>), dim3(1), dim3(eq_tail_block<F>()), 0, h->stream, pin, pw,
                      h->in_work ? 0 : 1, L, (int)j, (int)v, (int)v_l, eql, h->eq_l_len, eqr, h->eq_r_len,
-                     have ? 1 : 0, h->d_cur, h->d_fs, E, h->d_polys + j * 2 * E, h->d_chals + j * E, h->d_cur,
+                     have ? 1 : 0, cur_in, h->d_fs, E, h->d_polys + j * 2 * E, h->d_chals + j * E, h->d_cur,
                      h->d_fin, h->w);
   HIPCHK(hipGetLastError());
   // post-state: every variable bound

@@ -132,11 +132,18 @@ def test_eq_factored_parity(ext, v):
     check_proof("eq", ext, ext, 1 << v, v, 900 + v)
 
 
-@pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 17), (1, 20), (3, 11), (3, 12), (3, 15)])
+@pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 20), (1, 23), (3, 11), (3, 12), (3, 15)])
 def test_eq_factored_device_loop(ext, v, monkeypatch):
     # large p: multi-workgroup rounds with in-reduction eq folds, then the
     # one-workgroup tail; against the oracle and the per-round host loop
+    # base field: two rounds per pass against EQ = eq_l x eq_r computed on
+    # the fly (v = 23 has a pass across the left/right split, v_L = 11);
+    # EFFSC_HIP_ONE_ROUND keeps one round per launch
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    if ext == 1:
+        monkeypatch.setenv("EFFSC_HIP_ONE_ROUND", "1")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+        monkeypatch.delenv("EFFSC_HIP_ONE_ROUND")
     monkeypatch.setenv("EFFSC_HIP_HOST_LOOP", "1")
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.delenv("EFFSC_HIP_HOST_LOOP")
