@@ -301,6 +301,23 @@ def bench_single(args, workload):
                 "kernel": "whole proof (algorithmic bytes / proof time; latency-bound at this size)"}
     roof["per_class"] = {names[i]: {"launches": int(launches[i]), "ms": ms[i], "gbs": gbs[i], "gmul_s": gmuls[i]}
                          for i in range(4) if launches[i]}
+    if roof["bound"] == "hbm":
+        # the measured side of the HBM roofline: kernels with no arithmetic
+        # moving the same bytes (a stream copy; for the two-round passes also
+        # their read-pass and fold-pass access patterns) at this size
+        pats = [("copy_gbs", 0)]
+        if spec["kind"] in ("ip", "ip_lsb", "ml"):
+            pats += [("read_pass_gbs", 1), ("fold_pass_gbs", 2)]
+        att = {}
+        for name, pat in pats:
+            r = C.c_double()
+            N.check(L.effsc_hip_hbm_rate(dev, pat, 1 << max(v, 10), C.byref(r)))
+            att[name] = r.value / 1e9
+        att["frac_of_copy"] = roof["achieved"] / att["copy_gbs"]
+        att["source"] = (f"effsc_hip_hbm_rate: no-arithmetic kernels, 2^{max(v, 10)} words per table, "
+                         "best of 5 launches")
+        roof["attainable"] = att
+        log("bench: attainable " + ", ".join(f"{k} {att[k]:.0f}" for k, _ in pats))
 
     cpu = None
     if not args.no_cpu:

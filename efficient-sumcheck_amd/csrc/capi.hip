@@ -2823,6 +2823,106 @@ extern "C" int effsc_hip_mul_rate(int device, double* muls_per_s) {
   return EFFSC_HIP_OK;
 }
 
+// Attainable HBM rates: kernels with no field arithmetic that move the bytes
+// of a stream copy, of the read pass (P = 0: two tables, four quarter
+// regions each) and of the fold pass (P = 2: sixteen sixteenth regions read
+// and four quarter regions of the level-n/4 table written per table), in the
+// order and with the load/store kinds of k_pass.  The read kernels keep
+// their xor live through a store that no input makes.
+__global__ __launch_bounds__(kBlock) void k_bw_copy(const v2u64* s, v2u64* d, size_t n2) {
+  for (size_t i = gtid(); i < n2; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bw_read(const uint64_t* a, const uint64_t* b, size_t q, size_t chunk,
+                                                    uint64_t* sink) {
+  v2u64 acc = {0, 0};
+  const size_t uend = min(q, (blockIdx.x + 1) * chunk);
+  for (size_t u = blockIdx.x * chunk + threadIdx.x * 2; u < uend; u += (size_t)blockDim.x * 2) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      acc ^= __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(a + u + j * q));
+      acc ^= __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(b + u + j * q));
+    }
+  }
+  if (acc.x == 0x9E3779B97F4A7C15ull && acc.y == 0x243F6A8885A308D3ull) sink[0] = acc.x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bw_fold(const uint64_t* a, const uint64_t* b, uint64_t* oa, uint64_t* ob,
+                                                    size_t q) {
+  for (size_t u = gtid() * 2; u < q; u += (size_t)gridDim.x * blockDim.x * 2) {
+    v2u64 x[2][4] = {};
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        x[0][j] ^= __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(a + u + (j + 4 * m) * q));
+        x[1][j] ^= __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(b + u + (j + 4 * m) * q));
+      }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      __builtin_nontemporal_store(x[0][j], reinterpret_cast<v2u64*>(oa + u + j * q));
+      __builtin_nontemporal_store(x[1][j], reinterpret_cast<v2u64*>(ob + u + j * q));
+    }
+  }
+}
+
+extern "C" int effsc_hip_hbm_rate(int device, int pattern, size_t n, double* bytes_per_s) {
+  if (!bytes_per_s) return fail(EFFSC_HIP_EINVAL, "null out");
+  if (pattern < EFFSC_HIP_HBM_COPY || pattern > EFFSC_HIP_HBM_FOLD_PASS)
+    return fail(EFFSC_HIP_EINVAL, "unknown access pattern");
+  if (!is_pow2(n) || n < 1024) return fail(EFFSC_HIP_EINVAL, "n must be a power of two >= 1024");
+  HIPCHK(hipSetDevice(device));
+  const size_t gmax = (size_t)grid_max_for(device) * kPassGridScale;
+  // two tables of n words, their level-n/4 outputs, one sink word
+  uint64_t* buf = nullptr;
+  TRY(alloc_dev(&buf, 2 * n + n / 2 + 32));
+  uint64_t *a = buf, *b = buf + n, *oa = buf + 2 * n, *ob = oa + n / 4, *sink = ob + n / 4;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(buf, 0, (2 * n + n / 2 + 32) * 8, st);
+  double bytes = 0;
+  auto launch = [&] {
+    if (pattern == EFFSC_HIP_HBM_COPY) {
+      bytes = 16.0 * n;
+      hipLaunchKernelGGL(k_bw_copy, dim3((unsigned)std::min<size_t>(n / 2 / kBlock, gmax / 8)), dim3(kBlock), 0, st,
+                         reinterpret_cast<const v2u64*>(a), reinterpret_cast<v2u64*>(b), n / 2);
+    } else if (pattern == EFFSC_HIP_HBM_READ_PASS) {
+      bytes = 16.0 * n;
+      const size_t q = n / 4, step = (size_t)kBlock * 2;
+      const size_t g = std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax);
+      const size_t chunk = (q + step * g - 1) / (step * g) * step;
+      hipLaunchKernelGGL(k_bw_read, dim3((unsigned)((q + chunk - 1) / chunk)), dim3(kBlock), 0, st, a, b, q, chunk,
+                         sink);
+    } else {
+      bytes = 20.0 * n;
+      const size_t q = n / 16;
+      hipLaunchKernelGGL(k_bw_fold, dim3((unsigned)std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax)), dim3(kBlock),
+                         0, st, a, b, oa, ob, q);
+    }
+  };
+  float best = 0;
+  for (int rep = 0; rep < 6 && e == hipSuccess; rep++) {
+    e = hipEventRecord(e0, st);
+    if (e == hipSuccess) launch();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && rep > 0 && (best == 0 || ms < best)) best = ms;  // the first launch warms up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(EFFSC_HIP_EDEVICE, std::string("hbm rate: ") + hipGetErrorString(e));
+  *bytes_per_s = bytes / (best * 1e-3);
+  return EFFSC_HIP_OK;
+}
+
 extern "C" int effsc_hip_device_alloc(int device, size_t words, uint64_t** out) {
   if (!out) return fail(EFFSC_HIP_EINVAL, "null out");
   HIPCHK(hipSetDevice(device));

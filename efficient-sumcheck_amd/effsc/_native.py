@@ -95,6 +95,7 @@ _SIGS = {
     "effsc_hip_copy": (C.c_int, [C.c_int, u64p, u64p, sz]),
     "effsc_hip_fill_uniform": (C.c_int, [C.c_int, u64p, sz, u64, u64]),
     "effsc_hip_mul_rate": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "effsc_hip_hbm_rate": (C.c_int, [C.c_int, C.c_int, sz, C.POINTER(C.c_double)]),
     "effsc_hip_stream_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
     "effsc_hip_stream_num_variables": (C.c_int, [vp, szp]),
     "effsc_hip_stream_read": (C.c_int, [vp, sz, sz, u64p]),

@@ -386,6 +386,16 @@ int effsc_hip_fill_uniform(int device, uint64_t* dst, size_t n, uint64_t seed, u
  * second over independent chains filling every CU): the compute roofline of
  * the degree >= 3 rounds.  No reference counterpart. */
 int effsc_hip_mul_rate(int device, double* muls_per_s);
+/* Measured attainable HBM rate (bytes per second, best of five launches) of
+ * a kernel with no field arithmetic that moves the bytes of one access
+ * pattern over two tables of n words (n a power of two, >= 1024; the call
+ * allocates 2.5 n words): a stream copy of one table (16 n bytes), the read
+ * pass of the two-round passes (both tables read in four quarter regions,
+ * 16 n bytes), or the fold pass with two pending folds (both tables read in
+ * sixteen regions, level-n/4 tables written, 20 n bytes).  The HBM side of
+ * the roofline.  No reference counterpart. */
+enum { EFFSC_HIP_HBM_COPY = 0, EFFSC_HIP_HBM_READ_PASS = 1, EFFSC_HIP_HBM_FOLD_PASS = 2 };
+int effsc_hip_hbm_rate(int device, int pattern, size_t n, double* bytes_per_s);
 
 #ifdef __cplusplus
 }

@@ -452,3 +452,16 @@ def test_two_round_pass_equals_one_round_kernels(v, monkeypatch):
     two_ml = sumcheck(MultilinearProver(a), v, Transcript(b"2r"))
     assert one.round_polys == two.round_polys and one.final_value == two.final_value
     assert one_ml.round_polys == two_ml.round_polys and one_ml.final_value == two_ml.final_value
+
+
+def test_measurement_probes_report_rates_and_reject_bad_arguments():
+    L = N.lib()
+    r = C.c_double()
+    assert L.effsc_hip_mul_rate(0, C.byref(r)) == 0 and r.value > 1e9
+    for pat in (0, 1, 2):
+        r.value = 0
+        assert L.effsc_hip_hbm_rate(0, pat, 1 << 20, C.byref(r)) == 0, L.effsc_hip_last_error()
+        assert r.value > 1e9
+    assert L.effsc_hip_hbm_rate(0, 3, 1 << 20, C.byref(r)) == 1  # EINVAL: unknown pattern
+    assert L.effsc_hip_hbm_rate(0, 0, 3 << 10, C.byref(r)) == 1  # EINVAL: not a power of two
+    assert L.effsc_hip_hbm_rate(0, 0, 512, C.byref(r)) == 1  # EINVAL: below 1024
