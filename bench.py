@@ -275,8 +275,8 @@ def bench_single(args, workload):
     elif spec["kind"] == "product" and ext == 3:
         names[0] = "k_fused_seq (fold + evaluate, one F_p^3 table at a time)"
     elif spec["kind"] == "gkr":
-        names[0] = "k_gkr_fused (fold the four tables + evaluate q(0), q(1), q(2))"
-        names[1] = "k_gkr_eval (round 0: evaluate q(0), q(1), q(2))"
+        names[0] = "k_gkr_fused_fw (fold add and mult, factored witness, evaluate the wire)"
+        names[1] = "k_gkr_eval_fw (round 0: evaluate the wire, factored witness)"
     if profiled:
         dom = max(range(4), key=lambda i: ms[i])
         kernel = names[dom]

@@ -163,6 +163,7 @@ struct effsc_hip_prover {
   bool ooc = false;
   const effsc_hip_stream* ooc_src[kMaxK] = {};
   uint64_t* ooc_parts = nullptr;  // partials of the streamed fold pass
+  uint64_t* gkr_b = nullptr;  // GKR factored witness: two ping-pong planesets of n/2 elements (3 planes)
   size_t ooc_nparts = 0;
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
@@ -771,6 +772,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->d_pass0) (void)hipFree(h->d_pass0);
   if (h->ooc_parts) (void)hipFree(h->ooc_parts);
+  if (h->gkr_b) (void)hipFree(h->gkr_b);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
@@ -1592,6 +1594,148 @@ __global__ __launch_bounds__((tail_block<F, 4>())) void k_gkr_tail(Views<FI, 4> 
   }
 }
 
+// Factored witness (n = 2^k).  While the b variables are bound (j <= k folds
+// applied), the level-2^{2k-j} broadcast tables are w_b[x] = B_j[x >> k]
+// and w_c[x] = W[x & (n - 1)], with B_0 = W and B_{j+1} the MSB fold of B_j:
+// the pair (x, x + L/2) of w_b is (B_j[x >> k], B_j[(x >> k) + 2^{k-j-1}]),
+// and the pair of w_c is two equal elements, which fold to the same element
+// (lo + r (lo - lo) = lo).  So those rounds fold only add and mult through
+// HBM, half the traffic of the four tables, and read B_j (2^{k-j}
+// elements) and W (n elements) from cache.  The tables left behind are the
+// same field elements as the four-table rounds.
+//
+// In every round of a power-of-two layer one witness table is constant over
+// each pair (w_c while b is bound, w_b after), so with v that constant and u
+// the other, f(t) = a(t) (u(t) + v) + m(t) u(t) v has degree 2 and
+//   q(0)   = u0 (a0 + m0 v) + a0 v,
+//   q(inf) = (u1 - u0) ((a1 - a0) + v (m1 - m0)),
+// the leading coefficient, which equals the reference's
+// (q(0) + q(2) - 2 q(1)) / 2 (src/provers/gkr.rs:176-181) as a field element:
+// 5 products per pair instead of 9 for q(0), q(1), q(2).
+template <class F>
+__device__ __forceinline__ void gkr_pair(F a0, F a1, F m0, F m1, F u0, F u1, F v, uint64_t w, F& q0, F& qi) {
+  q0 = gl::add(gl::mul(u0, gl::add(a0, gl::mul(m0, v, w)), w), gl::mul(a0, v, w));
+  qi = gl::mul(gl::sub(u1, u0), gl::add(gl::sub(a1, a0), gl::mul(v, gl::sub(m1, m0), w)), w);
+}
+
+// k_gkr_eval_fw is round 0 over the input (w_c constant over the pairs);
+// k_gkr_fused_fw folds add and mult with the pending challenge, writes
+// B_{j+1} into the other of two planesets and evaluates the folded pairs,
+// with w_b folded on the fly.  Both leave the wire [q(0), q(inf)] per
+// workgroup.
+// Two elements per thread: the GKR prover owns its planes, whose strides are
+// multiples of 64 words, so (x, x + 1) with x even is one 16-byte load per
+// plane, and with k >= 1 both share x >> k and (x + L/2) >> k.
+template <class FI>
+__global__ __launch_bounds__(kBlock) void k_gkr_eval_fw(Views<FI, 2> t, View<FI> wit, unsigned k, size_t half,
+                                                        uint64_t w, Reduce r) {
+  FI acc[2] = {gl::zero<FI>(), gl::zero<FI>()};
+  const size_t mask = ((size_t)1 << k) - 1;
+  const size_t step = (size_t)gridDim.x * blockDim.x * 2;
+  for (size_t x = gtid() * 2; x < half; x += step) {
+    FI a0[2], a1[2], m0[2], m1[2], v[2];
+    t.t[0].ld2(x, a0[0], a0[1]);
+    t.t[0].ld2(x + half, a1[0], a1[1]);
+    t.t[1].ld2(x, m0[0], m0[1]);
+    t.t[1].ld2(x + half, m1[0], m1[1]);
+    wit.ld2(x & mask, v[0], v[1]);
+    const FI u0 = wit.ld(x >> k), u1 = wit.ld((x + half) >> k);
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      FI q0, qi;
+      gkr_pair(a0[e], a1[e], m0[e], m1[e], u0, u1, v[e], w, q0, qi);
+      acc[0] = gl::add(acc[0], q0);
+      acc[1] = gl::add(acc[1], qi);
+    }
+  }
+  finish<FI, 2>(acc, r);
+}
+
+// From level 4q with j folds applied to level 2q with j + 1 (j + 1 <= k):
+// bsrc holds B_j (2 hb elements), bdst receives B_{j+1} (hb elements).
+// cconst: the level-2q pairs bind a b variable (j + 1 < k), so w_c is the
+// constant of each pair; else w_b is (B_k has one element).
+template <class FI, class FW, class F>
+__global__ __launch_bounds__(kBlock) void k_gkr_fused_fw(Views<FI, 2> in, Views<F, 2> out, View<FI> bsrc,
+                                                         View<F> bdst, size_t hb, View<FW> wit, unsigned k, int cconst,
+                                                         const uint64_t* cptr, size_t q, uint64_t w, Reduce r) {
+  const F c = gl::load_words<F>(cptr);
+  const size_t mask = ((size_t)1 << k) - 1;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gtid(); i < hb; i += step) bdst.st(i, fold1<F>(bsrc.ld(i), bsrc.ld(i + hb), c, w));
+  F acc[2] = {gl::zero<F>(), gl::zero<F>()};
+  for (size_t x = gtid() * 2; x < q; x += step * 2) {
+    F lo[2][2], hi[2][2], cl[2], ch[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      FI x0[2], x1[2], y0[2], y1[2];
+      in.t[j].ld2(x, x0[0], x0[1]);
+      in.t[j].ld2(x + q, x1[0], x1[1]);
+      in.t[j].ld2(x + 2 * q, y0[0], y0[1]);
+      in.t[j].ld2(x + 3 * q, y1[0], y1[1]);
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        lo[j][e] = fold1<F>(x0[e], y0[e], c, w);
+        hi[j][e] = fold1<F>(x1[e], y1[e], c, w);
+      }
+      out.t[j].st2(x, lo[j][0], lo[j][1]);
+      out.t[j].st2(x + q, hi[j][0], hi[j][1]);
+    }
+    // positions x, x + 1 and x + q, x + q + 1 of the level-2q tables
+    const size_t bl = x >> k;
+    const F blo = fold1<F>(bsrc.ld(bl), bsrc.ld(bl + hb), c, w);
+    FW w0, w1;
+    wit.ld2(x & mask, w0, w1);
+    cl[0] = gl::lift_to<F>(w0);
+    cl[1] = gl::lift_to<F>(w1);
+    if (cconst) {
+      const size_t bh = (x + q) >> k;
+      const F bhi = fold1<F>(bsrc.ld(bh), bsrc.ld(bh + hb), c, w);
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        F q0, qi;
+        gkr_pair(lo[0][e], hi[0][e], lo[1][e], hi[1][e], blo, bhi, cl[e], w, q0, qi);
+        acc[0] = gl::add(acc[0], q0);
+        acc[1] = gl::add(acc[1], qi);
+      }
+    } else {
+      wit.ld2((x + q) & mask, w0, w1);
+      ch[0] = gl::lift_to<F>(w0);
+      ch[1] = gl::lift_to<F>(w1);
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        F q0, qi;
+        gkr_pair(lo[0][e], hi[0][e], lo[1][e], hi[1][e], cl[e], ch[e], blo, w, q0, qi);
+        acc[0] = gl::add(acc[0], q0);
+        acc[1] = gl::add(acc[1], qi);
+      }
+    }
+  }
+  finish<F, 2>(acc, r);
+}
+
+// Sums the per-workgroup wires and runs the transcript step.
+template <class F>
+__global__ __launch_bounds__(kReduceBlock) void k_gkr_wire_fs(Reduce r, unsigned nblocks, DevFs* t, int ext,
+                                                              uint64_t* poly, uint64_t* chal_out, uint64_t* chal_cur) {
+  F s[2] = {gl::zero<F>(), gl::zero<F>()};
+  sum_partials<F, 2>(r.partials, nblocks, s);
+  block_reduce<F, 2>(s);
+  if (threadIdx.x >= 64) return;
+  emit_round<F, 2>(s, ext, t, poly, chal_out, chal_cur);
+}
+
+// The level-M broadcast tables from the factored witness (j >= 1 folds).
+template <class FW, class F>
+__global__ __launch_bounds__(kBlock) void k_gkr_expand(View<F> b, View<FW> wit, unsigned k, size_t M, View<F> wb,
+                                                       View<F> wc) {
+  const size_t mask = ((size_t)1 << k) - 1;
+  for (size_t x = gtid(); x < M; x += (size_t)gridDim.x * blockDim.x) {
+    wb.st(x, b.ld(x >> k));
+    wc.st(x, gl::lift_to<F>(wit.ld(x & mask)));
+  }
+}
+
 int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   // src/provers/gkr.rs:93-101: fold all four tables with the challenge
   if (chal) {
@@ -1961,6 +2105,59 @@ int gkr_device_run(effsc_hip_prover* h, size_t num_rounds, bool* tail_used) {
   const size_t tail = F::D == 3 ? ((size_t)1 << 10) : ((size_t)1 << 12);
   size_t r = 0;
   bool have = false;
+  // factored-witness rounds from the input of a power-of-two layer, then the
+  // broadcast tables are written out at the level reached
+  unsigned kb = 0;
+  while (((size_t)1 << (2 * kb)) < h->n) kb++;
+  if (!h->in_work && h->len == h->n && ((size_t)1 << (2 * kb)) == h->n && kb >= 1 && h->len > tail &&
+      h->in_stride % 2 == 0 && h->work_stride % 2 == 0 && !std::getenv("EFFSC_HIP_GKR_BROADCAST")) {
+    const size_t hcap = ((size_t)1 << kb) / 2;
+    if (!h->gkr_b) TRY(alloc_dev(&h->gkr_b, 2 * hcap * 3));
+    auto bplanes = [&](size_t j) { return View<F>{h->gkr_b + (j & 1) * hcap * 3, hcap}; };
+    const View<FI> wit{h->in[3], h->in_stride};
+    const Views<FI, 2> in2 = views<FI, 2>(h->in, h->in_stride);
+    const Views<F, 2> w2 = views<F, 2>(h->work, h->work_stride);
+    size_t M = h->len;
+    unsigned g = blocks_for(h, M / 4);
+    TRY(prof_begin(h));
+    hipLaunchKernelGGL(k_gkr_eval_fw<FI>, dim3(g), dim3(kBlock), 0, h->stream, in2, wit, kb, M / 2, h->w, h->red);
+    HIPCHK(hipGetLastError());
+    TRY(prof_end(h, 1, 8.0 * 2 * (double)M * FI::D, 0.0));
+    hipLaunchKernelGGL(k_gkr_wire_fs<FI>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
+                       h->d_polys, h->d_chals, h->d_cur);
+    HIPCHK(hipGetLastError());
+    r = 1;
+    have = true;
+    size_t j = 0;  // folds applied to the tables
+    while (r < num_rounds && h->len > tail && h->len >= 4 && j + 1 <= kb) {
+      M = h->len;
+      g = blocks_for(h, M / 8);
+      const size_t hb = ((size_t)1 << kb) >> (j + 1);
+      TRY(prof_begin(h));
+      const int cconst = j + 1 < kb ? 1 : 0;
+      if (j == 0)
+        hipLaunchKernelGGL((k_gkr_fused_fw<FI, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, in2, w2, wit,
+                           bplanes(1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
+      else
+        hipLaunchKernelGGL((k_gkr_fused_fw<F, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, w2, w2, bplanes(j),
+                           bplanes(j + 1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
+      HIPCHK(hipGetLastError());
+      const int din = j == 0 ? FI::D : F::D;
+      TRY(prof_end(h, 0, 8.0 * 2 * ((double)M * din + (double)(M / 2) * F::D), 0.0));
+      h->len = M / 2;
+      j++;
+      hipLaunchKernelGGL(k_gkr_wire_fs<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
+                         h->d_polys + r * 2 * E, h->d_chals + r * E, h->d_cur);
+      HIPCHK(hipGetLastError());
+      r++;
+    }
+    if (j >= 1) {
+      hipLaunchKernelGGL((k_gkr_expand<FI, F>), dim3(blocks_for(h, h->len)), dim3(kBlock), 0, h->stream, bplanes(j),
+                         wit, kb, h->len, View<F>{h->work[2], h->work_stride}, View<F>{h->work[3], h->work_stride});
+      HIPCHK(hipGetLastError());
+      h->in_work = true;
+    }
+  }
   while (r < num_rounds && h->len > tail && h->len >= 4) {
     const size_t M = h->len;
     const Views<FI, 4> in = views<FI, 4>(h->in, h->in_stride);

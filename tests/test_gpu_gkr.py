@@ -102,16 +102,18 @@ def test_gkr_length_errors():
 
 
 @pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
-@pytest.mark.parametrize("k", [6, 7, 8])
+@pytest.mark.parametrize("k", [6, 7, 8, 9])
 def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
-    # hookless proofs of power-of-two layers run on the device (round
-    # launches, then one workgroup); partial runs end on large tables
+    # hookless proofs of power-of-two layers run on the device (factored
+    # witness rounds while the b variables are bound, four-table round
+    # launches, then one workgroup); partial runs end on large tables, inside
+    # the factored rounds (2, 3), at their end (k) and after it (k + 1)
     n = 1 << k
     seed = 50 * k + ext + input_ext
     add, mult, wit = (O.uniform(n * n * input_ext, seed), O.uniform(n * n * input_ext, seed + 1),
                       O.uniform(n * input_ext, seed + 2))
     lifted = [lift(x, ext // input_ext) for x in (add, mult, wit)]
-    for rounds in sorted({1, 3, 2 * k - 1, 2 * k}):
+    for rounds in sorted({1, 2, 3, k, k + 1, 2 * k - 1, 2 * k}):
         for tr in ("shake", "rng"):
             say(f"gkr device loop k={k} ext={ext}/{input_ext} rounds={rounds} tr={tr}")
             mk = (lambda: Transcript(b"gkr-dev", ext=ext)) if tr == "shake" else (lambda: TestTranscript(21, ext=ext))
@@ -130,3 +132,11 @@ def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
             assert ph.round_polys == pg.round_polys and ph.final_value == pg.final_value
             if rounds == 2 * k:
                 assert g.claimed_w_values() == h.claimed_w_values()
+            # and so does the device loop over the four broadcast tables
+            monkeypatch.setenv("EFFSC_HIP_GKR_BROADCAST", "1")
+            f = GkrProver(add, mult, wit, ext=ext, input_ext=input_ext)
+            pf = sumcheck(f, rounds, mk())
+            monkeypatch.delenv("EFFSC_HIP_GKR_BROADCAST")
+            assert pf.round_polys == pg.round_polys and pf.final_value == pg.final_value
+            for i in range(4):
+                assert np.array_equal(f.table_words(i), g.table_words(i)), f"table {i}"
