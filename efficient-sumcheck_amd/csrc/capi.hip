@@ -1090,6 +1090,72 @@ __global__ __launch_bounds__(kBlock) void k_eq_pass(const uint64_t* src, uint64_
   finish<Fp, pass_values<2>()>(vals, r);
 }
 
+// Split-value form of k_eq_pass, for q a multiple of 2^m (a left variable
+// still free): EQ(u + j q) = L_j R(u) with L_j = eql[(u >> m) + j (q >> m)]
+// and R(u) = eqr[u & (2^m - 1)], and the units of one workgroup (one chunk
+// inside a block of 2^m units) share the L_j.  With A = p, every raw sum of
+// pass_accumulate<2> is then an L-combination times a T-combination,
+// T_j = sum over the workgroup's units of A_j R:
+//   P1 = L0 T0, P2 = L1 T1, P3 = (L2 - L0)(T2 - T0), P4 = (L3 - L1)(T3 - T1),
+//   P5 = L2 T2, P6 = (L1 - L0)(T1 - T0), P7 = (L3 - L2)(T3 - T2),
+//   P8 = (L3 - L2 - L1 + L0)(T3 - T2 - T1 + T0),
+// so a unit costs 4 products (A_j R) instead of 4 EQ products and 8 pass
+// products; the 8 raw sums are formed once per workgroup.  The same field
+// elements as k_eq_pass.
+template <int P, bool NT>
+__global__ __launch_bounds__(kBlock) void k_eq_pass_sv(const uint64_t* src, uint64_t* dst, const uint64_t* cptr,
+                                                       size_t q, size_t chunk, const uint64_t* eql,
+                                                       const uint64_t* eqr, unsigned m_log, Reduce r) {
+  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0;
+  const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
+  Acc128 T[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) T[j].zero();
+  const size_t u0 = (size_t)blockIdx.x * chunk;
+  for (size_t u = u0 + threadIdx.x * 2; u < u0 + chunk; u += (size_t)blockDim.x * 2) {
+    uint64_t E0[4], E1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, E0[j], E1[j]);
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if constexpr (NT)
+          __builtin_nontemporal_store(v2u64{E0[j], E1[j]}, reinterpret_cast<v2u64*>(dst + u + j * q));
+        else
+          *reinterpret_cast<ulonglong2*>(dst + u + j * q) = make_ulonglong2(E0[j], E1[j]);
+      }
+    }
+    // u even: R(u) and R(u + 1) are one aligned pair of eqr
+    const v2u64 R = *reinterpret_cast<const v2u64*>(eqr + (u & mask));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      T[j].mac(E0[j], R.x);
+      T[j].mac(E1[j], R.y);
+    }
+  }
+  Fp t[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) t[j] = T[j].reduce();
+  block_reduce<Fp, 4>(t);
+  if (threadIdx.x == 0) {
+    const size_t b = u0 >> m_log, qs = q >> m_log;
+    uint64_t l[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) l[j] = eql[b + j * qs];
+    auto m = [](uint64_t x, uint64_t y) { return Fp{gl::mul(x, y)}; };
+    const uint64_t l20 = gl::sub(l[2], l[0]), l31 = gl::sub(l[3], l[1]), l10 = gl::sub(l[1], l[0]),
+                   l32 = gl::sub(l[3], l[2]);
+    const uint64_t t20 = gl::sub(t[2].v, t[0].v), t31 = gl::sub(t[3].v, t[1].v), t10 = gl::sub(t[1].v, t[0].v),
+                   t32 = gl::sub(t[3].v, t[2].v);
+    const Fp raw[8] = {m(l[0], t[0].v), m(l[1], t[1].v), m(l20, t20),         m(l31, t31),
+                       m(l[2], t[2].v), m(l10, t10),     m(l32, t32),         m(gl::sub(l31, l20), gl::sub(t31, t20))};
+    Fp vals[pass_values<2>()];
+    pass_values_from_raw<2>(raw, vals);
+#pragma unroll
+    for (int i = 0; i < pass_values<2>(); i++) r.partials[(size_t)blockIdx.x * pass_values<2>() + i] = vals[i].v;
+  }
+}
+
 // Two transcript rounds from the pass values, then the MSB folds of the eq
 // table of round s's variable (t0, half h0) with x and of round s+1's (t1,
 // half h1) with y; t0 and t1 may be the same table (h1 = h0 / 2 then).
@@ -1969,12 +2035,28 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         size_t m_log = 0;
         while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
         const size_t q = Lv / 4;
-        const unsigned g = (unsigned)std::max<size_t>(
-            1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, (size_t)h->grid_max * kPassGridScale));
+        const size_t gmax = (size_t)h->grid_max * kPassGridScale;
+        // split-value form while q is a multiple of 2^m: chunks of at least
+        // 512 units (one step of the workgroup), inside one block of 2^m
+        size_t chunk = 2 * kBlock;
+        while (q / chunk > gmax) chunk *= 2;
+        const bool sv = q % ((size_t)1 << m_log) == 0 && chunk <= ((size_t)1 << m_log) && q % chunk == 0 &&
+                        !std::getenv("EFFSC_HIP_EQ_DIRECT");
+        const unsigned g = sv ? (unsigned)(q / chunk)
+                              : (unsigned)std::max<size_t>(1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax));
         const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
         const bool nt = 8.0 * (double)L >= kNtMinBytes;
         TRY(prof_begin(h));
-        if (pend == 0)
+        if (sv && pend == 0)
+          hipLaunchKernelGGL((k_eq_pass_sv<0, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur,
+                             q, chunk, h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        else if (sv && nt)
+          hipLaunchKernelGGL((k_eq_pass_sv<2, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur,
+                             q, chunk, h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        else if (sv)
+          hipLaunchKernelGGL((k_eq_pass_sv<2, false>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0],
+                             h->d_cur, q, chunk, h->eq_l, h->eq_r, (unsigned)m_log, h->red);
+        else if (pend == 0)
           hipLaunchKernelGGL((k_eq_pass<0, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur, q,
                              h->eq_l, h->eq_r, (unsigned)m_log, h->red);
         else if (nt)
@@ -1984,7 +2066,8 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
           hipLaunchKernelGGL((k_eq_pass<2, false>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur,
                              q, h->eq_l, h->eq_r, (unsigned)m_log, h->red);
         HIPCHK(hipGetLastError());
-        TRY(prof_end(h, 3, 8.0 * ((double)L + (pend ? (double)Lv : 0.0)), (double)q * (8.0 + 8.0 + (pend ? 12.0 : 0.0))));
+        TRY(prof_end(h, 3, 8.0 * ((double)L + (pend ? (double)Lv : 0.0)),
+                     (double)q * ((sv ? 4.0 : 8.0 + 8.0) + (pend ? 12.0 : 0.0))));
         if (pend) {
           L = Lv;
           h->len = Lv;

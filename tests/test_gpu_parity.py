@@ -132,15 +132,21 @@ def test_eq_factored_parity(ext, v):
     check_proof("eq", ext, ext, 1 << v, v, 900 + v)
 
 
-@pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 20), (1, 23), (3, 11), (3, 12), (3, 15)])
+@pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 18), (1, 20), (1, 23), (3, 11), (3, 12),
+                                   (3, 15)])
 def test_eq_factored_device_loop(ext, v, monkeypatch):
     # large p: multi-workgroup rounds with in-reduction eq folds, then the
     # one-workgroup tail; against the oracle and the per-round host loop
-    # base field: two rounds per pass against EQ = eq_l x eq_r computed on
-    # the fly (v = 23 has a pass across the left/right split, v_L = 11);
+    # base field: two rounds per pass against EQ = eq_l x eq_r, in the
+    # split-value form while both variables of a pass are left ones, else
+    # computed on the fly (v = 23 has a pass across the left/right split,
+    # v_L = 11); EFFSC_HIP_EQ_DIRECT computes EQ on the fly in every pass,
     # EFFSC_HIP_ONE_ROUND keeps one round per launch
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     if ext == 1:
+        monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+        monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
         monkeypatch.setenv("EFFSC_HIP_ONE_ROUND", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v)
         monkeypatch.delenv("EFFSC_HIP_ONE_ROUND")
