@@ -147,6 +147,7 @@ struct effsc_hip_prover {
   uint64_t* eq_r = nullptr;
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
   uint64_t* eq_point = nullptr;  // canonical point, v * ext words (rebuilds the eq tables on reset)
+  uint64_t* eq_full = nullptr;   // EQ = eq_l x eq_r at the register-tail level (at most 2^14 words)
   // LSB (pair-split) provers hold their tables bit-reversed and run the MSB
   // path; post-state reads reverse back (src/streams/memory/core.rs:20-47)
   bool lsb = false;
@@ -168,11 +169,12 @@ struct effsc_hip_prover {
   // Device-resident proof loop (effsc_hip_sumcheck without a hook)
   bool dev_mode = false;       // Ops skip the plain reduction; grid in last_grid
   unsigned last_grid = 0;
-  DevFs* d_fs = nullptr;
-  uint64_t* d_polys = nullptr;  // kMaxRounds * kMaxK * 3 words
+  // one device block, polys | chals | fin | fs, the layout of h_proof
+  uint64_t* d_polys = nullptr;  // kMaxRounds * kMaxK * 3 words (block start)
   uint64_t* d_chals = nullptr;  // kMaxRounds * 3 words
-  uint64_t* d_cur = nullptr;    // 3 words: current challenge
-  uint64_t* d_fin = nullptr;    // 3 words
+  uint64_t* d_fin = nullptr;    // 4 words
+  DevFs* d_fs = nullptr;
+  uint64_t* d_cur = nullptr;    // 3 words: current challenge (own allocation)
   uint64_t* h_proof = nullptr;  // pinned staging: polys | chals | fin | fs
   // Kernel timing (effsc_hip_profile): HIP events around each launch on the
   // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold,
@@ -770,13 +772,13 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_l) (void)hipFree(h->eq_l);
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
+  if (h->eq_full) (void)hipFree(h->eq_full);
   if (h->d_pass0) (void)hipFree(h->d_pass0);
   if (h->ooc_parts) (void)hipFree(h->ooc_parts);
   if (h->gkr_b) (void)hipFree(h->gkr_b);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
-  for (uint64_t* p : {h->d_polys, h->d_chals, h->d_cur, h->d_fin}) if (p) (void)hipFree(p);
-  if (h->d_fs) (void)hipFree(h->d_fs);
+  for (uint64_t* p : {h->d_polys, h->d_cur}) if (p) (void)hipFree(p);  // d_chals, d_fin, d_fs: inside d_polys
   if (h->h_proof) (void)hipHostFree(h->h_proof);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1836,17 +1838,29 @@ bool device_loop_applies(const effsc_hip_prover* h, size_t num_rounds) {
   return num_rounds <= logl;  // every round evaluates a table of length >= 2
 }
 
+// Words of the proof block: polys | chals | fin (4) | transcript state.  The
+// device block and the pinned host staging share this layout, so one copy
+// brings a whole proof back (each small copy is a blit of about 5 us on the
+// stream).
+constexpr size_t kProofWords = kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4 + (sizeof(DevFs) + 7) / 8;
+
 int ensure_dev_bufs(effsc_hip_prover* h) {
   if (h->d_fs) return EFFSC_HIP_OK;
-  uint64_t* fs = nullptr;
-  TRY(alloc_dev(&fs, (sizeof(DevFs) + 7) / 8));
-  h->d_fs = reinterpret_cast<DevFs*>(fs);
-  TRY(alloc_dev(&h->d_polys, kMaxRounds * kMaxK * 3));
-  TRY(alloc_dev(&h->d_chals, kMaxRounds * 3));
+  uint64_t* blk = nullptr;
+  TRY(alloc_dev(&blk, kProofWords));
+  h->d_polys = blk;
+  h->d_chals = blk + kMaxRounds * kMaxK * 3;
+  h->d_fin = h->d_chals + kMaxRounds * 3;
+  h->d_fs = reinterpret_cast<DevFs*>(h->d_fin + 4);
   TRY(alloc_dev(&h->d_cur, 4));
-  TRY(alloc_dev(&h->d_fin, 4));
-  size_t words = kMaxRounds * kMaxK * 3 + kMaxRounds * 3 + 4 + (sizeof(DevFs) + 7) / 8;
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), words * 8, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), kProofWords * 8, hipHostMallocDefault));
+  return EFFSC_HIP_OK;
+}
+
+// The proof block back to the host staging (polys, challenges, final value
+// and transcript state in one copy).
+int copy_proof_out(effsc_hip_prover* h) {
+  HIPCHK(hipMemcpyAsync(h->h_proof, h->d_polys, kProofWords * 8, hipMemcpyDeviceToHost, h->stream));
   return EFFSC_HIP_OK;
 }
 
@@ -1981,10 +1995,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   uint64_t* hp = h->h_proof;
   uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
   uint64_t* hf = hc + kMaxRounds * 3;
-  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * K * E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
-  if (tail_used) HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  TRY(copy_proof_out(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * K * E, round_polys);
@@ -2011,6 +2022,45 @@ bool eq_device_applies(const effsc_hip_prover* h, size_t num_rounds) {
   return h->kind == K_EQ && h->rounds_elapsed == 0 && num_rounds == h->v && h->v >= 1 && h->v <= kMaxRounds &&
          h->len == ((size_t)1 << h->v) && !h->in_work && h->eq_l_len == ((size_t)1 << h->v_l) &&
          h->eq_r_len == ((size_t)1 << (h->v - h->v_l));
+}
+
+// Base-field EqFactored rounds in the register tail: the proof is the inner
+// product of p with EQ = eq_l x eq_r (src/provers/eq_factored.rs:411-446),
+// so once p is down to Lt = 2^11..2^14 elements the remaining rounds are
+// k_tail_reg<2> over p and EQ written out at that level.  The final value
+// p(r) EQ(r) equals p(r) eq_l(r_L) eq_r(r_R) (the MLE of a tensor product is
+// the product of the MLEs); k_eq_post_fold then folds eq_l and eq_r with the
+// tail's challenges for the post-state.
+// k_eq_tail_prep: p folded with the two pending challenges (level 4 Lt ->
+// Lt, in place when src == dst: element e reads e + k Lt, k < 4, and only
+// its own thread writes e) and EQ[e] = eq_l[e >> m] eq_r[e & (2^m - 1)].
+__global__ __launch_bounds__(kBlock) void k_eq_tail_prep(const uint64_t* src, uint64_t* dst, const uint64_t* cptr,
+                                                         size_t Lt, const uint64_t* eql, const uint64_t* eqr,
+                                                         unsigned m_log, uint64_t* eqf) {
+  const uint64_t c1 = cptr[0], c2 = cptr[1];
+  const size_t mask = ((size_t)1 << m_log) - 1;
+  for (size_t e = gtid(); e < Lt; e += (size_t)gridDim.x * blockDim.x) {
+    dst[e] = fold_chain<2>(src, e, Lt, c1, c2);
+    eqf[e] = gl::mul(eql[e >> m_log], eqr[e & mask]);
+  }
+}
+
+// One workgroup: folds eq_l with the challenges of the first `left` rounds
+// and eq_r with the rest (MSB folds in place, one barrier per round), and
+// leaves the last challenge in cur.
+__global__ __launch_bounds__(kReduceBlock) void k_eq_post_fold(uint64_t* eql, size_t eql_len, uint64_t* eqr,
+                                                               size_t eqr_len, const uint64_t* chals, int rounds,
+                                                               int left, uint64_t* cur) {
+  for (int r = 0; r < rounds; r++) {
+    const uint64_t c = chals[r];
+    uint64_t* t = r < left ? eql : eqr;
+    size_t& len = r < left ? eql_len : eqr_len;
+    const size_t h = len / 2;
+    for (size_t k = threadIdx.x; k < h; k += blockDim.x) t[k] = gl::add(t[k], gl::mul(c, gl::sub(t[k + h], t[k])));
+    len = h;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && rounds > 0) cur[0] = chals[rounds - 1];
 }
 
 template <class FI, class F>
@@ -2087,6 +2137,49 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         HIPCHK(hipGetLastError());
         j += 2;
         pend = 2;
+      }
+      const size_t Lt = L >> pend;
+      if (pend == 2 && Lt >= 2048 && Lt <= 16384 && !std::getenv("EFFSC_HIP_EQ_TAIL")) {
+        // the remaining v - j rounds in the register tail over p and EQ
+        if (!h->eq_full) TRY(alloc_dev(&h->eq_full, (size_t)1 << 14));
+        size_t m_log = 0;
+        while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+        const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
+        hipLaunchKernelGGL(k_eq_tail_prep, dim3(blocks_for(h, Lt)), dim3(kBlock), 0, h->stream, src, h->work[0],
+                           h->d_cur, Lt, h->eq_l, h->eq_r, (unsigned)m_log, h->eq_full);
+        HIPCHK(hipGetLastError());
+        Views<Fp, 2> tv{};
+        tv.t[0] = View<Fp>{h->work[0], 0};
+        tv.t[1] = View<Fp>{h->eq_full, 0};
+        const int rounds = (int)(v - j);
+        switch (Lt) {
+          case 2048:
+            hipLaunchKernelGGL((k_tail_reg<2, 2>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds, 0,
+                               h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin);
+            break;
+          case 4096:
+            hipLaunchKernelGGL((k_tail_reg<2, 4>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds, 0,
+                               h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin);
+            break;
+          case 8192:
+            hipLaunchKernelGGL((k_tail_reg<2, 8>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds, 0,
+                               h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin);
+            break;
+          default:
+            hipLaunchKernelGGL((k_tail_reg<2, 16>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds,
+                               0, h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin);
+        }
+        HIPCHK(hipGetLastError());
+        const int left = j < v_l ? (int)(v_l - j) : 0;
+        hipLaunchKernelGGL(k_eq_post_fold, dim3(1), dim3(kReduceBlock), 0, h->stream, h->eq_l, h->eq_l_len, h->eq_r,
+                           h->eq_r_len, h->d_chals + j, rounds, left, h->d_cur);
+        HIPCHK(hipGetLastError());
+        h->len = 1;
+        h->in_work = true;
+        h->eq_l_len = 1;
+        h->eq_r_len = 1;
+        h->rounds_elapsed = v;
+        return EFFSC_HIP_OK;
       }
       if (pend == 2) {
         // the older pending challenge folds p with a multi-block fold; the
@@ -2336,10 +2429,7 @@ int prove_device_gkr(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcrip
   uint64_t* hp = h->h_proof;
   uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
   uint64_t* hf = hc + kMaxRounds * 3;
-  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * 2 * E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
-  if (tail_used) HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  TRY(copy_proof_out(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * 2 * E, round_polys);
@@ -2377,10 +2467,7 @@ int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript
   uint64_t* hp = h->h_proof;
   uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
   uint64_t* hf = hc + kMaxRounds * 3;
-  HIPCHK(hipMemcpyAsync(hp, h->d_polys, num_rounds * 2 * E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hc, h->d_chals, num_rounds * E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hf, h->d_fin, E * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(hs, h->d_fs, sizeof(DevFs), hipMemcpyDeviceToHost, h->stream));
+  TRY(copy_proof_out(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * 2 * E, round_polys);

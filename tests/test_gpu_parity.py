@@ -140,13 +140,21 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     # base field: two rounds per pass against EQ = eq_l x eq_r, in the
     # split-value form while both variables of a pass are left ones, else
     # computed on the fly (v = 23 has a pass across the left/right split,
-    # v_L = 11); EFFSC_HIP_EQ_DIRECT computes EQ on the fly in every pass,
+    # v_L = 11), then the register tail over p and EQ from 2^11..2^14
+    # elements; EFFSC_HIP_EQ_DIRECT computes EQ on the fly in every pass,
+    # EFFSC_HIP_EQ_TAIL keeps the one-workgroup eq tail,
     # EFFSC_HIP_ONE_ROUND keeps one round per launch
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     if ext == 1:
         monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v)
         monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
+        # the one-workgroup eq tail instead of the register tail over p and EQ
+        monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
+        monkeypatch.delenv("EFFSC_HIP_EQ_TAIL")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
         monkeypatch.setenv("EFFSC_HIP_ONE_ROUND", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v)
         monkeypatch.delenv("EFFSC_HIP_ONE_ROUND")
