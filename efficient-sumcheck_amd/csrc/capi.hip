@@ -320,12 +320,15 @@ struct Ops {
     Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
     FO c = chal ? from_words<FO>(chal) : gl::zero<FO>();
     bool vec = (q % 2 == 0) && (src_stride % 2 == 0) && (h->work_stride % 2 == 0);
-    if constexpr (FO::D == 3 && K >= 3) vec = false;
+    if constexpr (FO::D == 3 && K >= 3) vec = vec && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
     TRY(prof_begin(h));
     if constexpr (FO::D == 3 && K >= 3) {
-      unsigned g = blocks_for(h, q);
-      hipLaunchKernelGGL((k_fused_seq<FI, FO, K>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w,
-                         h->red);
+      if (vec)
+        hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 2>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in,
+                           out, c, cptr, q, h->w, h->red);
+      else
+        hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in, out,
+                           c, cptr, q, h->w, h->red);
     } else if (vec) {
       unsigned g = blocks_for(h, q / 2);
       hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w, h->red);
