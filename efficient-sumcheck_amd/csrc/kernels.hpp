@@ -255,16 +255,19 @@ __device__ __forceinline__ void finish(F* acc, Reduce r) {
 // workgroup latency-bound (8192 blocks of 8 values took about 8 us).
 template <class F, int K>
 __device__ __forceinline__ void sum_partials(const uint64_t* partials, unsigned nblocks, F* s) {
+  // two blocks per step for F_p^3 sums of three or more values: four spilled
+  // registers in the 1024-thread reduction kernels
+  constexpr int U = (F::D == 3 && K >= 3) ? 2 : 4;
   const unsigned st = blockDim.x;
   unsigned b = threadIdx.x;
-  for (; b + 3 * st < nblocks; b += 4 * st) {
-    F v[4][K];
+  for (; b + (U - 1) * st < nblocks; b += U * st) {
+    F v[U][K];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < U; u++)
 #pragma unroll
       for (int i = 0; i < K; i++) v[u][i] = gl::load_words<F>(partials + ((size_t)(b + u * st) * K + i) * F::D);
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < U; u++)
 #pragma unroll
       for (int i = 0; i < K; i++) s[i] = gl::add(s[i], v[u][i]);
   }
