@@ -1022,6 +1022,48 @@ __global__ __launch_bounds__(kBlock) void k_eq_round(View<FI> pin, View<F> pout,
   finish<F, 2>(acc, r);
 }
 
+// Split-value form of k_eq_round for the left phase: pair k = a 2^m + b has
+// weights L_lo(a) R(b) and L_hi(a) R(b), and a workgroup owns one chunk of
+// pairs inside one block of 2^m, so its a is fixed:
+//   q(0)   = L_lo(a) sum R lo,   q(inf) = (L_hi(a) - L_lo(a)) sum R (hi - lo),
+// two products per pair instead of four, and two per workgroup.  The same
+// partials as k_eq_round.
+template <class FI, class F>
+__global__ __launch_bounds__(kBlock) void k_eq_round_sv(View<FI> pin, View<F> pout, size_t L, int fold,
+                                                        const uint64_t* cptr, View<F> eql, size_t eql_len,
+                                                        View<F> eqr, size_t m_log, size_t chunk, uint64_t w,
+                                                        Reduce r) {
+  const F c = fold ? gl::load_words<F>(cptr) : gl::zero<F>();
+  const size_t H2 = fold ? L / 4 : L / 2;
+  const size_t mask = ((size_t)1 << m_log) - 1;
+  F T[2] = {gl::zero<F>(), gl::zero<F>()};
+  const size_t k0 = (size_t)blockIdx.x * chunk;
+  for (size_t k = k0 + threadIdx.x; k < k0 + chunk; k += blockDim.x) {
+    F lo, hi;
+    if (fold) {
+      FI x0 = pin.ld(k), x1 = pin.ld(k + H2), x2 = pin.ld(k + 2 * H2), x3 = pin.ld(k + 3 * H2);
+      lo = fold1<F>(x0, x2, c, w);
+      hi = fold1<F>(x1, x3, c, w);
+      pout.st(k, lo);
+      pout.st(k + H2, hi);
+    } else {
+      lo = gl::lift_to<F>(pin.ld(k));
+      hi = gl::lift_to<F>(pin.ld(k + H2));
+    }
+    const F R = eqr.ld(k & mask);
+    T[0] = gl::add(T[0], gl::mul(R, lo, w));
+    T[1] = gl::add(T[1], gl::mul(R, gl::sub(hi, lo), w));
+  }
+  block_reduce<F, 2>(T);
+  if (threadIdx.x == 0) {
+    const size_t a = k0 >> m_log;
+    const F llo = eql.ld(a), lhi = eql.ld(a + (eql_len >> 1));
+    const F v[2] = {gl::mul(llo, T[0], w), gl::mul(gl::sub(lhi, llo), T[1], w)};
+#pragma unroll
+    for (int i = 0; i < 2; i++) gl::store_words(v[i], r.partials + ((size_t)blockIdx.x * 2 + i) * F::D);
+  }
+}
+
 // Sums the round's partials, scales by eq_l[0] in the right phase, runs the
 // transcript step, then folds the eq table of this round's variable in
 // place (fold_half > 0) with the new challenge.
@@ -2206,11 +2248,23 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     size_t m_log = 0;
     while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
     const size_t units = have ? L / 4 : L / 2;
-    const unsigned g = blocks_for(h, units);
+    // split-value form in the left phase: chunks of at least 256 pairs inside
+    // one block of 2^m
+    size_t chunk = kBlock;
+    while (units / chunk > (size_t)h->grid_max * kPassGridScale) chunk *= 2;
+    const bool sv = left && chunk <= ((size_t)1 << m_log) && units % chunk == 0 &&
+                    !std::getenv("EFFSC_HIP_EQ_DIRECT");
+    const unsigned g = sv ? (unsigned)(units / chunk) : blocks_for(h, units);
     const View<FI> pin{h->in_work ? nullptr : h->in[0], h->in_stride};
     const View<F> pw{h->work[0], h->work_stride};
     TRY(prof_begin(h));
-    if (have && h->in_work) {
+    if (sv && have && h->in_work) {
+      hipLaunchKernelGGL((k_eq_round_sv<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, cur_in, eql,
+                         h->eq_l_len, eqr, m_log, chunk, h->w, h->red);
+    } else if (sv) {
+      hipLaunchKernelGGL((k_eq_round_sv<FI, F>), dim3(g), dim3(kBlock), 0, h->stream, pin, pw, L, have ? 1 : 0, cur_in,
+                         eql, h->eq_l_len, eqr, m_log, chunk, h->w, h->red);
+    } else if (have && h->in_work) {
       hipLaunchKernelGGL((k_eq_round<F, F>), dim3(g), dim3(kBlock), 0, h->stream, pw, pw, L, 1, cur_in, eql,
                          h->eq_l_len, eqr, m_log, left, h->w, h->red);
     } else {
@@ -2225,7 +2279,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
       const int din = (have && h->in_work) ? F::D : FI::D;
       const double pairs = (double)units;
       const double bytes = have ? 8.0 * (4.0 * pairs * din + 2.0 * pairs * F::D) : 8.0 * 2.0 * pairs * din;
-      const double muls = pairs * ((left ? 4.0 : 2.0) + (have ? 2.0 : 0.0)) * mul_cost<F>();
+      const double muls = pairs * ((left && !sv ? 4.0 : 2.0) + (have ? 2.0 : 0.0)) * mul_cost<F>();
       TRY(prof_end(h, have ? 0 : 1, bytes, muls));
     }
     if (have) {

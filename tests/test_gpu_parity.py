@@ -133,7 +133,7 @@ def test_eq_factored_parity(ext, v):
 
 
 @pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 18), (1, 20), (1, 23), (3, 11), (3, 12),
-                                   (3, 15)])
+                                   (3, 15), (3, 17), (3, 18)])
 def test_eq_factored_device_loop(ext, v, monkeypatch):
     # large p: multi-workgroup rounds with in-reduction eq folds, then the
     # one-workgroup tail; against the oracle and the per-round host loop
@@ -141,14 +141,15 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     # split-value form while both variables of a pass are left ones, else
     # computed on the fly (v = 23 has a pass across the left/right split,
     # v_L = 11), then the register tail over p and EQ from 2^11..2^14
-    # elements; EFFSC_HIP_EQ_DIRECT computes EQ on the fly in every pass,
+    # elements; over F_p^3 one round per launch, split-value in the left
+    # phase; EFFSC_HIP_EQ_DIRECT computes EQ per element in every pass,
     # EFFSC_HIP_EQ_TAIL keeps the one-workgroup eq tail,
     # EFFSC_HIP_ONE_ROUND keeps one round per launch
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
+    check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
     if ext == 1:
-        monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
-        check_proof("eq", ext, ext, 1 << v, v, 950 + v)
-        monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
         # the one-workgroup eq tail instead of the register tail over p and EQ
         monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v)
