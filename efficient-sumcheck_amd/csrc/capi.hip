@@ -3403,22 +3403,43 @@ struct effsc_hip_stream {
 
 namespace {
 
-// Host copy of `bytes` with up to 8 threads (the mapped file is pageable, so
-// it is staged through pinned memory by the CPU).
-void par_copy(void* dst, const void* src, size_t bytes) {
+// Host copy of several segments (the mapped files are pageable, so they are
+// staged through pinned memory by the CPU) with up to 8 threads sharing the
+// total bytes: a chunk is 4 or 16 regions per table of a few MiB each, too
+// small to split one at a time.
+struct CopySeg {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+void par_copy_segs(const std::vector<CopySeg>& segs) {
+  size_t total = 0;
+  for (const CopySeg& sg : segs) total += sg.bytes;
   const size_t per = (size_t)4 << 20;
-  int nt = (int)std::min<size_t>(8, (bytes + per - 1) / per);
+  static const int max_threads = [] {
+    const char* e = std::getenv("EFFSC_HIP_COPY_THREADS");
+    const int x = e ? std::atoi(e) : 0;
+    return x >= 1 && x <= 64 ? x : 8;
+  }();
+  const int nt = (int)std::min<size_t>((size_t)max_threads, (total + per - 1) / per);
   if (nt <= 1) {
-    std::memcpy(dst, src, bytes);
+    for (const CopySeg& sg : segs) std::memcpy(sg.dst, sg.src, sg.bytes);
     return;
   }
+  const size_t share = (total + nt - 1) / nt;
   std::vector<std::thread> th;
-  const size_t chunk = (bytes + nt - 1) / nt;
   for (int t = 0; t < nt; t++) {
-    size_t o = (size_t)t * chunk;
-    if (o >= bytes) break;
-    size_t len = std::min(chunk, bytes - o);
-    th.emplace_back([=] { std::memcpy((uint8_t*)dst + o, (const uint8_t*)src + o, len); });
+    const size_t lo = (size_t)t * share, hi = std::min(total, lo + share);
+    if (lo >= hi) break;
+    th.emplace_back([&segs, lo, hi] {
+      size_t base = 0;
+      for (const CopySeg& sg : segs) {
+        const size_t a = std::max(lo, base), b = std::min(hi, base + sg.bytes);
+        if (a < b) std::memcpy((uint8_t*)sg.dst + (a - base), (const uint8_t*)sg.src + (a - base), b - a);
+        base += sg.bytes;
+        if (base >= hi) break;
+      }
+    });
   }
   for (auto& x : th) x.join();
 }
@@ -3548,9 +3569,11 @@ int ooc_stream(effsc_hip_prover* h, size_t nreg, Consume&& consume) {
     const size_t u0 = c * ch.B, cnt = std::min(ch.B, R - u0);
     // pinned buffer b was last read by the copy of chunk c - 2
     if (c >= 2 && !chk(hipEventSynchronize(copied[b]), "wait")) break;
+    std::vector<CopySeg> segs;
     for (int i = 0; i < K; i++)
       for (size_t m = 0; m < nreg; m++)
-        par_copy(pin[b] + ((size_t)i * nreg + m) * ch.B * 8, h->ooc_src[i]->map + (m * R + u0) * 8, cnt * 8);
+        segs.push_back({pin[b] + ((size_t)i * nreg + m) * ch.B * 8, h->ooc_src[i]->map + (m * R + u0) * 8, cnt * 8});
+    par_copy_segs(segs);
     // device staging b was last read by the kernel of chunk c - 2
     if (c >= 2) chk(hipStreamWaitEvent(cs, used[b], 0), "wait");
     chk(hipMemcpyAsync(stage[b], pin[b], words * 8, hipMemcpyHostToDevice, cs), "copy");
@@ -3786,7 +3809,9 @@ int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, 
     const int b = (int)(c & 1);
     const size_t first = c * B, cnt = std::min(B, n - first);
     if (c >= 2 && !chk(hipEventSynchronize(used[b]), "wait")) break;  // staging b is free again
-    for (int j = 0; j < k; j++) par_copy(pin[b] + (size_t)j * B * el, streams[j]->map + first * el, cnt * el);
+    std::vector<CopySeg> segs;
+    for (int j = 0; j < k; j++) segs.push_back({pin[b] + (size_t)j * B * el, streams[j]->map + first * el, cnt * el});
+    par_copy_segs(segs);
     for (int j = 0; j < k && rc == EFFSC_HIP_OK; j++)
       chk(hipMemcpyAsync(dst[b] + (size_t)j * B * ext, pin[b] + (size_t)j * B * el, cnt * el, hipMemcpyHostToDevice,
                          cs), "copy");
@@ -3962,9 +3987,11 @@ int effsc_hip_stream_prover_new(const effsc_hip_config* cfg, int kind, effsc_hip
       const size_t u0 = c * B, cnt = std::min(B, q - u0);
       // pinned buffer b was last read by the copy of chunk c - 2
       if (c >= 2 && !chk(hipEventSynchronize(copied[b]), "wait")) break;
+      std::vector<CopySeg> segs;
       for (int i = 0; i < K; i++)
         for (int j = 0; j < 4; j++)
-          par_copy(pin[b] + ((size_t)i * 4 + j) * B * 8, streams[i]->map + (j * q + u0) * 8, cnt * 8);
+          segs.push_back({pin[b] + ((size_t)i * 4 + j) * B * 8, streams[i]->map + (j * q + u0) * 8, cnt * 8});
+      par_copy_segs(segs);
       for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++)
         for (int j = 0; j < 4 && rc == EFFSC_HIP_OK; j++)
           chk(hipMemcpyAsync(h->in[i] + j * q + u0, pin[b] + ((size_t)i * 4 + j) * B * 8, cnt * 8,
