@@ -140,3 +140,21 @@ def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
             assert pf.round_polys == pg.round_polys and pf.final_value == pg.final_value
             for i in range(4):
                 assert np.array_equal(f.table_words(i), g.table_words(i)), f"table {i}"
+
+
+def test_gkr_full_size_layer_matches_oracle():
+    # the benchmark layer: k = 12, 2^24 gate entries, all 24 rounds, against
+    # the oracle (proof, post-state tables, claimed W values)
+    k = 12
+    n = 1 << k
+    add, mult, wit = O.uniform(n * n, 0x5EED0041), O.uniform(n * n, 0x5EED0042), O.uniform(n, 0x5EED0043)
+    say(f"gkr full-size layer k={k}")
+    g = GkrProver(add, mult, wit)
+    pg = sumcheck(g, 2 * k, Transcript(b"gkr-full"))
+    o = O.Prover.gkr(add, mult, wit)
+    po = o.sumcheck(2 * k, O.Transcript.shake(b"gkr-full"))
+    assert tuple(as_oracle(pg, 1)) == tuple(po)
+    for i in range(4):
+        assert np.array_equal(g.table_words(i), o.table(i)), f"table {i}"
+    cw = o.claimed_w_values()
+    assert tuple(g.claimed_w_values()) == (int(cw[0][0]), int(cw[1][0]))

@@ -480,3 +480,28 @@ def test_measurement_probes_report_rates_and_reject_bad_arguments():
     assert L.effsc_hip_hbm_rate(0, 3, 1 << 20, C.byref(r)) == 1  # EINVAL: unknown pattern
     assert L.effsc_hip_hbm_rate(0, 0, 3 << 10, C.byref(r)) == 1  # EINVAL: not a power of two
     assert L.effsc_hip_hbm_rate(0, 0, 512, C.byref(r)) == 1  # EINVAL: below 1024
+
+
+def test_full_size_inner_product_matches_oracle():
+    # the benchmark configuration itself: two 2^28-element tables from the
+    # counter-based generator, proved on the GPU (two-round passes, register
+    # tail, device transcript) and by the oracle's threaded restatement of the
+    # reference prover (two folds per round, serial evaluate): bit-identical
+    # round polynomials, challenges and final value
+    v = 28
+    n = 1 << v
+    say(f"full-size inner product v={v}")
+    a, b = O.uniform(n, 0x5EED0002), O.uniform(n, 0x5EED0003)
+    proof = sumcheck(InnerProductProver(a, b), v, Transcript(b"effsc-hip/bench/ip"))
+    _, polys, chals, fin = O.bench_ip(a, b, v, 16)  # consumes a and b
+    assert [w for r in proof.round_polys for w in r] == [int(x) for x in polys]
+    assert proof.challenges == [int(x) for x in chals]
+    assert proof.final_value == fin
+
+
+@pytest.mark.parametrize("kind,ext,input_ext", [("p3", 3, 1), ("eq", 1, 1), ("eq", 3, 3)])
+def test_full_size_degree3_and_eq_configurations(kind, ext, input_ext):
+    # the v = 24 configurations at full size against the oracle: product of
+    # three base tables under F_p^3 challenges, EqFactored over the base
+    # field and over F_p^3 (proof words and post-state)
+    check_proof(kind, ext, input_ext, 1 << 24, 24, 0x5EED0004)
