@@ -69,3 +69,27 @@ def test_autotuned_batch_matches_oracle(graph):
         rp, ch, fin = o.sumcheck(v, O.Transcript.shake(b"tuned-%d" % i))
         assert proofs[i].round_polys == [[x[0] for x in r] for r in rp], f"instance {i} v={v}"
         assert proofs[i].final_value == fin[0]
+
+
+def test_full_size_batch_matches_oracle():
+    # the benchmark batch: 512 inner-product instances, v ~ U[12, 24] (about
+    # 21 GB of tables), one batched proof; the 8 largest instances and 8 more
+    # chosen at random against the oracle's threaded restatement
+    import random as _r
+
+    rng = _r.Random(0x5EED0004)
+    vs = [rng.randint(12, 24) for _ in range(512)]
+    print(f"full-size batch: {len(vs)} instances, {2 * sum(1 << v for v in vs) * 8 / 2**30:.1f} GiB", flush=True)
+    a = [DeviceTable.uniform(1 << v, 0x5EED0004 + 2 * i) for i, v in enumerate(vs)]
+    b = [DeviceTable.uniform(1 << v, 0x5EED0004 + 2 * i + 1) for i, v in enumerate(vs)]
+    bp = BatchProver(a, b, kind=2, graph=True)
+    proofs = bp.prove([Transcript(b"effsc-hip/bench/ip") for _ in vs])
+    order = sorted(range(len(vs)), key=lambda i: -vs[i])
+    sample = order[:8] + _r.Random(7).sample(order[8:], 8)
+    for i in sample:
+        v = vs[i]
+        _, polys, chals, fin = O.bench_ip(O.uniform(1 << v, 0x5EED0004 + 2 * i), O.uniform(1 << v, 0x5EED0004 + 2 * i + 1),
+                                          v, 16)
+        assert [w for r in proofs[i].round_polys for w in r] == [int(x) for x in polys], f"instance {i} v={v}"
+        assert proofs[i].challenges == [int(x) for x in chals]
+        assert proofs[i].final_value == fin
