@@ -1243,6 +1243,9 @@ __global__ __launch_bounds__((eq_tail_block<F>())) void k_eq_tail(View<FI> pin, 
                                                                    uint64_t* cur_out, uint64_t* fin, uint64_t w) {
   __shared__ uint64_t chal_sh[3];
   if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal && (int)threadIdx.x < ext ? cur_in[threadIdx.x] : 0;
+  // the sponge stays in wave 0's registers for the whole tail
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
   __syncthreads();
   const unsigned tid = threadIdx.x, nt = blockDim.x;
   bool src_in = from_input != 0;
@@ -1298,7 +1301,7 @@ __global__ __launch_bounds__((eq_tail_block<F>())) void k_eq_tail(View<FI> pin, 
       acc[1] = gl::mul(acc[1], sc, w);
     }
     if (tid < 64)
-      emit_round<F, 2>(acc, ext, t, polys + (size_t)(j - j0) * 2 * ext, chals + (size_t)(j - j0) * ext, chal_sh);
+      emit_round_fs<F, 2>(acc, ext, fs, polys + (size_t)(j - j0) * 2 * ext, chals + (size_t)(j - j0) * ext, chal_sh);
     __syncthreads();
     // fold the eq table of variable j with its challenge
     const F c = gl::load_words<F>(chal_sh);
@@ -1330,6 +1333,7 @@ __global__ __launch_bounds__((eq_tail_block<F>())) void k_eq_tail(View<FI> pin, 
     gl::store_words(gl::mul(gl::mul(p0, eql.ld(0), w), eqr.ld(0), w), fin);
     for (int i = 0; i < ext; i++) cur_out[i] = chal_sh[i];
   }
+  if (tid < 64) fs.store(t);
 }
 
 int read0(effsc_hip_prover* h, const uint64_t* planes, size_t stride, int d, uint64_t* out) {
@@ -1623,6 +1627,9 @@ __global__ __launch_bounds__((tail_block<F, 4>())) void k_gkr_tail(Views<FI, 4> 
                                                                   const uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
   __shared__ uint64_t chal_sh[3];
   if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal ? chal_cur[threadIdx.x] : 0;
+  // the sponge stays in wave 0's registers for the whole tail
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
   __syncthreads();
   const unsigned tid = threadIdx.x, nt = blockDim.x;
   bool src_in = from_input != 0;
@@ -1673,7 +1680,7 @@ __global__ __launch_bounds__((tail_block<F, 4>())) void k_gkr_tail(Views<FI, 4> 
     if (tid < 64) {
       F wire[2];
       gkr_wire<F>(acc, wire);
-      emit_round<F, 2>(wire, ext, t, polys + (size_t)r * 2 * ext, chals + (size_t)r * ext, chal_sh);
+      emit_round_fs<F, 2>(wire, ext, fs, polys + (size_t)r * 2 * ext, chals + (size_t)r * ext, chal_sh);
     }
     __syncthreads();
     have_chal = 1;
@@ -1705,6 +1712,7 @@ __global__ __launch_bounds__((tail_block<F, 4>())) void k_gkr_tail(Views<FI, 4> 
     gl::store_words(v, wd);
     for (int c = 0; c < ext; c++) fin[c] = wd[c];
   }
+  if (tid < 64) fs.store(t);
 }
 
 // Factored witness (n = 2^k).  While the b variables are bound (j <= k folds

@@ -181,11 +181,9 @@ struct LaneFs {
 // Writes them (lifted to ext words) to `poly`, absorbs them, squeezes the
 // challenge into chal_out and chal_cur; lane 0 does the stores.
 template <class F, int K>
-__device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64_t* poly, uint64_t* chal_out,
-                                           uint64_t* chal_cur) {
+__device__ __forceinline__ void emit_round_fs(const F* s, int ext, LaneFs& fs, uint64_t* poly, uint64_t* chal_out,
+                                              uint64_t* chal_cur) {
   const bool lane0 = (threadIdx.x & 63) == 0;
-  LaneFs fs;
-  fs.load(t);
 #pragma unroll
   for (int i = 0; i < K; i++) {
     uint64_t wd[3] = {0, 0, 0};
@@ -210,6 +208,15 @@ __device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64
       }
     }
   }
+}
+
+// emit_round_fs with the sponge loaded from and stored back to t.
+template <class F, int K>
+__device__ __forceinline__ void emit_round(const F* s, int ext, DevFs* t, uint64_t* poly, uint64_t* chal_out,
+                                           uint64_t* chal_cur) {
+  LaneFs fs;
+  fs.load(t);
+  emit_round_fs<F, K>(s, ext, fs, poly, chal_out, chal_cur);
   fs.store(t);
 }
 
@@ -304,6 +311,9 @@ __global__ __launch_bounds__((tail_block<F, K>())) void k_tail(Views<FI, K> in, 
                                                      uint64_t* chals, uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
   __shared__ uint64_t chal_sh[3];
   if (threadIdx.x < 3) chal_sh[threadIdx.x] = have_chal ? chal_cur[threadIdx.x] : 0;
+  // the sponge stays in wave 0's registers for the whole tail
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
   __syncthreads();
   const unsigned tid = threadIdx.x, nt = blockDim.x;
   bool src_in = from_input != 0;
@@ -357,7 +367,7 @@ __global__ __launch_bounds__((tail_block<F, K>())) void k_tail(Views<FI, K> in, 
       }
     }
     block_reduce<F, K>(acc);
-    if (tid < 64) emit_round<F, K>(acc, ext, t, polys + (size_t)r * K * ext, chals + (size_t)r * ext, chal_sh);
+    if (tid < 64) emit_round_fs<F, K>(acc, ext, fs, polys + (size_t)r * K * ext, chals + (size_t)r * ext, chal_sh);
     __syncthreads();
     have_chal = 1;
   }
@@ -390,6 +400,7 @@ __global__ __launch_bounds__((tail_block<F, K>())) void k_tail(Views<FI, K> in, 
     gl::store_words(v, wd);
     for (int c = 0; c < ext; c++) fin[c] = wd[c];
   }
+  if (tid < 64) fs.store(t);
 }
 
 // Transcript step of one round with the sponge held in registers by the 64
