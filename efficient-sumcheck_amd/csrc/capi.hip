@@ -388,7 +388,7 @@ struct DevOps {
       // base-field tables of 2^11 .. 2^14 elements (after the pending fold):
       // the register-resident tail
       const size_t L1 = have ? L / 2 : L;
-      if (rounds > 0 && L1 >= 2048 && L1 <= 16384) {
+      if (rounds > 0 && L1 >= 2048 && L1 <= 16384 && !std::getenv("EFFSC_HIP_TAIL_PLAIN")) {
         auto go = [&](auto m0) {
           constexpr int M0 = decltype(m0)::value;
           hipLaunchKernelGGL((k_tail_reg<K, M0>), dim3(1), dim3(kTailRegThreads), 0, h->stream, in, work,
@@ -401,6 +401,25 @@ struct DevOps {
           case 8: go(std::integral_constant<int, 8>{}); break;
           default: go(std::integral_constant<int, 16>{}); break;
         }
+        HIPCHK(hipGetLastError());
+        return EFFSC_HIP_OK;
+      }
+    }
+    if constexpr (std::is_same<FI, Fp3>::value && std::is_same<F, Fp3>::value && K >= 3 && K <= 4) {
+      // F_p^3 products of 2^8 or 2^9 work elements (after the pending
+      // fold): the register-resident tail (at 2^10 it spills)
+      const size_t L1 = L / 2;
+      if (have && !from_input && rounds > 0 && L1 >= 256 && L1 <= 512 && !std::getenv("EFFSC_HIP_TAIL_PLAIN")) {
+        auto go = [&](auto m0) {
+          constexpr int M0 = decltype(m0)::value;
+          hipLaunchKernelGGL((k_tail_reg3<K, M0>), dim3(1), dim3(kTailReg3Threads), 0, h->stream, work, rounds,
+                             h->d_fs, h->d_polys + first_round * K * 3, h->d_chals + first_round * 3, cur, h->d_fin,
+                             h->w);
+        };
+        if (L1 == 256)
+          go(std::integral_constant<int, 1>{});
+        else
+          go(std::integral_constant<int, 2>{});
         HIPCHK(hipGetLastError());
         return EFFSC_HIP_OK;
       }

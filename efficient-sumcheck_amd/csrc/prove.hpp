@@ -606,4 +606,212 @@ __global__ __launch_bounds__(kTailRegThreads) void k_tail_reg(Views<Fp, K> in, V
   if (tid < 64) fs.store(t);
 }
 
+// ---------------------------------------------------------------------------
+// k_tail for products of K >= 3 F_p^3 tables of 2^8 or 2^9 elements after the
+// pending fold, with the tables in registers (as k_tail_reg for the base
+// field): 256 threads, thread t holding elements t + 256 m (m < M0) of every
+// table.  While a thread holds two or more elements a round's pairs are in
+// the thread; at 256 elements the tables move through LDS to wave 0 (lane l
+// holding l + 64 m) and the remaining rounds run in that wave alone, in-lane
+// down to 64 elements, then by shuffles.  The sponge stays in wave 0's
+// registers.  Same round polynomials, challenges, final tables (written back
+// when the rounds stop early) and final value as k_tail.
+constexpr int kTailReg3Threads = 256;
+
+// Transcript step of one F_p^3 round with the sponge in registers: writes
+// poly (K x 3 words) and chal_out (3 words), returns the challenge
+// (wave-uniform).
+template <int K>
+__device__ __forceinline__ gl::Fp3 emit_round_regs3(LaneFs& fs, const gl::Fp3* s, uint64_t* poly, uint64_t* chal_out) {
+  const bool lane0 = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    uint64_t wd[3] = {uniform64(s[i].c0), uniform64(s[i].c1), uniform64(s[i].c2)};
+    if (lane0)
+#pragma unroll
+      for (int c = 0; c < 3; c++) poly[3 * i + c] = wd[c];
+    fs.send(3, wd);
+  }
+  uint64_t ch[3];
+  fs.challenge(3, ch);
+  if (lane0)
+#pragma unroll
+    for (int c = 0; c < 3; c++) chal_out[c] = ch[c];
+  return gl::Fp3{ch[0], ch[1], ch[2]};
+}
+
+template <int K, int M0>
+__global__ __launch_bounds__(kTailReg3Threads) void k_tail_reg3(Views<gl::Fp3, K> work, int rounds, DevFs* t,
+                                                               uint64_t* polys, uint64_t* chals,
+                                                               const uint64_t* chal_cur, uint64_t* fin, uint64_t w) {
+  using gl::Fp3;
+  static_assert(M0 == 1 || M0 == 2, "M0: 1 or 2");
+  constexpr int LG0 = ilog2c(M0);
+  constexpr int T = kTailReg3Threads;
+  __shared__ uint64_t tab[K][3][T];
+  __shared__ uint64_t chal_sh[3];
+  const unsigned tid = threadIdx.x;
+  const int lane = tid & 63;
+  Fp3 x[K][M0];
+  {
+    // the pending fold: length 2 L1 -> L1, pairs (m, m + M0) of this thread
+    const Fp3 c = gl::load_words<Fp3>(chal_cur);
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int m = 0; m < M0; m++)
+        x[i][m] = fold1<Fp3>(work.t[i].ld(tid + T * m), work.t[i].ld(tid + T * (m + M0)), c, w);
+  }
+  LaneFs fs;
+  if (tid < 64) fs.load(t);
+  int r = 0;
+  int Lc = T * M0;
+  // ---- all waves: rounds with in-thread pairs
+#pragma unroll
+  for (int lg = LG0; lg >= 1; lg--) {
+    const int Mc = 1 << lg;
+    if (r < rounds) {
+      Fp3 acc[K];
+#pragma unroll
+      for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp3>();
+#pragma unroll
+      for (int m = 0; m < Mc / 2; m++) {
+        Fp3 lo[K], hi[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          lo[i] = x[i][m];
+          hi[i] = x[i][m + Mc / 2];
+        }
+        eval_pair<Fp3, K>(lo, hi, acc, w);
+      }
+      block_reduce<Fp3, K>(acc);
+      if (tid < 64) {
+        const Fp3 c = emit_round_regs3<K>(fs, acc, polys + (size_t)r * K * 3, chals + (size_t)r * 3);
+        if (tid == 0) {
+          chal_sh[0] = c.c0;
+          chal_sh[1] = c.c1;
+          chal_sh[2] = c.c2;
+        }
+      }
+      __syncthreads();
+      const Fp3 c = gl::load_words<Fp3>(chal_sh);
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int m = 0; m < Mc / 2; m++) x[i][m] = fold1<Fp3>(x[i][m], x[i][m + Mc / 2], c, w);
+      Lc /= 2;
+      r++;
+    }
+  }
+  if (r < rounds) {
+    // ---- hand-over: 256 elements to wave 0 through LDS; lane l owns
+    // positions l + 64 m of `tab` (no other lane touches them)
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      tab[i][0][tid] = x[i][0].c0;
+      tab[i][1][tid] = x[i][0].c1;
+      tab[i][2][tid] = x[i][0].c2;
+    }
+    __syncthreads();
+    if (tid >= 64) return;
+    auto ld = [&](int i, int pos) { return Fp3{tab[i][0][pos], tab[i][1][pos], tab[i][2][pos]}; };
+    auto st = [&](int i, int pos, Fp3 v) {
+      tab[i][0][pos] = v.c0;
+      tab[i][1][pos] = v.c1;
+      tab[i][2][pos] = v.c2;
+    };
+#pragma unroll
+    for (int lg = 2; lg >= 1; lg--) {
+      const int Mw = 1 << lg;
+      if (r < rounds) {
+        Fp3 acc[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp3>();
+#pragma unroll
+        for (int m = 0; m < Mw / 2; m++) {
+          Fp3 lo[K], hi[K];
+#pragma unroll
+          for (int i = 0; i < K; i++) {
+            lo[i] = ld(i, lane + 64 * m);
+            hi[i] = ld(i, lane + 64 * (m + Mw / 2));
+          }
+          eval_pair<Fp3, K>(lo, hi, acc, w);
+        }
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int mm = 32; mm >= 1; mm >>= 1) acc[i] = gl::add(acc[i], shfl_xor(acc[i], mm));
+        const Fp3 c = emit_round_regs3<K>(fs, acc, polys + (size_t)r * K * 3, chals + (size_t)r * 3);
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int m = 0; m < Mw / 2; m++)
+            st(i, lane + 64 * m, fold1<Fp3>(ld(i, lane + 64 * m), ld(i, lane + 64 * (m + Mw / 2)), c, w));
+        Lc /= 2;
+        r++;
+      }
+    }
+    // 64 elements, one per lane: pairs (l, l + h)
+    Fp3 z[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) z[i] = ld(i, lane);
+#pragma unroll
+    for (int lg = 5; lg >= 0; lg--) {
+      const int h = 1 << lg;
+      if (r < rounds) {
+        Fp3 lo[K], hi[K], acc[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          lo[i] = z[i];
+          hi[i] = shfl_xor(z[i], h);  // lanes l < h read l + h
+          acc[i] = gl::zero<Fp3>();
+        }
+        if (lane < h) eval_pair<Fp3, K>(lo, hi, acc, w);
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int mm = 32; mm >= 1; mm >>= 1) acc[i] = gl::add(acc[i], shfl_xor(acc[i], mm));
+        const Fp3 c = emit_round_regs3<K>(fs, acc, polys + (size_t)r * K * 3, chals + (size_t)r * 3);
+#pragma unroll
+        for (int i = 0; i < K; i++) z[i] = fold1<Fp3>(lo[i], hi[i], c, w);
+        Lc /= 2;
+        r++;
+      }
+    }
+    // write back the tables (length Lc; lanes >= Lc hold no element)
+    if (Lc >= 64) {
+      const int Mw = Lc / 64;
+#pragma unroll
+      for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+          if (m < Mw) work.t[i].st(lane + 64 * m, ld(i, lane + 64 * m));
+    } else {
+#pragma unroll
+      for (int i = 0; i < K; i++)
+        if (lane < Lc) work.t[i].st(lane, z[i]);
+    }
+    if (lane == 0) {
+      Fp3 v = gl::zero<Fp3>();
+      if (Lc == 1) {
+        v = z[0];
+#pragma unroll
+        for (int i = 1; i < K; i++) v = gl::mul(v, z[i], w);
+      }
+      gl::store_words(v, fin);
+    }
+    fs.store(t);
+    return;
+  }
+  // rounds ended with the tables still spread over the workgroup
+  const int Mc = Lc / T;
+#pragma unroll
+  for (int i = 0; i < K; i++)
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+      if (m < Mc) work.t[i].st(tid + T * m, x[i][m]);
+  if (tid == 0) gl::store_words(gl::zero<Fp3>(), fin);  // more than one element left
+  if (tid < 64) fs.store(t);
+}
+
 }  // namespace effsc_hip

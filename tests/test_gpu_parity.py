@@ -407,6 +407,24 @@ def test_register_tail_parity(kind, v, tail, monkeypatch):
             check_proof(kind, 1, 1, n, v, 4000 + v, transcript=tr, rounds=rounds)
 
 
+@pytest.mark.parametrize("tail", [None, 1 << 9, 1 << 10])
+@pytest.mark.parametrize("kind,input_ext", [("p3", 1), ("p3", 3), ("p4", 3)])
+@pytest.mark.parametrize("v", [10, 11, 12])
+def test_ext3_register_tail_parity(kind, input_ext, v, tail, monkeypatch):
+    # F_p^3 products with tails of 2^8..2^9 work elements run in registers:
+    # stop in every phase (all waves, in-lane pairs, shuffles); the plain
+    # one-workgroup tail gives the same proof
+    if tail is not None:
+        monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
+    n = 1 << v
+    for rounds in sorted({1, 2, 3, 4, 5, 6, v - 4, v - 1, v}):
+        for tr in ("shake", "rng"):
+            g, pg = check_proof(kind, 3, input_ext, n, v, 4600 + v, transcript=tr, rounds=rounds)
+    monkeypatch.setenv("EFFSC_HIP_TAIL_PLAIN", "1")
+    _, plain = check_proof(kind, 3, input_ext, n, v, 4600 + v, transcript="rng", rounds=v)
+    assert plain.round_polys == pg.round_polys and plain.final_value == pg.final_value
+
+
 @pytest.mark.parametrize("tail", [2, 1 << 14])
 def test_device_loop_partial_then_continue(tail, monkeypatch):
     monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
