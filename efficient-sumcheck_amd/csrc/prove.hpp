@@ -720,14 +720,14 @@ __global__ __launch_bounds__(kTailReg3Threads) void k_tail_reg3(Views<gl::Fp3, K
       tab[i][1][pos] = v.c1;
       tab[i][2][pos] = v.c2;
     };
-#pragma unroll
+    // the round loops below are too large to unroll; they index no register
+    // arrays by round
     for (int lg = 2; lg >= 1; lg--) {
       const int Mw = 1 << lg;
       if (r < rounds) {
         Fp3 acc[K];
 #pragma unroll
         for (int i = 0; i < K; i++) acc[i] = gl::zero<Fp3>();
-#pragma unroll
         for (int m = 0; m < Mw / 2; m++) {
           Fp3 lo[K], hi[K];
 #pragma unroll
@@ -744,7 +744,6 @@ __global__ __launch_bounds__(kTailReg3Threads) void k_tail_reg3(Views<gl::Fp3, K
         const Fp3 c = emit_round_regs3<K>(fs, acc, polys + (size_t)r * K * 3, chals + (size_t)r * 3);
 #pragma unroll
         for (int i = 0; i < K; i++)
-#pragma unroll
           for (int m = 0; m < Mw / 2; m++)
             st(i, lane + 64 * m, fold1<Fp3>(ld(i, lane + 64 * m), ld(i, lane + 64 * (m + Mw / 2)), c, w));
         Lc /= 2;
@@ -755,7 +754,6 @@ __global__ __launch_bounds__(kTailReg3Threads) void k_tail_reg3(Views<gl::Fp3, K
     Fp3 z[K];
 #pragma unroll
     for (int i = 0; i < K; i++) z[i] = ld(i, lane);
-#pragma unroll
     for (int lg = 5; lg >= 0; lg--) {
       const int h = 1 << lg;
       if (r < rounds) {
