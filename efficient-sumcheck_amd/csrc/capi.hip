@@ -320,7 +320,13 @@ struct Ops {
     Views<FO, K> out = views<FO, K>(h->work, h->work_stride);
     FO c = chal ? from_words<FO>(chal) : gl::zero<FO>();
     bool vec = (q % 2 == 0) && (src_stride % 2 == 0) && (h->work_stride % 2 == 0);
-    if constexpr (FO::D == 3 && K >= 3) vec = vec && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
+    // F_p^3 products: one quadruple per thread below 2^18 quadruples.  There
+    // the launch is latency-bound (few waves, each a chain of about a dozen
+    // F_p^3 products per quadruple), and twice the threads halve the chain:
+    // 12.8-16.3 us against 22.5-25.1 us for 2^9..2^16 quadruples, 25.0
+    // against 28.7 us at 2^17; from 2^18 on two per thread are faster.
+    if constexpr (FO::D == 3 && K >= 3)
+      vec = vec && q >= ((size_t)1 << 18) && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
     TRY(prof_begin(h));
     if constexpr (FO::D == 3 && K >= 3) {
       if (vec)
