@@ -2141,6 +2141,38 @@ __global__ __launch_bounds__(kReduceBlock) void k_eq_post_fold(uint64_t* eql, si
   if (threadIdx.x == 0 && rounds > 0) cur[0] = chals[rounds - 1];
 }
 
+// F_p^3 EqFactored rounds in the register tail of the F_p^3 product
+// (k_tail_reg3<2>) over p and EQ, as the base-field form above.  p enters
+// with its pending fold; k_eq_expand_dup writes EQ at p's level 2 L1 as two
+// copies of eq_l x eq_r at level L1, so the tail's pending fold
+// (lo + c (hi - lo) with hi = lo) leaves EQ as it is.
+__global__ __launch_bounds__(kBlock) void k_eq_expand_dup(View<Fp3> eql, View<Fp3> eqr, unsigned m_log, size_t L1,
+                                                          View<Fp3> eqf, uint64_t w) {
+  const size_t mask = ((size_t)1 << m_log) - 1;
+  for (size_t e = gtid(); e < L1; e += (size_t)gridDim.x * blockDim.x) {
+    const Fp3 x = gl::mul(eql.ld(e >> m_log), eqr.ld(e & mask), w);
+    eqf.st(e, x);
+    eqf.st(e + L1, x);
+  }
+}
+
+// k_eq_post_fold over F_p^3 tables; leaves the last challenge (3 words) in
+// cur.
+__global__ __launch_bounds__(kReduceBlock) void k_eq_post_fold3(View<Fp3> eql, size_t eql_len, View<Fp3> eqr,
+                                                                size_t eqr_len, const uint64_t* chals, int rounds,
+                                                                int left, uint64_t* cur, uint64_t w) {
+  for (int r = 0; r < rounds; r++) {
+    const Fp3 c = gl::load_words<Fp3>(chals + 3 * r);
+    const View<Fp3>& t = r < left ? eql : eqr;
+    size_t& len = r < left ? eql_len : eqr_len;
+    const size_t h = len / 2;
+    for (size_t k = threadIdx.x; k < h; k += blockDim.x) t.st(k, fold1<Fp3>(t.ld(k), t.ld(k + h), c, w));
+    len = h;
+    __syncthreads();
+  }
+  if (threadIdx.x < 3 && rounds > 0) cur[threadIdx.x] = chals[3 * (rounds - 1) + threadIdx.x];
+}
+
 template <class FI, class F>
 int eq_device_run(effsc_hip_prover* h, size_t v) {
   const int E = h->ext;
@@ -2337,6 +2369,37 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     have = true;
     cur_in = h->d_cur;  // k_eq_reduce_fs wrote this round's challenge there
     j++;
+  }
+  if constexpr (std::is_same<F, Fp3>::value) {
+    // the loop above leaves L = 1024 with a pending fold whenever it ran a
+    // fold (v >= 11)
+    if (have && h->in_work && j < v && L == 1024 && !std::getenv("EFFSC_HIP_EQ_TAIL")) {
+      // the remaining v - j = log2(L / 2) rounds in the F_p^3 register tail
+      if (!h->eq_full) TRY(alloc_dev(&h->eq_full, (size_t)1 << 14));
+      const size_t L1 = L / 2;
+      size_t m_log = 0;
+      while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+      const View<Fp3> eqf{h->eq_full, 1024};  // three planes of 2 L1 = 1024 elements
+      hipLaunchKernelGGL(k_eq_expand_dup, dim3(blocks_for(h, L1)), dim3(kBlock), 0, h->stream, eql, eqr,
+                         (unsigned)m_log, L1, eqf, h->w);
+      HIPCHK(hipGetLastError());
+      Views<Fp3, 2> tv{};
+      tv.t[0] = View<Fp3>{h->work[0], h->work_stride};
+      tv.t[1] = eqf;
+      const int rounds = (int)(v - j);
+      hipLaunchKernelGGL((k_tail_reg3<2, 2>), dim3(1), dim3(kTailReg3Threads), 0, h->stream, tv, rounds, h->d_fs,
+                         h->d_polys + j * 2 * 3, h->d_chals + j * 3, cur_in, h->d_fin, h->w);
+      HIPCHK(hipGetLastError());
+      const int left = j < v_l ? (int)(v_l - j) : 0;
+      hipLaunchKernelGGL(k_eq_post_fold3, dim3(1), dim3(kReduceBlock), 0, h->stream, eql, h->eq_l_len, eqr,
+                         h->eq_r_len, h->d_chals + j * 3, rounds, left, h->d_cur, h->w);
+      HIPCHK(hipGetLastError());
+      h->len = 1;
+      h->eq_l_len = 1;
+      h->eq_r_len = 1;
+      h->rounds_elapsed = v;
+      return EFFSC_HIP_OK;
+    }
   }
   // remaining rounds, finalize and final value
   const View<FI> pin{h->in[0], h->in_stride};

@@ -142,13 +142,19 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     # computed on the fly (v = 23 has a pass across the left/right split,
     # v_L = 11), then the register tail over p and EQ from 2^11..2^14
     # elements; over F_p^3 one round per launch, split-value in the left
-    # phase; EFFSC_HIP_EQ_DIRECT computes EQ per element in every pass,
+    # phase, then the F_p^3 register tail over p and EQ from 2^10 elements;
+    # EFFSC_HIP_EQ_DIRECT computes EQ per element in every pass,
     # EFFSC_HIP_EQ_TAIL keeps the one-workgroup eq tail,
     # EFFSC_HIP_ONE_ROUND keeps one round per launch
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
+    if ext == 3:
+        monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
+        monkeypatch.delenv("EFFSC_HIP_EQ_TAIL")
+        check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
     if ext == 1:
         # the one-workgroup eq tail instead of the register tail over p and EQ
         monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
