@@ -295,6 +295,10 @@ int reduce(effsc_hip_prover* h, unsigned nblocks) {
   return EFFSC_HIP_OK;
 }
 
+// planes of at least this many bytes are streamed past the caches by the
+// passes that read or write them once (nontemporal loads and stores)
+constexpr double kNtMinBytes = 134217728.0;
+
 template <class FI, class FO, int K>
 struct Ops {
   static int eval(effsc_hip_prover* h, uint64_t* const* planes, size_t stride, size_t half, size_t paired) {
@@ -328,7 +332,29 @@ struct Ops {
     if constexpr (FO::D == 3 && K >= 3)
       vec = vec && q >= ((size_t)1 << 18) && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
     TRY(prof_begin(h));
-    if constexpr (FO::D == 3 && K >= 3) {
+    if constexpr (std::is_same<FI, Fp>::value && std::is_same<FO, Fp3>::value && K == 3) {
+      // base tables folded into F_p^3: sums of base-field coefficients of
+      // c^k (EFFSC_HIP_FSEQ_GENERIC keeps k_fused_seq)
+      const bool generic = std::getenv("EFFSC_HIP_FSEQ_GENERIC") != nullptr;
+      // planes of 128 MiB and more: nontemporal (v = 24: 1.424-1.432 against
+      // 1.446-1.452 ms per proof)
+      const bool nt = 8.0 * 4.0 * (double)q >= kNtMinBytes;
+      if (vec && nt && !generic)
+        hipLaunchKernelGGL((k_fused_base3<2, true>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in, out, c,
+                           cptr, q, h->w, h->red);
+      else if (vec && !generic)
+        hipLaunchKernelGGL((k_fused_base3<2>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in, out, c, cptr,
+                           q, h->w, h->red);
+      else if (!generic)
+        hipLaunchKernelGGL((k_fused_base3<1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in, out, c, cptr, q,
+                           h->w, h->red);
+      else if (vec)
+        hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 2>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in,
+                           out, c, cptr, q, h->w, h->red);
+      else
+        hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in, out,
+                           c, cptr, q, h->w, h->red);
+    } else if constexpr (FO::D == 3 && K >= 3) {
       if (vec)
         hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 2>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in,
                            out, c, cptr, q, h->w, h->red);
@@ -588,7 +614,6 @@ int gkr_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
 int gkr_final_value(effsc_hip_prover* h, uint64_t* out);
 
 // ---------------------------------------------------------- two-round passes
-constexpr double kNtMinBytes = 134217728.0;
 
 template <int K, int P>
 int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {

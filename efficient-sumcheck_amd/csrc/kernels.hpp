@@ -90,6 +90,11 @@ struct View<Fp3> {
     *reinterpret_cast<ulonglong2*>(p + stride + i) = make_ulonglong2(a.c1, b.c1);
     *reinterpret_cast<ulonglong2*>(p + 2 * stride + i) = make_ulonglong2(a.c2, b.c2);
   }
+  __device__ __forceinline__ void st2_nt(size_t i, Fp3 a, Fp3 b) const {
+    __builtin_nontemporal_store(v2u64{a.c0, b.c0}, reinterpret_cast<v2u64*>(p + i));
+    __builtin_nontemporal_store(v2u64{a.c1, b.c1}, reinterpret_cast<v2u64*>(p + stride + i));
+    __builtin_nontemporal_store(v2u64{a.c2, b.c2}, reinterpret_cast<v2u64*>(p + 2 * stride + i));
+  }
   __device__ __forceinline__ void ld2_nt(size_t i, Fp3& a, Fp3& b) const {
     v2u64 q0 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + i));
     v2u64 q1 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + stride + i));
@@ -418,6 +423,98 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) 
       for (int t = 0; t < K; t++) acc[t] = gl::add(acc[t], p[e][t]);
   }
   finish<FO, K>(acc, r);
+}
+
+// The first fused round of a product of three base-field tables under an
+// F_p^3 challenge c.  Every folded value is then affine in c with base-field
+// coefficients: lo' = x0 + c (x2 - x0), and the evaluation points of a pair
+// are h(0) = lo', h(inf) = hi' - lo', h(2) = 2 hi' - lo', each a_i + b_i c.
+// Their product over the three tables is e0 + e1 c + e2 c^2 + e3 c^3 with
+//   e0 = a0 a1 a2,  e1 = b0 a1 a2 + a0 (a1 b2 + b1 a2),
+//   e2 = b0 (a1 b2 + b1 a2) + a0 b1 b2,  e3 = b0 b1 b2,
+// so the kernel sums e_k over its pairs in the base field (10 products per
+// point instead of two F_p^3 products) and multiplies by c^k once.  The sums
+// are the same field elements as k_fused_seq's; the folded tables are
+// written as by k_fused_seq.  NT: nontemporal loads and stores (VEC = 2),
+// for planes too large to be reused from the caches.
+template <int VEC, bool NT = false>
+__global__ __launch_bounds__(kBlock) void k_fused_base3(Views<Fp, 3> in, Views<Fp3, 3> out, Fp3 c, const uint64_t* cptr,
+                                                        size_t q, uint64_t w, Reduce r) {
+  if (cptr) c = gl::load_words<Fp3>(cptr);
+  Fp e[3][4];
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) e[t][k] = gl::zero<Fp>();
+  const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
+  for (size_t k = gtid() * VEC; k < q; k += step) {
+    Fp a[VEC][3][3], b[VEC][3][3];  // [element][point][table]
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      Fp x0[VEC], x1[VEC], x2[VEC], x3[VEC];
+      if constexpr (VEC == 2 && NT) {
+        in.t[i].ld2_nt(k, x0[0], x0[1]);
+        in.t[i].ld2_nt(k + q, x1[0], x1[1]);
+        in.t[i].ld2_nt(k + 2 * q, x2[0], x2[1]);
+        in.t[i].ld2_nt(k + 3 * q, x3[0], x3[1]);
+      } else if constexpr (VEC == 2) {
+        in.t[i].ld2(k, x0[0], x0[1]);
+        in.t[i].ld2(k + q, x1[0], x1[1]);
+        in.t[i].ld2(k + 2 * q, x2[0], x2[1]);
+        in.t[i].ld2(k + 3 * q, x3[0], x3[1]);
+      } else {
+        x0[0] = in.t[i].ld(k);
+        x1[0] = in.t[i].ld(k + q);
+        x2[0] = in.t[i].ld(k + 2 * q);
+        x3[0] = in.t[i].ld(k + 3 * q);
+      }
+      Fp3 lo[VEC], hi[VEC];
+#pragma unroll
+      for (int u = 0; u < VEC; u++) {
+        lo[u] = fold1<Fp3>(x0[u], x2[u], c, w);
+        hi[u] = fold1<Fp3>(x1[u], x3[u], c, w);
+        const Fp dl = gl::sub(x2[u], x0[u]), dh = gl::sub(x3[u], x1[u]);
+        const Fp ai = gl::sub(x1[u], x0[u]), bi = gl::sub(dh, dl);
+        a[u][0][i] = x0[u];
+        b[u][0][i] = dl;
+        a[u][1][i] = ai;
+        b[u][1][i] = bi;
+        a[u][2][i] = gl::add(x1[u], ai);  // 2 x1 - x0
+        b[u][2][i] = gl::add(dh, bi);     // 2 dh - dl
+      }
+      if constexpr (VEC == 2 && NT) {
+        out.t[i].st2_nt(k, lo[0], lo[1]);
+        out.t[i].st2_nt(k + q, hi[0], hi[1]);
+      } else if constexpr (VEC == 2) {
+        out.t[i].st2(k, lo[0], lo[1]);
+        out.t[i].st2(k + q, hi[0], hi[1]);
+      } else {
+        out.t[i].st(k, lo[0]);
+        out.t[i].st(k + q, hi[0]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < VEC; u++)
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const Fp* at = a[u][t];
+        const Fp* bt = b[u][t];
+        const Fp a12 = gl::mul(at[1], at[2], w), b12 = gl::mul(bt[1], bt[2], w);
+        const Fp m12 = gl::add(gl::mul(at[1], bt[2], w), gl::mul(bt[1], at[2], w));
+        e[t][0] = gl::add(e[t][0], gl::mul(at[0], a12, w));
+        e[t][1] = gl::add(e[t][1], gl::add(gl::mul(bt[0], a12, w), gl::mul(at[0], m12, w)));
+        e[t][2] = gl::add(e[t][2], gl::add(gl::mul(bt[0], m12, w), gl::mul(at[0], b12, w)));
+        e[t][3] = gl::add(e[t][3], gl::mul(bt[0], b12, w));
+      }
+  }
+  const Fp3 c2 = gl::mul(c, c, w), c3 = gl::mul(c2, c, w);
+  Fp3 acc[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    acc[t] = gl::add(gl::add(gl::lift_to<Fp3>(e[t][0]), gl::mul(c, e[t][1], w)),
+                     gl::add(gl::mul(c2, e[t][2], w), gl::mul(c3, e[t][3], w)));
+  }
+  finish<Fp3, 3>(acc, r);
 }
 #pragma clang diagnostic pop
 

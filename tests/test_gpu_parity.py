@@ -431,6 +431,18 @@ def test_ext3_register_tail_parity(kind, input_ext, v, tail, monkeypatch):
     assert plain.round_polys == pg.round_polys and plain.final_value == pg.final_value
 
 
+@pytest.mark.parametrize("v", [3, 9, 12, 19, 20])
+def test_base_input_ext3_product_first_fold(v, monkeypatch):
+    # round 1 of a product of three base tables under F_p^3 challenges sums
+    # base-field coefficients of c^k (one or two quadruples per thread, from
+    # 2^18 quadruples at v = 20); EFFSC_HIP_FSEQ_GENERIC keeps the F_p^3 form
+    _, pg = check_proof("p3", 3, 1, 1 << v, v, 5200 + v, transcript="rng")
+    _, ps = check_proof("p3", 3, 1, 1 << v, v, 5200 + v, transcript="shake")
+    monkeypatch.setenv("EFFSC_HIP_FSEQ_GENERIC", "1")
+    _, pgen = check_proof("p3", 3, 1, 1 << v, v, 5200 + v, transcript="rng")
+    assert pgen.round_polys == pg.round_polys and pgen.final_value == pg.final_value
+
+
 @pytest.mark.parametrize("tail", [2, 1 << 14])
 def test_device_loop_partial_then_continue(tail, monkeypatch):
     monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
