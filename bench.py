@@ -273,7 +273,7 @@ def bench_single(args, workload):
         names[0] = "k_eq_round (fold p with the challenge + evaluate with eq weights)"
         names[1] = "k_eq_round (round 0: evaluate with eq weights)"
     elif spec["kind"] == "product" and ext == 3:
-        names[0] = "k_fused_seq (fold + evaluate, one F_p^3 table at a time)"
+        names[0] = "k_fused_base3 (round 1) and k_fused_seq (fold + evaluate, one F_p^3 table at a time)"
     elif spec["kind"] == "gkr":
         names[0] = "k_gkr_fused_fw (fold add and mult, factored witness, evaluate the wire)"
         names[1] = "k_gkr_eval_fw (round 0: evaluate the wire, factored witness)"
