@@ -425,6 +425,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) 
   finish<FO, K>(acc, r);
 }
 
+// Base-field terms of a product of three values a_i + b_i c (i = 0..2):
+// adds e0 = a0 a1 a2, e1 = b0 a1 a2 + a0 (a1 b2 + b1 a2),
+// e2 = b0 (a1 b2 + b1 a2) + a0 b1 b2 and e3 = b0 b1 b2 into e[0..3].
+__device__ __forceinline__ void base3_terms(const Fp* a, const Fp* b, Fp* e) {
+  const Fp a12 = gl::mul(a[1], a[2], 0), b12 = gl::mul(b[1], b[2], 0);
+  const Fp m12 = gl::add(gl::mul(a[1], b[2], 0), gl::mul(b[1], a[2], 0));
+  e[0] = gl::add(e[0], gl::mul(a[0], a12, 0));
+  e[1] = gl::add(e[1], gl::add(gl::mul(b[0], a12, 0), gl::mul(a[0], m12, 0)));
+  e[2] = gl::add(e[2], gl::add(gl::mul(b[0], m12, 0), gl::mul(a[0], b12, 0)));
+  e[3] = gl::add(e[3], gl::mul(b[0], b12, 0));
+}
+
+// The evaluation-point coefficients of one base quadruple (x0, x1, x2, x3)
+// of a table folded with c: h(0) = lo', h(inf) = hi' - lo', h(2) = 2 hi' -
+// lo', each a[t] + b[t] c.
+__device__ __forceinline__ void base3_points(Fp x0, Fp x1, Fp x2, Fp x3, Fp* a, Fp* b) {
+  const Fp dl = gl::sub(x2, x0), dh = gl::sub(x3, x1);
+  const Fp ai = gl::sub(x1, x0), bi = gl::sub(dh, dl);
+  a[0] = x0;
+  b[0] = dl;
+  a[1] = ai;
+  b[1] = bi;
+  a[2] = gl::add(x1, ai);  // 2 x1 - x0
+  b[2] = gl::add(dh, bi);  // 2 dh - dl
+}
+
+// sum_k e[t][k] c^k for the three points, added into acc.
+__device__ __forceinline__ void base3_finish(const Fp (*e)[4], Fp3 c, uint64_t w, Fp3* acc) {
+  const Fp3 c2 = gl::mul(c, c, w), c3 = gl::mul(c2, c, w);
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+    acc[t] = gl::add(acc[t], gl::add(gl::add(gl::lift_to<Fp3>(e[t][0]), gl::mul(c, e[t][1], w)),
+                                     gl::add(gl::mul(c2, e[t][2], w), gl::mul(c3, e[t][3], w))));
+}
+
 // The first fused round of a product of three base-field tables under an
 // F_p^3 challenge c.  Every folded value is then affine in c with base-field
 // coefficients: lo' = x0 + c (x2 - x0), and the evaluation points of a pair
@@ -473,14 +508,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_base3(Views<Fp, 3> in, Views<F
       for (int u = 0; u < VEC; u++) {
         lo[u] = fold1<Fp3>(x0[u], x2[u], c, w);
         hi[u] = fold1<Fp3>(x1[u], x3[u], c, w);
-        const Fp dl = gl::sub(x2[u], x0[u]), dh = gl::sub(x3[u], x1[u]);
-        const Fp ai = gl::sub(x1[u], x0[u]), bi = gl::sub(dh, dl);
-        a[u][0][i] = x0[u];
-        b[u][0][i] = dl;
-        a[u][1][i] = ai;
-        b[u][1][i] = bi;
-        a[u][2][i] = gl::add(x1[u], ai);  // 2 x1 - x0
-        b[u][2][i] = gl::add(dh, bi);     // 2 dh - dl
+        Fp pa[3], pb[3];
+        base3_points(x0[u], x1[u], x2[u], x3[u], pa, pb);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+          a[u][t][i] = pa[t];
+          b[u][t][i] = pb[t];
+        }
       }
       if constexpr (VEC == 2 && NT) {
         out.t[i].st2_nt(k, lo[0], lo[1]);
@@ -496,24 +530,10 @@ __global__ __launch_bounds__(kBlock) void k_fused_base3(Views<Fp, 3> in, Views<F
 #pragma unroll
     for (int u = 0; u < VEC; u++)
 #pragma unroll
-      for (int t = 0; t < 3; t++) {
-        const Fp* at = a[u][t];
-        const Fp* bt = b[u][t];
-        const Fp a12 = gl::mul(at[1], at[2], w), b12 = gl::mul(bt[1], bt[2], w);
-        const Fp m12 = gl::add(gl::mul(at[1], bt[2], w), gl::mul(bt[1], at[2], w));
-        e[t][0] = gl::add(e[t][0], gl::mul(at[0], a12, w));
-        e[t][1] = gl::add(e[t][1], gl::add(gl::mul(bt[0], a12, w), gl::mul(at[0], m12, w)));
-        e[t][2] = gl::add(e[t][2], gl::add(gl::mul(bt[0], m12, w), gl::mul(at[0], b12, w)));
-        e[t][3] = gl::add(e[t][3], gl::mul(bt[0], b12, w));
-      }
+      for (int t = 0; t < 3; t++) base3_terms(a[u][t], b[u][t], e[t]);
   }
-  const Fp3 c2 = gl::mul(c, c, w), c3 = gl::mul(c2, c, w);
-  Fp3 acc[3];
-#pragma unroll
-  for (int t = 0; t < 3; t++) {
-    acc[t] = gl::add(gl::add(gl::lift_to<Fp3>(e[t][0]), gl::mul(c, e[t][1], w)),
-                     gl::add(gl::mul(c2, e[t][2], w), gl::mul(c3, e[t][3], w)));
-  }
+  Fp3 acc[3] = {gl::zero<Fp3>(), gl::zero<Fp3>(), gl::zero<Fp3>()};
+  base3_finish(e, c, w, acc);
   finish<Fp3, 3>(acc, r);
 }
 #pragma clang diagnostic pop

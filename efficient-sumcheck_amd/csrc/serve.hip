@@ -118,6 +118,39 @@ __device__ __forceinline__ void serve_chunk(const SInst* ip, uint32_t j, uint64_
   // j = 1, else the work table, in place) and evaluate the folded pairs
   const uint64_t q = (uint64_t)1 << (v - j - 1);
   const FW c = gl::load_words<FW>(ip->cur);
+  if constexpr (FW::D == 3 && K == 3) {
+    if (j == 1) {
+      // base inputs folded into F_p^3: the evaluation as base-field
+      // coefficients of c^k, converted once (as k_fused_base3)
+      Fp e[3][4];
+#pragma unroll
+      for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) e[t][m] = gl::zero<Fp>();
+      for (uint64_t k = begin + threadIdx.x; k < end; k += blockDim.x) {
+        Fp a[3][3], b[3][3];  // [point][table]
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          View<FW> d{ip->work[i], ip->ws};
+          const uint64_t* s = ip->in[i];
+          const Fp x0 = {s[k]}, x1 = {s[k + q]}, x2 = {s[k + 2 * q]}, x3 = {s[k + 3 * q]};
+          d.st(k, fold1<FW>(x0, x2, c, w));
+          d.st(k + q, fold1<FW>(x1, x3, c, w));
+          Fp pa[3], pb[3];
+          base3_points(x0, x1, x2, x3, pa, pb);
+#pragma unroll
+          for (int t = 0; t < 3; t++) {
+            a[t][i] = pa[t];
+            b[t][i] = pb[t];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) base3_terms(a[t], b[t], e[t]);
+      }
+      base3_finish(e, c, w, acc);
+      return;
+    }
+  }
   if constexpr (FW::D == 3 && K >= 3) {
     // one table at a time with running products (as k_fused_seq): the
     // all-tables-at-once form needs more registers than two waves allow
