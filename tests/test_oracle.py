@@ -339,3 +339,44 @@ def test_lsb_equals_msb_on_bit_reversed_table():
         lsb = O.Prover.inner_product_lsb(a, b).sumcheck(v, O.Transcript.shake(b"r"))
         msb = O.Prover.inner_product(a[rev], b[rev]).sumcheck(v, O.Transcript.shake(b"r"))
         assert lsb == msb
+
+
+def test_base_coefficient_form_of_first_fold():
+    # Round 1 of a product of three base tables under an F_p^3 challenge c:
+    # the GPU sums base-field coefficients e_k of c^k (k_fused_base3).  The
+    # same formulas, restated here, equal the F_p^3 evaluation from the
+    # definition: fold each pair with c, then h(0), h(inf), h(2) of the
+    # product of the three folded tables, summed over the quadruples.
+    F = B.Ext(3)
+    rng = random.Random(77)
+    for trial in range(20):
+        c = tuple(rng.randrange(P) for _ in range(3))
+        q = rng.randrange(1, 9)
+        quads = [[[rng.randrange(P) for _ in range(4)] for _ in range(3)] for _ in range(q)]
+        if trial == 0:
+            quads[0] = [[0, P - 1, 1, P - 1] for _ in range(3)]  # edge values
+        want = [F.zero()] * 3
+        e = [[0] * 4 for _ in range(3)]
+        for quad in quads:
+            prods = [F.one()] * 3
+            ab = []
+            for x0, x1, x2, x3 in quad:
+                lo = F.add(F.lift(x0), F.scale(c, (x2 - x0) % P))
+                hi = F.add(F.lift(x1), F.scale(c, (x3 - x1) % P))
+                d = F.sub(hi, lo)
+                for t, h in enumerate((lo, d, F.add(hi, d))):
+                    prods[t] = F.mul(prods[t], h)
+                dl, dh = (x2 - x0) % P, (x3 - x1) % P
+                ai, bi = (x1 - x0) % P, (dh - dl) % P
+                ab.append(((x0, dl), (ai, bi), ((x1 + ai) % P, (dh + bi) % P)))
+            want = [F.add(want[t], prods[t]) for t in range(3)]
+            for t in range(3):
+                (a0, b0), (a1, b1), (a2, b2) = (ab[i][t] for i in range(3))
+                a12, b12, m12 = a1 * a2 % P, b1 * b2 % P, (a1 * b2 + b1 * a2) % P
+                for k, term in enumerate((a0 * a12, b0 * a12 + a0 * m12, b0 * m12 + a0 * b12, b0 * b12)):
+                    e[t][k] = (e[t][k] + term) % P
+        c2 = F.mul(c, c)
+        c3 = F.mul(c2, c)
+        got = [F.add(F.add(F.lift(e[t][0]), F.scale(c, e[t][1])), F.add(F.scale(c2, e[t][2]), F.scale(c3, e[t][3])))
+               for t in range(3)]
+        assert got == want
