@@ -1,4 +1,5 @@
-// Goldilocks field p = 2^64 - 2^32 + 1 and its cubic extension, canonical form,
+// Goldilocks field p = 2^64 - 2^32 + 1 and its quadratic and cubic extensions
+// (F_p[X]/(X^2 - w), F_p[X]/(X^3 - w), src/tests/fields.rs:41-100), canonical form,
 // shared by the gfx950 kernels and the host protocol code.
 //
 // Every stored value is canonical (in [0, p)).  The reference keeps arkworks
@@ -86,11 +87,18 @@ GLHD uint64_t pow(uint64_t a, uint64_t e) {
 GLHD uint64_t inv(uint64_t a) { return pow(a, P - 2); }
 
 // ---------------------------------------------------------------------------
-// Field element types.  Fp: base field.  Fp3: F_p[X]/(X^3 - w), w runtime.
+// Field element types.  Fp: base field.  Fp2: F_p[X]/(X^2 - w).  Fp3:
+// F_p[X]/(X^3 - w).  w is a runtime value (the reference's default is
+// F64::from_raw(7) for both extensions, src/tests/fields.rs:49,80).
 // ---------------------------------------------------------------------------
 struct Fp {
   uint64_t v;
   static constexpr int D = 1;
+};
+
+struct Fp2 {
+  uint64_t c0, c1;
+  static constexpr int D = 2;
 };
 
 struct Fp3 {
@@ -101,6 +109,8 @@ struct Fp3 {
 GLHD Fp add(Fp a, Fp b) { return {add(a.v, b.v)}; }
 GLHD Fp sub(Fp a, Fp b) { return {sub(a.v, b.v)}; }
 GLHD Fp mul(Fp a, Fp b, uint64_t) { return {mul(a.v, b.v)}; }
+GLHD Fp2 add(Fp2 a, Fp2 b) { return {add(a.c0, b.c0), add(a.c1, b.c1)}; }
+GLHD Fp2 sub(Fp2 a, Fp2 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1)}; }
 GLHD Fp3 add(Fp3 a, Fp3 b) { return {add(a.c0, b.c0), add(a.c1, b.c1), add(a.c2, b.c2)}; }
 GLHD Fp3 sub(Fp3 a, Fp3 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1), sub(a.c2, b.c2)}; }
 
@@ -132,6 +142,27 @@ GLHD void mac(Acc3& a, uint64_t x, uint64_t y) {
 // 2^128 = -2^32 (mod p)
 GLHD uint64_t reduce(const Acc3& a) { return sub(reduce128(a.lo, a.hi), (uint64_t)a.top << 32); }
 
+// A multiplier prepared for products mod X^2 - w: b and w b1.
+struct Fp2Pre {
+  uint64_t b0, b1, wb1;
+};
+GLHD Fp2Pre prepare(Fp2 b, uint64_t w) { return {b.c0, b.c1, mul(w, b.c1)}; }
+
+// (a0 + a1 X)(b0 + b1 X) mod X^2 - w = (a0 b0 + a1 w b1) + (a0 b1 + a1 b0) X,
+// schoolbook with one reduction per coefficient: 4 products and 2
+// reductions (each coefficient a sum of two unreduced products < 2^129).
+GLHD Fp2 mul(Fp2 a, const Fp2Pre& b) {
+  Acc3 r0 = prod(a.c0, b.b0), r1 = prod(a.c0, b.b1);
+  mac(r0, a.c1, b.wb1);
+  mac(r1, a.c1, b.b0);
+  return {reduce(r0), reduce(r1)};
+}
+
+// General product: the first operand is prepared (1 product by w), then the
+// prepared form; a loop-invariant first operand (a fold's challenge) is
+// prepared once, outside the loop.
+GLHD Fp2 mul(Fp2 a, Fp2 b, uint64_t w) { return mul(b, prepare(a, w)); }
+
 // A multiplier prepared for products mod X^3 - w: b and w b1, w b2.
 struct Fp3Pre {
   uint64_t b0, b1, b2, wb1, wb2;
@@ -158,6 +189,8 @@ GLHD Fp3 mul(Fp3 a, const Fp3Pre& b) {
 GLHD Fp3 mul(Fp3 a, Fp3 b, uint64_t w) { return mul(b, prepare(a, w)); }
 
 // Mixed products: extension element times base element.
+GLHD Fp2 mul(Fp2 a, Fp b, uint64_t) { return {mul(a.c0, b.v), mul(a.c1, b.v)}; }
+GLHD Fp2 mul(Fp a, Fp2 b, uint64_t) { return {mul(a.v, b.c0), mul(a.v, b.c1)}; }
 GLHD Fp3 mul(Fp3 a, Fp b, uint64_t) { return {mul(a.c0, b.v), mul(a.c1, b.v), mul(a.c2, b.v)}; }
 GLHD Fp3 mul(Fp a, Fp3 b, uint64_t) { return {mul(a.v, b.c0), mul(a.v, b.c1), mul(a.v, b.c2)}; }
 
@@ -167,10 +200,14 @@ GLHD F lift(Fp x);
 template <>
 GLHD Fp lift<Fp>(Fp x) { return x; }
 template <>
+GLHD Fp2 lift<Fp2>(Fp x) { return {x.v, 0}; }
+template <>
 GLHD Fp3 lift<Fp3>(Fp x) { return {x.v, 0, 0}; }
 
 // Lift into a (possibly) larger field.
 GLHD void lift_into(Fp x, Fp& o) { o = x; }
+GLHD void lift_into(Fp x, Fp2& o) { o = {x.v, 0}; }
+GLHD void lift_into(Fp2 x, Fp2& o) { o = x; }
 GLHD void lift_into(Fp x, Fp3& o) { o = {x.v, 0, 0}; }
 GLHD void lift_into(Fp3 x, Fp3& o) { o = x; }
 template <class FO, class FI>
@@ -185,6 +222,8 @@ GLHD F zero();
 template <>
 GLHD Fp zero<Fp>() { return {0}; }
 template <>
+GLHD Fp2 zero<Fp2>() { return {0, 0}; }
+template <>
 GLHD Fp3 zero<Fp3>() { return {0, 0, 0}; }
 
 template <class F>
@@ -192,12 +231,22 @@ GLHD F one();
 template <>
 GLHD Fp one<Fp>() { return {1}; }
 template <>
+GLHD Fp2 one<Fp2>() { return {1, 0}; }
+template <>
 GLHD Fp3 one<Fp3>() { return {1, 0, 0}; }
 
 GLHD bool is_zero(Fp a) { return a.v == 0; }
+GLHD bool is_zero(Fp2 a) { return (a.c0 | a.c1) == 0; }
 GLHD bool is_zero(Fp3 a) { return (a.c0 | a.c1 | a.c2) == 0; }
 
 GLHD Fp inv(Fp a, uint64_t) { return {a.v ? inv(a.v) : 0}; }
+// (a0 + a1 X)^-1 = (a0 - a1 X) / (a0^2 - w a1^2), the norm being nonzero for
+// a nonzero element when w is a non-residue.
+GLHD Fp2 inv(Fp2 a, uint64_t w) {
+  const uint64_t n = sub(mul(a.c0, a.c0), mul(w, mul(a.c1, a.c1)));
+  const uint64_t ni = n ? inv(n) : 0;
+  return {mul(a.c0, ni), mul(neg(a.c1), ni)};
+}
 // Inverse through the adjugate of the multiplication matrix.
 GLHD Fp3 inv(Fp3 a, uint64_t w) {
   uint64_t t0 = sub(mul(a.c0, a.c0), mul(w, mul(a.c1, a.c2)));
@@ -210,6 +259,10 @@ GLHD Fp3 inv(Fp3 a, uint64_t w) {
 
 // Component access (element -> words).
 GLHD void store_words(Fp a, uint64_t* w) { w[0] = a.v; }
+GLHD void store_words(Fp2 a, uint64_t* w) {
+  w[0] = a.c0;
+  w[1] = a.c1;
+}
 GLHD void store_words(Fp3 a, uint64_t* w) {
   w[0] = a.c0;
   w[1] = a.c1;
@@ -219,6 +272,8 @@ template <class F>
 GLHD F load_words(const uint64_t* w);
 template <>
 GLHD Fp load_words<Fp>(const uint64_t* w) { return {canon(w[0])}; }
+template <>
+GLHD Fp2 load_words<Fp2>(const uint64_t* w) { return {canon(w[0]), canon(w[1])}; }
 template <>
 GLHD Fp3 load_words<Fp3>(const uint64_t* w) { return {canon(w[0]), canon(w[1]), canon(w[2])}; }
 

@@ -31,6 +31,7 @@
 namespace effsc_hip {
 
 using gl::Fp;
+using gl::Fp2;
 using gl::Fp3;
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
@@ -38,8 +39,9 @@ constexpr int kMaxK = 4;             // tables per product
 constexpr int kMaxWaves = 16;              // block_reduce supports up to 1024 threads
 
 // ---------------------------------------------------------------------------
-// Table views.  Base tables are one u64 plane; Fp3 tables are three planes
-// (structure of arrays) `stride` words apart, so every load is coalesced.
+// Table views.  Base tables are one u64 plane; Fp2 and Fp3 tables are two and
+// three planes (structure of arrays) `stride` words apart, so every load is
+// coalesced.
 // ---------------------------------------------------------------------------
 typedef uint64_t v2u64 __attribute__((ext_vector_type(2)));
 
@@ -65,6 +67,37 @@ struct View<Fp> {
     v2u64 q = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + i));
     a.v = q.x;
     b.v = q.y;
+  }
+};
+
+template <>
+struct View<Fp2> {
+  uint64_t* p;
+  size_t stride;
+  __device__ __forceinline__ Fp2 ld(size_t i) const { return {p[i], p[stride + i]}; }
+  __device__ __forceinline__ void st(size_t i, Fp2 x) const {
+    p[i] = x.c0;
+    p[stride + i] = x.c1;
+  }
+  __device__ __forceinline__ void ld2(size_t i, Fp2& a, Fp2& b) const {
+    ulonglong2 q0 = *reinterpret_cast<const ulonglong2*>(p + i);
+    ulonglong2 q1 = *reinterpret_cast<const ulonglong2*>(p + stride + i);
+    a = {q0.x, q1.x};
+    b = {q0.y, q1.y};
+  }
+  __device__ __forceinline__ void st2(size_t i, Fp2 a, Fp2 b) const {
+    *reinterpret_cast<ulonglong2*>(p + i) = make_ulonglong2(a.c0, b.c0);
+    *reinterpret_cast<ulonglong2*>(p + stride + i) = make_ulonglong2(a.c1, b.c1);
+  }
+  __device__ __forceinline__ void st2_nt(size_t i, Fp2 a, Fp2 b) const {
+    __builtin_nontemporal_store(v2u64{a.c0, b.c0}, reinterpret_cast<v2u64*>(p + i));
+    __builtin_nontemporal_store(v2u64{a.c1, b.c1}, reinterpret_cast<v2u64*>(p + stride + i));
+  }
+  __device__ __forceinline__ void ld2_nt(size_t i, Fp2& a, Fp2& b) const {
+    v2u64 q0 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + i));
+    v2u64 q1 = __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(p + stride + i));
+    a = {q0.x, q1.x};
+    b = {q0.y, q1.y};
   }
 };
 
@@ -162,6 +195,7 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
   return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 __device__ __forceinline__ Fp shfl_xor(Fp a, int m) { return {shfl_xor64(a.v, m)}; }
+__device__ __forceinline__ Fp2 shfl_xor(Fp2 a, int m) { return {shfl_xor64(a.c0, m), shfl_xor64(a.c1, m)}; }
 __device__ __forceinline__ Fp3 shfl_xor(Fp3 a, int m) {
   return {shfl_xor64(a.c0, m), shfl_xor64(a.c1, m), shfl_xor64(a.c2, m)};
 }
@@ -170,6 +204,7 @@ constexpr int ilog2c(int p) { return p <= 1 ? 0 : 1 + ilog2c(p / 2); }
 constexpr int pow2_ceil(int k) { return k <= 1 ? 1 : 2 * pow2_ceil((k + 1) / 2); }
 
 __device__ __forceinline__ Fp select(bool c, Fp a, Fp b) { return {c ? a.v : b.v}; }
+__device__ __forceinline__ Fp2 select(bool c, Fp2 a, Fp2 b) { return {c ? a.c0 : b.c0, c ? a.c1 : b.c1}; }
 __device__ __forceinline__ Fp3 select(bool c, Fp3 a, Fp3 b) {
   return {c ? a.c0 : b.c0, c ? a.c1 : b.c1, c ? a.c2 : b.c2};
 }
@@ -178,6 +213,7 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ Fp readlane(Fp a, int l) { return {readlane64(a.v, l)}; }
+__device__ __forceinline__ Fp2 readlane(Fp2 a, int l) { return {readlane64(a.c0, l), readlane64(a.c1, l)}; }
 __device__ __forceinline__ Fp3 readlane(Fp3 a, int l) {
   return {readlane64(a.c0, l), readlane64(a.c1, l), readlane64(a.c2, l)};
 }

@@ -291,11 +291,11 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
 }
 
 constexpr int kTailBlock = 1024;
-// F_p^3 products of two or more tables need more than the 128 VGPRs a
-// 1024-thread workgroup allows (they spilled there): 256 threads.
+// Extension-field products of two or more tables need more than the 128
+// VGPRs a 1024-thread workgroup allows (they spilled there): 256 threads.
 template <class F, int K>
 constexpr int tail_block() {
-  return (F::D == 3 && K >= 2) ? 256 : kTailBlock;
+  return (F::D >= 2 && K >= 2) ? 256 : kTailBlock;
 }
 
 // All remaining rounds in one workgroup.  On entry the K tables have pow2

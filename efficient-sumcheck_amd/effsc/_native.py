@@ -74,6 +74,7 @@ _SIGS = {
     "effsc_hip_profile_read_muls": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "effsc_hip_synchronize": (C.c_int, [C.c_int]),
     "effsc_hip_fold": (C.c_int, [C.c_int, u64p, sz, u64, szp]),
+    "effsc_hip_fold_ext": (C.c_int, [C.c_int, C.c_int, u64, u64p, sz, u64p, szp]),
     "effsc_hip_transcript_new": (vp, [C.c_char_p, sz]),
     "effsc_hip_transcript_new_test": (vp, [u64]),
     "effsc_hip_transcript_send": (C.c_int, [vp, C.c_int, u64p]),

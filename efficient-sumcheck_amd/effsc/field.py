@@ -1,7 +1,7 @@
 """Goldilocks field helpers for the host side (src/field.rs:151, src/tests/fields.rs:26-100).
 
-Elements: an int in [0, p) for the base field; a tuple (c0, c1, c2) for
-F_p[X]/(X^3 - w).  Words crossing the C ABI are canonical unless a prover
+Elements: an int in [0, p) for the base field; a tuple (c0, c1) for
+F_p[X]/(X^2 - w) and (c0, c1, c2) for F_p[X]/(X^3 - w).  Words crossing the C ABI are canonical unless a prover
 was built with repr=MONT64.
 """
 from __future__ import annotations
@@ -23,11 +23,12 @@ def from_mont(m: int) -> int:
 
 
 class Field:
-    """Arithmetic of F_p (ext = 1) or F_p[X]/(X^3 - w) (ext = 3)."""
+    """Arithmetic of F_p (ext = 1), F_p[X]/(X^2 - w) (ext = 2) or
+    F_p[X]/(X^3 - w) (ext = 3)."""
 
     def __init__(self, ext: int = 1, w: int = W_FROM_RAW7):
-        if ext not in (1, 3):
-            raise ValueError("ext must be 1 or 3")
+        if ext not in (1, 2, 3):
+            raise ValueError("ext must be 1, 2 or 3")
         self.ext = ext
         self.w = w % P
 
@@ -40,16 +41,16 @@ class Field:
     def elem(self, words):
         if self.ext == 1:
             return int(words[0])
-        return tuple(int(c) for c in words[:3])
+        return tuple(int(c) for c in words[:self.ext])
 
     def zero(self):
-        return 0 if self.ext == 1 else (0, 0, 0)
+        return 0 if self.ext == 1 else (0,) * self.ext
 
     def one(self):
-        return 1 if self.ext == 1 else (1, 0, 0)
+        return 1 if self.ext == 1 else (1,) + (0,) * (self.ext - 1)
 
     def from_u64(self, x):
-        return x % P if self.ext == 1 else (x % P, 0, 0)
+        return x % P if self.ext == 1 else (x % P,) + (0,) * (self.ext - 1)
 
     def add(self, a, b):
         if self.ext == 1:
@@ -67,6 +68,9 @@ class Field:
     def mul(self, a, b):
         if self.ext == 1:
             return a * b % P
+        if self.ext == 2:
+            (a0, a1), (b0, b1) = a, b
+            return ((a0 * b0 + self.w * a1 * b1) % P, (a0 * b1 + a1 * b0) % P)
         a0, a1, a2 = a
         b0, b1, b2 = b
         w = self.w
@@ -77,6 +81,11 @@ class Field:
     def inv(self, a):
         if self.ext == 1:
             return pow(a, P - 2, P) if a % P else 0
+        if self.ext == 2:
+            a0, a1 = a
+            n = (a0 * a0 - self.w * a1 * a1) % P
+            ni = pow(n, P - 2, P) if n else 0
+            return (a0 * ni % P, (-a1) * ni % P)
         a0, a1, a2 = a
         w = self.w
         t0 = (a0 * a0 - w * a1 * a2) % P

@@ -6,7 +6,7 @@ Every method forwards to the C ABI (include/effsc_hip.h); the tables live in
 device memory for the prover's lifetime.
 
 Inputs: numpy uint64 arrays of words (AoS, `input_ext` words per element) or
-sequences of elements (ints, or 3-tuples for ext = 3).  With borrow=True the
+sequences of elements (ints, or ext-tuples for ext = 2, 3).  With borrow=True the
 inputs are device addresses (ints) of canonical planes, see
 EFFSC_HIP_BORROW_INPUT.
 """
@@ -65,7 +65,7 @@ class NativeProver:
         return out.value
 
     def _elem(self, words):
-        return int(words[0]) if self.ext == 1 else tuple(int(x) for x in words[:3])
+        return int(words[0]) if self.ext == 1 else tuple(int(x) for x in words[:self.ext])
 
     def _chal_words(self, c):
         if self.ext == 1:
@@ -363,23 +363,39 @@ class GkrProver(NativeProver):
         return self._elem(out[:self.ext]), self._elem(out[self.ext:])
 
 
-def fold(values, weight, device=0):
-    """fold(&mut Vec<F>, weight) (src/fold.rs:14) for base-field tables:
-    returns the folded list (length next_pow2(n)/2), computed on the device."""
-    v = _words(values, 1)
-    n = len(v)
+def fold(values, weight, device=0, *, ext=1, w=W_FROM_RAW7):
+    """fold(&mut Vec<F>, weight) (src/fold.rs:14): returns the folded list
+    (length next_pow2(n)/2), computed on the device.  ext = 1 takes ints;
+    ext = 2 or 3 takes ext-tuples (or AoS words) and an ext-tuple weight."""
+    v = _words(values, ext)
+    n = len(v) // ext
     if n <= 1:
-        return [int(x) for x in v]
+        return [int(x) for x in v] if ext == 1 else [tuple(int(c) for c in v[i * ext:(i + 1) * ext]) for i in range(n)]
     ptr = N.u64p()
-    N.check(N.lib().effsc_hip_device_alloc(device, n, C.byref(ptr)))
+    N.check(N.lib().effsc_hip_device_alloc(device, n * ext, C.byref(ptr)))
     try:
-        N.check(N.lib().effsc_hip_copy(device, ptr, v.ctypes.data_as(N.u64p), n))
-        m = fold_device(C.cast(ptr, C.c_void_p).value, n, weight, device)
-        out = np.zeros(m, dtype=np.uint64)
-        N.check(N.lib().effsc_hip_copy(device, out.ctypes.data_as(N.u64p), ptr, m))
+        N.check(N.lib().effsc_hip_copy(device, ptr, v.ctypes.data_as(N.u64p), n * ext))
+        if ext == 1:
+            m = fold_device(C.cast(ptr, C.c_void_p).value, n, weight, device)
+        else:
+            m = fold_device_ext(C.cast(ptr, C.c_void_p).value, n, weight, ext, w, device)
+        out = np.zeros(m * ext, dtype=np.uint64)
+        N.check(N.lib().effsc_hip_copy(device, out.ctypes.data_as(N.u64p), ptr, m * ext))
     finally:
         N.lib().effsc_hip_device_free(device, ptr)
-    return [int(x) for x in out]
+    if ext == 1:
+        return [int(x) for x in out]
+    return [tuple(int(c) for c in out[i * ext:(i + 1) * ext]) for i in range(m)]
+
+
+def fold_device_ext(ptr, n, weight, ext, w=W_FROM_RAW7, device=0):
+    """In-place fold of n extension elements (AoS canonical words, `ext` per
+    element) at device address `ptr` with an ext-tuple weight."""
+    out = C.c_size_t(0)
+    wt = np.array([int(c) % P for c in weight], dtype=np.uint64)
+    N.check(N.lib().effsc_hip_fold_ext(device, ext, w, C.cast(C.c_void_p(ptr), N.u64p), n, wt.ctypes.data_as(N.u64p),
+                                       C.byref(out)))
+    return out.value
 
 
 def fold_device(ptr, n, weight, device=0):

@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
+from .field import CANONICAL, MONT64, from_mont, to_mont
 from .proof import SumcheckProof
 from .provers import NativeProver
 from .transcript import NativeTranscript
@@ -22,24 +23,40 @@ def noop_hook(round, transcript):
     return None
 
 
+def _map(x, fn):
+    return fn(x) if isinstance(x, int) else tuple(fn(c) for c in x)
+
+
 def sumcheck(prover, num_rounds, transcript, hook=None) -> SumcheckProof:
     if hook is noop_hook:
         hook = None
+    # A prover built with repr=MONT64 speaks arkworks Montgomery words (the
+    # reference's F64 encoding, src/tests/fields.rs:26-30); transcripts take
+    # and give canonical values.  Its proof is the canonical proof with every
+    # element in Montgomery form.
+    mont = getattr(prover, "repr", CANONICAL) == MONT64
     if isinstance(prover, NativeProver) and isinstance(transcript, NativeTranscript):
         if transcript.ext != prover.ext:
             raise ValueError("transcript and prover fields differ")
-        if prover.repr == 0:
-            return _native_sumcheck(prover, num_rounds, transcript, hook)
+        # the library's runner works on canonical words whatever the prover's
+        # boundary encoding
+        proof = _native_sumcheck(prover, num_rounds, transcript, hook)
+        if not mont:
+            return proof
+        return SumcheckProof([[_map(x, to_mont) for x in r] for r in proof.round_polys],
+                             [_map(c, to_mont) for c in proof.challenges], _map(proof.final_value, to_mont))
     round_polys, challenges = [], []
     prev = None
     for rnd in range(num_rounds):
         evals = prover.round(prev)
         for v in evals:
-            transcript.send(v)
+            transcript.send(_map(v, from_mont) if mont else v)
         round_polys.append(evals)
         if hook is not None:
             hook(rnd, transcript)
         r = transcript.challenge()
+        if mont:
+            r = _map(r, to_mont)
         challenges.append(r)
         prev = r
     if prev is not None:

@@ -30,7 +30,7 @@ def sumcheck_verify(claimed_sum, expected_degree, num_rounds, transcript, hook=N
     if hook is noop_hook_verify:
         hook = None
     if ext is None:
-        ext = getattr(transcript, "ext", 1 if isinstance(claimed_sum, int) else 3)
+        ext = getattr(transcript, "ext", 1 if isinstance(claimed_sum, int) else len(claimed_sum))
     if isinstance(transcript, NativeTranscript) and transcript._tape is not None:
         return _native_verify(claimed_sum, expected_degree, num_rounds, transcript, hook, ext, w)
     F = Field(ext, w)
@@ -115,5 +115,5 @@ def _native_verify(claimed_sum, d, num_rounds, t, hook, ext, w):
     if rc == 5:
         raise SumcheckError("TranscriptError" if err_kind.value == 1 else "HookError", err_round.value)
     N.check(rc)
-    el = (lambda a: int(a[0])) if ext == 1 else (lambda a: tuple(int(x) for x in a[:3]))
+    el = (lambda a: int(a[0])) if ext == 1 else (lambda a: tuple(int(x) for x in a[:ext]))
     return SumcheckResult([el(chals[r * ext:(r + 1) * ext]) for r in range(num_rounds)], el(fin))

@@ -22,8 +22,8 @@
 namespace effsc {
 
 // ------------------------------------------------------------------ fields
-// Elements are canonical words: F64 one word, F64Ext3 three (c0, c1, c2) of
-// F_p[X]/(X^3 - w).  Arithmetic lives on the device; the host types are
+// Elements are canonical words: F64 one word, F64Ext2 two (c0, c1) of
+// F_p[X]/(X^2 - w), F64Ext3 three (c0, c1, c2) of F_p[X]/(X^3 - w).  Arithmetic lives on the device; the host types are
 // plain carriers (the verifier's arithmetic runs inside the library).
 struct F64 {
   static constexpr int EXT = 1;
@@ -34,6 +34,17 @@ struct F64 {
   static F64 from_words(const uint64_t* w) { return F64(w[0]); }
   bool operator==(const F64& o) const { return v == o.v; }
   bool operator!=(const F64& o) const { return v != o.v; }
+};
+
+struct F64Ext2 {
+  static constexpr int EXT = 2;
+  std::array<uint64_t, 2> c{0, 0};
+  F64Ext2() = default;
+  F64Ext2(uint64_t c0, uint64_t c1) : c{c0, c1} {}
+  const uint64_t* words() const { return c.data(); }
+  static F64Ext2 from_words(const uint64_t* w) { return F64Ext2(w[0], w[1]); }
+  bool operator==(const F64Ext2& o) const { return c == o.c; }
+  bool operator!=(const F64Ext2& o) const { return c != o.c; }
 };
 
 struct F64Ext3 {

@@ -9,9 +9,10 @@
  *
  * Conventions
  *  - Field elements are u64 words: `ext` words per element (ext = 1 for
- *    F_p, 3 for F_p[X]/(X^3 - w)), components in order c0, c1, c2 (the
- *    arkworks layout the reference reinterprets through _as_u64_slice,
- *    src/field.rs:118-145).
+ *    F_p, 2 for F_p[X]/(X^2 - w), 3 for F_p[X]/(X^3 - w): the reference's
+ *    F64, F64Ext2 and F64Ext3, src/tests/fields.rs:26-100), components in
+ *    order c0, c1, c2 (the arkworks layout the reference reinterprets
+ *    through _as_u64_slice, src/field.rs:118-145).
  *  - `repr` selects the word encoding at this boundary: canonical values in
  *    [0, p), or arkworks Montgomery words (R = 2^64), which is what the
  *    reference's F64 stores (src/tests/fields.rs:26-30).  Device tables are
@@ -49,19 +50,19 @@ typedef enum {
 typedef enum { EFFSC_HIP_CANONICAL = 0, EFFSC_HIP_MONT64 = 1 } effsc_hip_repr;
 
 /* Input flag: `tables` are device pointers to canonical words in the device
- * layout (base: one plane; ext3: three planes of `n` words each), owned by the
+ * layout (base: one plane; ext2 / ext3: two / three planes of `n` words each), owned by the
  * caller and kept alive and unmodified until effsc_hip_free. */
 #define EFFSC_HIP_BORROW_INPUT 0x1u
 
 /* The reference's extension non-residue F64::from_raw(7), canonical value
- * 7 * 2^-64 mod p (src/tests/fields.rs:49,80). */
+ * 7 * 2^-64 mod p, for both F64Ext2 and F64Ext3 (src/tests/fields.rs:49,80). */
 #define EFFSC_HIP_W_FROM_RAW7 0xFFFFFFF800000001ull
 
 typedef struct {
-  int ext;          /* field of tables, challenges and round polynomials: 1 or 3 */
-  int input_ext;    /* field of the input words: 1 (lifted when ext = 3) or ext */
+  int ext;          /* field of tables, challenges and round polynomials: 1, 2 or 3 */
+  int input_ext;    /* field of the input words: 1 (lifted when ext > 1) or ext */
   int repr;         /* effsc_hip_repr of every word crossing this ABI */
-  uint64_t ext_w;   /* canonical non-residue w of X^3 - w (ignored when ext = 1) */
+  uint64_t ext_w;   /* canonical non-residue w of X^ext - w (ignored when ext = 1) */
   int device;       /* HIP device ordinal */
   unsigned flags;   /* EFFSC_HIP_BORROW_INPUT */
 } effsc_hip_config;
@@ -190,6 +191,13 @@ const char* effsc_hip_last_error(void);
  * field) with implicit zero padding; *new_len = next_pow2(n) / 2 (or n if
  * n <= 1). */
 int effsc_hip_fold(int device, uint64_t* values, size_t n, uint64_t weight, size_t* new_len);
+/* The same fold over an extension field (fold<F64Ext2> / fold<F64Ext3>,
+ * src/fold.rs:14; the reference benchmarks fold/F64Ext3,
+ * benches/sumcheck.rs:156-180): n elements of `ext` canonical words each, AoS
+ * (the layout of the reference's Vec<F> reinterpreted as u64 words), in
+ * place on the device; `weight` is ext host words, w the non-residue. */
+int effsc_hip_fold_ext(int device, int ext, uint64_t w, uint64_t* values, size_t n, const uint64_t* weight,
+                       size_t* new_len);
 
 /* --------------------------------------------------------- transcripts */
 
@@ -278,7 +286,7 @@ void effsc_hip_stream_close(effsc_hip_stream* s);
 /* multivariate_claim (k = 1) / multivariate_product_claim
  * (src/streams/stream.rs:3-41): sum_i prod_j s_j[i] on the device, the
  * streams moved in chunks through pinned buffers (ext words out; w is the
- * extension non-residue when ext = 3). */
+ * extension non-residue when ext > 1). */
 int effsc_hip_stream_claim(int device, effsc_hip_stream* const* streams, int k, uint64_t w, uint64_t* out);
 /* MultilinearProver (kind 1) / InnerProductProver (kind 2) over base-field
  * tables read from streams.  The upload runs chunk by chunk on a copy stream

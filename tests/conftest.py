@@ -20,10 +20,3 @@ def golden():
     with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
         return json.load(f)
 
-
-def gpu_available():
-    try:
-        import torch  # noqa: F401  (device presence only)
-        return torch.cuda.is_available()
-    except Exception:
-        return False

@@ -48,7 +48,7 @@ def shake_vectors():
 
 def transcript_vectors():
     out = []
-    for ds, ext in ((b"", 1), (b"effsc-hip/test", 1), (b"x" * 150, 3)):
+    for ds, ext in ((b"", 1), (b"effsc-hip/test", 1), (b"x" * 150, 3), (b"effsc-hip/fp2", 2)):
         t = B.ShakeTranscript(ds)
         ops = []
         for i in range(6):
@@ -103,6 +103,12 @@ def proof_vectors():
     add("product4", 1, 3, [15, 16, 17, 18])
     add("eq", 1, 4, [19, 20])
     add("eq", 1, 5, [19, 20])
+    # F_p^2 = F_p[X]/(X^2 - w), w = from_raw(7): base inputs lifted
+    add("ml", 2, 3, [21])
+    add("ml", 2, 4, [21], n=11)
+    add("ip", 2, 4, [22, 23])
+    add("product2", 2, 3, [22, 23], transcript="rng")
+    add("product3", 2, 4, [24, 25, 26])
     return cases
 
 

@@ -36,7 +36,7 @@ def rounds_for(n):
     return (n * n - 1).bit_length() if n > 1 else 0
 
 
-@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1), (2, 2), (2, 1)])
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 16, 37, 64])
 def test_gkr_parity(n, ext, input_ext):
     seed = 1000 * n + 10 * ext + input_ext
@@ -101,7 +101,7 @@ def test_gkr_length_errors():
         GkrProver(O.uniform(16, 1), O.uniform(9, 2), O.uniform(4, 3))
 
 
-@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1), (2, 2), (2, 1)])
 @pytest.mark.parametrize("k", [6, 7, 8, 9])
 def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
     # hookless proofs of power-of-two layers run on the device (factored

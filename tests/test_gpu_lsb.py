@@ -82,7 +82,7 @@ def make(kind, ext, input_ext, tabs):
 
 
 @pytest.mark.parametrize("kind", ["ml", "ip", "p3"])
-@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3)])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3), (2, 1), (2, 2)])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 6, 10, 13])
 def test_lsb_parity(kind, ext, input_ext, v):
     if kind == "p3" and input_ext != 1:

@@ -36,17 +36,25 @@ def as_oracle(proof, ext):
     return rp, ch, fin
 
 
+def lift(words, d):
+    """Base words -> extension words (c0 = x, other components 0); d = 1 keeps them."""
+    if d == 1:
+        return words
+    out = np.zeros(len(words) * d, dtype=np.uint64)
+    out[::d] = words
+    return out
+
+
 def make_pair(kind, ext, input_ext, tabs, wpt=None):
     """(gpu prover, oracle prover) over the same canonical input words."""
     if kind == "ml":
+        # base inputs of an extension prover: the oracle prover over the
+        # lifted words (same ML padding rules for any length)
         g = MultilinearProver(tabs[0], ext=ext, input_ext=input_ext)
-        o = O.Prover.multilinear(tabs[0], ext=ext) if input_ext == ext else O.Prover.product([tabs[0]], ext=ext)
+        o = O.Prover.multilinear(lift(tabs[0], ext // input_ext), ext=ext)
     elif kind == "ip":
         g = InnerProductProver(tabs[0], tabs[1], ext=ext, input_ext=input_ext)
-        if input_ext == ext:
-            o = O.Prover.inner_product(tabs[0], tabs[1], ext=ext)
-        else:
-            o = O.Prover.product(tabs[:2], ext=ext)
+        o = O.Prover.inner_product(lift(tabs[0], ext // input_ext), lift(tabs[1], ext // input_ext), ext=ext)
     elif kind.startswith("p"):
         k = int(kind[1:])
         g = CoefficientProver(tabs[:k], ext=ext, input_ext=input_ext)
@@ -105,7 +113,7 @@ def test_golden_proofs(golden):
 
 # ------------------------------------------------------------- oracle parity
 @pytest.mark.parametrize("kind", ["ml", "ip"])
-@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1)])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 3), (3, 1), (2, 2), (2, 1)])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 4, 5, 8, 11, 14])
 def test_pow2_parity(kind, ext, input_ext, v):
     check_proof(kind, ext, input_ext, 1 << v, v, 1000 + v)
@@ -117,23 +125,25 @@ def test_non_pow2_parity(kind, n):
     v = (n - 1).bit_length()
     check_proof(kind, 1, 1, n, v, 77 + n)
     check_proof(kind, 3, 3, n, v, 78 + n, transcript="rng")
+    check_proof(kind, 2, 2, n, v, 79 + n)
+    check_proof(kind, 2, 1, n, v, 80 + n, transcript="rng")
 
 
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
-@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3)])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3), (2, 1), (2, 2)])
 @pytest.mark.parametrize("v", [1, 2, 3, 6, 12])
 def test_product_parity(k, ext, input_ext, v):
     check_proof(f"p{k}", ext, input_ext, 1 << v, v, 500 + 10 * k + v)
 
 
-@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("ext", [1, 2, 3])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 5, 8, 13])
 def test_eq_factored_parity(ext, v):
     check_proof("eq", ext, ext, 1 << v, v, 900 + v)
 
 
 @pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 18), (1, 20), (1, 23), (3, 11), (3, 12),
-                                   (3, 15), (3, 17), (3, 18)])
+                                   (3, 15), (3, 17), (3, 18), (2, 11), (2, 12), (2, 14), (2, 17)])
 def test_eq_factored_device_loop(ext, v, monkeypatch):
     # large p: multi-workgroup rounds with in-reduction eq folds, then the
     # one-workgroup tail; against the oracle and the per-round host loop
@@ -150,7 +160,7 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
-    if ext == 3:
+    if ext >= 2:
         monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
         monkeypatch.delenv("EFFSC_HIP_EQ_TAIL")
@@ -257,6 +267,55 @@ def test_montgomery_boundary():
     assert [int(x) for x in gm.table_words(0)] == [to_mont(int(x)) for x in gc.table_words(0)]
 
 
+class _ForwardingTranscript:
+    """A transcript the runner does not know as native: forces the Python loop."""
+
+    def __init__(self, inner):
+        self.inner = inner
+        self.ext = inner.ext
+
+    def send(self, value):
+        self.inner.send(value)
+
+    def challenge(self):
+        return self.inner.challenge()
+
+
+@pytest.mark.parametrize("ext", [1, 2, 3])
+@pytest.mark.parametrize("python_loop", [False, True])
+def test_montgomery_prover_runner(ext, python_loop):
+    # sumcheck() over a repr=MONT64 prover: the proof of the canonical prover
+    # with every element in Montgomery form, accepted by the verifier
+    from effsc.field import from_mont
+    v = 9
+    a, b = O.uniform((1 << v) * ext, 15), O.uniform((1 << v) * ext, 16)
+    am = np.array([to_mont(int(x)) for x in a], dtype=np.uint64)
+    bm = np.array([to_mont(int(x)) for x in b], dtype=np.uint64)
+    mp = (lambda x: to_mont(x)) if ext == 1 else (lambda x: tuple(to_mont(c) for c in x))
+    say(f"mont runner ext={ext} python_loop={python_loop}")
+    gc = InnerProductProver(a, b, ext=ext)
+    pc = sumcheck(gc, v, Transcript(b"mont", ext=ext))
+    gm = InnerProductProver(am, bm, ext=ext, repr=MONT64)
+    t = Transcript(b"mont", ext=ext)
+    pm = sumcheck(gm, v, _ForwardingTranscript(t) if python_loop else t)
+    assert pm.round_polys == [[mp(x) for x in r] for r in pc.round_polys]
+    assert pm.challenges == [mp(c) for c in pc.challenges]
+    assert pm.final_value == mp(pc.final_value)
+    assert [int(x) for x in gm.table_words(0)] == [to_mont(int(x)) for x in gc.table_words(0)]
+    # the canonical form of the Montgomery proof verifies against the sum
+    # computed from the tables
+    F = effsc.Field(ext)
+    el = (lambda w, i: int(w[i])) if ext == 1 else (lambda w, i: tuple(int(x) for x in w[i * ext:(i + 1) * ext]))
+    total = F.zero()
+    for i in range(1 << v):
+        total = F.add(total, F.mul(el(a, i), el(b, i)))
+    back = effsc.SumcheckProof([[(from_mont(x) if ext == 1 else tuple(from_mont(c) for c in x)) for x in r]
+                                for r in pm.round_polys], [], None)
+    res = sumcheck_verify(total, 2, v, Transcript(b"mont", ext=ext).verifier(back))
+    assert res.challenges == pc.challenges
+    assert res.final_claim == pc.final_value
+
+
 def test_borrowed_device_input_and_fill_uniform():
     n, v = 1 << 14, 14
     ptrs = []
@@ -283,7 +342,7 @@ def test_borrowed_device_input_and_fill_uniform():
             N.lib().effsc_hip_device_free(0, p)
 
 
-@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("ext", [1, 2, 3])
 def test_eq_factored_borrowed_input_and_reset(ext):
     v = 11
     n = 1 << v
@@ -376,7 +435,8 @@ def _proof_both_loops(make, rounds, transcript_factory, monkeypatch, tail):
 
 @pytest.mark.parametrize("tail", [2, 64, 1 << 14, 1 << 22])
 @pytest.mark.parametrize("kind,ext,input_ext", [("ml", 1, 1), ("ip", 1, 1), ("ip", 3, 3), ("p3", 1, 1),
-                                                ("p3", 3, 1), ("p4", 3, 3), ("p2", 3, 1)])
+                                                ("p3", 3, 1), ("p4", 3, 3), ("p2", 3, 1), ("ml", 2, 1),
+                                                ("ip", 2, 2), ("p3", 2, 1), ("p4", 2, 2)])
 @pytest.mark.parametrize("v", [1, 2, 5, 11, 16])
 def test_device_loop_matches_oracle(kind, ext, input_ext, v, tail, monkeypatch):
     k = {"ml": 1, "ip": 2}.get(kind) or int(kind[1:])

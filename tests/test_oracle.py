@@ -114,12 +114,9 @@ def _oracle_proof(case):
     kind = case["kind"]
     tabs = [O.uniform(n, s) for s in seeds]
     if kind == "ml":
-        if ext == 1:
-            pr = O.Prover.multilinear(tabs[0])
-        else:
-            pr = O.Prover.product([tabs[0]], ext=ext)
+        pr = O.Prover.multilinear(tabs[0] if ext == 1 else _lift(tabs[0], ext), ext=ext)
     elif kind == "ip":
-        pr = O.Prover.inner_product(tabs[0], tabs[1], ext=ext)
+        pr = O.Prover.inner_product(*[t if ext == 1 else _lift(t, ext) for t in tabs[:2]], ext=ext)
     elif kind.startswith("product"):
         k = int(kind[-1])
         pr = O.Prover.product(tabs[:k], ext=ext)
@@ -139,7 +136,7 @@ def test_oracle_proofs_match_brute_force_golden(golden):
 
 
 @pytest.mark.parametrize("v", [1, 3, 6])
-@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("ext", [1, 2, 3])
 def test_eq_factored_equals_inner_product_with_eq_table(v, ext):
     # src/provers/eq_factored.rs:411-446: EqFactored == IP(p, build_eq_table(w))
     n = 1 << v
@@ -254,7 +251,7 @@ def test_plonky3_style_counter_closed_form():
 
 
 # ---------------------------------------------------------------- GKR
-@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("ext", [1, 2, 3])
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
 def test_gkr_oracle_matches_brute_force(ext, n):
     # src/provers/gkr.rs restated in C against the definition (tails included)
@@ -304,7 +301,7 @@ def test_gkr_reference_identities():
 
 # ---------------------------------------------------------------- LSB provers
 @pytest.mark.parametrize("kind,k", [("ml", 1), ("ip", 2), ("p3", 3)])
-@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("ext", [1, 2, 3])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 5])
 def test_lsb_oracle_matches_brute_force(kind, k, ext, v):
     n = 1 << v
