@@ -150,7 +150,7 @@ inline size_t next_pow2(size_t n) {
   return p;
 }
 inline bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
-inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+constexpr size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 int grid_max_for(int dev);
 
 constexpr size_t kMaxRounds = 64;

@@ -81,6 +81,7 @@ _SIGS = {
     "effsc_hip_transcript_challenge": (C.c_int, [vp, C.c_int, u64p]),
     "effsc_hip_transcript_free": (None, [vp]),
     "effsc_hip_sumcheck": (C.c_int, [vp, sz, vp, HOOK, vp, u64p, u64p, u64p]),
+    "effsc_hip_sumcheck_noalloc": (C.c_int, [C.c_int, C.c_int, u64p, u64p, sz, vp, u64p, u64p, u64p]),
     "effsc_hip_sumcheck_verify": (
         C.c_int,
         [C.c_int, u64, u64p, sz, sz, vp, u64p, sz, HOOK, vp, u64p, u64p, szp, C.POINTER(C.c_int)],

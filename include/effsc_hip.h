@@ -227,6 +227,22 @@ typedef int (*effsc_hip_hook)(void* ctx, size_t round, effsc_hip_transcript* t);
 int effsc_hip_sumcheck(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t, effsc_hip_hook hook,
                        void* hook_ctx, uint64_t* round_polys, uint64_t* challenges, uint64_t* final_value);
 
+/* A whole MultilinearProver (kind 1) or InnerProductProver (kind 2) proof that
+ * allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It is effsc_hip_sumcheck over all
+ * v = log2(n) rounds with no hook, on borrowed canonical device tables a and b
+ * (b ignored for kind 1; 16-byte aligned, n a power of two, 2 <= n <= 2^20),
+ * run on the device's null stream.  Every buffer beyond the inputs and outputs
+ * lives in one static device array of the library (about 9 MiB, held on each
+ * device it runs on until the process ends); larger n is EINVAL.  Outputs as
+ * effsc_hip_sumcheck (host words, canonical); the transcript state is read
+ * and updated.  Calls that overlap on one device share that array and race:
+ * serialise them. */
+int effsc_hip_sumcheck_noalloc(int device, int kind, const uint64_t* a, const uint64_t* b, size_t n,
+                               effsc_hip_transcript* t, uint64_t* round_polys, uint64_t* challenges,
+                               uint64_t* final_value);
+
 /* verifier::sumcheck_verify — src/verifier.rs:61-139.  The prover messages
  * are read from `proof` (proof_words canonical words).  On success returns
  * OK with challenges (num_rounds * ext) and final_claim (ext); the oracle
