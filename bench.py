@@ -10,6 +10,13 @@ criterion convention, Throughput::Elements(2^v), benches/sumcheck.rs:28,80).
 
 Other workloads (--workload):
   ml        MultilinearProver, v = 20
+  ml2, ip2  MultilinearProver / InnerProductProver over F_p^2 (F64Ext2
+            inputs), v = 26
+  ml3, ip3  the same over F_p^3 (F64Ext3 inputs), v = 24
+  matrix    the reference's criterion matrix (benches/sumcheck.rs):
+            {multilinear, inner_product} x {F64, F64Ext2, F64Ext3} x
+            {2^16, 2^20, 2^24} and fold/{F64, F64Ext2, F64Ext3}, each cell
+            beside the oracle on the host
   product3  CoefficientProver, product of 3 multilinears (degree 3), v = 24,
             base-field inputs, challenges and folded tables in F_p^3
   eq, eq3   EqFactoredProver, v = 24, random point, base field / F_p^3
@@ -20,14 +27,21 @@ Other workloads (--workload):
   serve     served request mix, saturated and Poisson, learned admission
   stream    the v = 28 inner product from host files, resident and out of core
 
-Prints one JSON line.  Besides the contract fields it carries
+Prints one JSON line.  The default line also carries the batched (512
+inner-product instances) and served (Poisson mix) results under "batched"
+and "served" (--no-extras skips them).  Besides the contract fields it
+carries
   roofline     - the dominant kernel class's achieved rate: its algorithmic
                  bytes (each input element read once, each output element
                  written once) or Goldilocks products over the summed
                  HIP-event durations of its launches in the timed region
                  (events recorded on the prover's stream around each launch);
   cpu_baseline - the oracle's restatement of the reference prover, timed on
-                 this host's cores on a bounded sample.
+                 this host's cores on a bounded sample: one warm-up and ten
+                 timed proofs, median with p10 / p90, the host CPU model and
+                 flags; for the inner product both the InnerProductProver
+                 round structure and the reference's faster legacy fused
+                 path, the faster one as the value.
 """
 import argparse
 import ctypes as C
@@ -40,6 +54,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "efficient-sumcheck_amd"))
 
 HBM_PEAK_GBS = 8000.0
+W_FROM_RAW7 = 0xFFFFFFF800000001
 SEED_A, SEED_B = 0x5EED0002, 0x5EED0003
 SEED_ML = 0x5EED0001
 SEEDS_P3 = (0x5EED0031, 0x5EED0032, 0x5EED0033)
@@ -55,6 +70,14 @@ WORKLOADS = {
                metric="Goldilocks sumcheck evals/s (InnerProductProver MSB, v=28)"),
     "ml": dict(kind="ml", v=20, ext=1, input_ext=1, tables=1,
                metric="Goldilocks sumcheck evals/s (MultilinearProver MSB, v=20)"),
+    "ml2": dict(kind="ml", v=26, ext=2, input_ext=2, tables=1,
+                metric="Goldilocks sumcheck evals/s (MultilinearProver MSB over F_p^2, v=26)"),
+    "ip2": dict(kind="ip", v=26, ext=2, input_ext=2, tables=2,
+                metric="Goldilocks sumcheck evals/s (InnerProductProver MSB over F_p^2, v=26)"),
+    "ml3": dict(kind="ml", v=24, ext=3, input_ext=3, tables=1,
+                metric="Goldilocks sumcheck evals/s (MultilinearProver MSB over F_p^3, v=24)"),
+    "ip3": dict(kind="ip", v=24, ext=3, input_ext=3, tables=2,
+                metric="Goldilocks sumcheck evals/s (InnerProductProver MSB over F_p^3, v=24)"),
     "product3": dict(kind="product", v=24, ext=3, input_ext=1, tables=3,
                      metric="Goldilocks sumcheck evals/s (CoefficientProver product of 3, F_p^3 challenges, v=24)"),
     "eq": dict(kind="eq", v=24, ext=1, input_ext=1, tables=1,
@@ -101,34 +124,88 @@ def recorded_traffic(path=os.path.join(ROOT, "profiles", "k_pass_ip_v28_traffic.
 
 
 # ---------------------------------------------------------------- CPU side
-def cpu_baseline(workload, v_sample, threads):
+CPU_SAMPLES = 10
+
+
+def cpu_info():
+    """Host CPU model, the SIMD flags the reference's backends key on, and
+    the logical CPU count."""
+    model, flags = None, []
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if model is None and line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                if not flags and line.startswith("flags"):
+                    flags = line.split(":", 1)[1].split()
+    except OSError:
+        pass
+    keep = [x for x in ("avx2", "bmi2", "adx", "avx512f", "avx512dq", "avx512vl", "avx512ifma") if x in flags]
+    return {"model": model, "flags": keep, "logical_cpus": os.cpu_count()}
+
+
+def _sampled(run, samples):
+    """One warm-up, then `samples` timed runs; run() returns the seconds of
+    its timed part (input set-up is outside it)."""
+    import numpy as np
+    run()
+    t = np.array([run() for _ in range(samples)])
+    return {"median_s": float(np.median(t)), "p10_s": float(np.percentile(t, 10)),
+            "p90_s": float(np.percentile(t, 90)), "samples": samples}
+
+
+def cpu_baseline(workload, v_sample, threads, samples=CPU_SAMPLES):
     """The oracle (a C restatement of the reference prover) timed on this
-    host.  Inputs are the same counter-based uniform tables."""
+    host over the same counter-based uniform tables."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     n = 1 << v_sample
+    protocol = f"1 warm-up + {samples} timed proofs at v={v_sample}, median (p10, p90)"
     if workload == "ip":
         a, b = O.uniform(n, SEED_A), O.uniform(n, SEED_B)
-        t, _, _, _ = O.bench_ip(a, b, v_sample, threads)
-        desc = "two parallel folds per round, serial evaluate, scalar Montgomery arithmetic"
-        return n / t, t, threads, desc
-    if workload == "lsb":
-        p = O.Prover.inner_product_lsb(O.uniform(n, SEED_A), O.uniform(n, SEED_B))
-    elif workload == "gkr":
-        k = v_sample // 2
-        m = 1 << k
-        p = O.Prover.gkr(O.uniform(m * m, SEEDS_GKR[0]), O.uniform(m * m, SEEDS_GKR[1]), O.uniform(m, SEEDS_GKR[2]))
-    elif workload == "ml":
-        p = O.Prover.multilinear(O.uniform(n, SEED_ML))
-    elif workload == "product3":
-        p = O.Prover.product([O.uniform(n, s) for s in SEEDS_P3], ext=3)
-    else:
-        ext = 3 if workload == "eq3" else 1
-        p = O.Prover.eq_factored(O.uniform(v_sample * ext, SEED_EQ_W), O.uniform(n * ext, SEED_EQ_P), ext=ext)
-    t0 = time.perf_counter()
-    p.sumcheck(v_sample, O.Transcript.shake(b"effsc-hip/bench/cpu"))
-    t = time.perf_counter() - t0
-    return n / t, t, 1, "serial scalar Montgomery restatement of the reference prover"
+        paths = {}
+        for name, fused, desc in (
+                ("legacy_fused_ip", True, "inner_product_sumcheck_partial with fused_fold_and_compute_polynomial "
+                                          "(src/inner_product_sumcheck.rs:199-360): fold and evaluate in one parallel "
+                                          "sweep per round"),
+                ("inner_product_prover", False, "InnerProductProver::round (src/provers/inner_product.rs:61-113): two "
+                                                "parallel folds, serial evaluate")):
+            st = _sampled(lambda: O.bench_ip(a.copy(), b.copy(), v_sample, threads, fused=fused)[0], samples)
+            paths[name] = dict(st, evals_per_s=n / st["median_s"], threads=threads, path=desc)
+            log(f"bench: CPU {name}: {st['median_s'] * 1e3:.1f} ms median at v={v_sample}, {threads} threads")
+        best = max(paths, key=lambda k: paths[k]["evals_per_s"])
+        return {"value": paths[best]["evals_per_s"], "unit": "evals/s", "cores": threads, "kind": "port",
+                "sample": f"{protocol}; best reference CPU path: {best}, scalar Montgomery arithmetic",
+                "best_path": best, "paths": paths, "cpu": cpu_info()}
+
+    def make():
+        if workload == "lsb":
+            return O.Prover.inner_product_lsb(O.uniform(n, SEED_A), O.uniform(n, SEED_B))
+        if workload == "gkr":
+            m = 1 << (v_sample // 2)
+            return O.Prover.gkr(O.uniform(m * m, SEEDS_GKR[0]), O.uniform(m * m, SEEDS_GKR[1]),
+                                O.uniform(m, SEEDS_GKR[2]))
+        if workload in ("ml", "ml2", "ml3"):
+            e = WORKLOADS[workload]["ext"]
+            return O.Prover.multilinear(O.uniform(n * e, SEED_ML), ext=e)
+        if workload in ("ip2", "ip3"):
+            e = WORKLOADS[workload]["ext"]
+            return O.Prover.inner_product(O.uniform(n * e, SEED_A), O.uniform(n * e, SEED_B), ext=e)
+        if workload == "product3":
+            return O.Prover.product([O.uniform(n, s) for s in SEEDS_P3], ext=3)
+        e = 3 if workload == "eq3" else 1
+        return O.Prover.eq_factored(O.uniform(v_sample * e, SEED_EQ_W), O.uniform(n * e, SEED_EQ_P), ext=e)
+
+    def run():
+        p = make()
+        t0 = time.perf_counter()
+        p.sumcheck(v_sample, O.Transcript.shake(b"effsc-hip/bench/cpu"))
+        return time.perf_counter() - t0
+    st = _sampled(run, samples)
+    log(f"bench: CPU oracle prover: {st['median_s'] * 1e3:.1f} ms median at v={v_sample}, 1 thread")
+    return {"value": n / st["median_s"], "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"{protocol}; serial scalar Montgomery restatement of the reference prover",
+            **st, "cpu": cpu_info()}
 
 
 # ---------------------------------------------------------------- GPU side
@@ -146,12 +223,13 @@ def device_tables(L, N, dev, spec):
         return C.cast(p, C.c_void_p).value
 
     if kind in ("ip", "ip_lsb"):
-        addrs = [table(n, SEED_A), table(n, SEED_B)]
+        # ext planes of n words each (the borrowed-input layout)
+        addrs = [table(n * ext, SEED_A), table(n * ext, SEED_B)]
     elif kind == "gkr":
         m = 1 << (v // 2)
         addrs = [table(n, SEEDS_GKR[0]), table(n, SEEDS_GKR[1]), table(m, SEEDS_GKR[2])]
     elif kind == "ml":
-        addrs = [table(n, SEED_ML)]
+        addrs = [table(n * ext, SEED_ML)]
     elif kind == "product":
         addrs = [table(n, s) for s in SEEDS_P3]
     else:
@@ -175,7 +253,7 @@ def make_prover(effsc, L, N, dev, spec, addrs):
     v, n, ext = spec["v"], 1 << spec["v"], spec["ext"]
     kind = spec["kind"]
     if kind == "ip":
-        return effsc.InnerProductProver(addrs[0], addrs[1], borrow=True, n=n, device=dev)
+        return effsc.InnerProductProver(addrs[0], addrs[1], ext=ext, input_ext=ext, borrow=True, n=n, device=dev)
     if kind == "ip_lsb":
         # the prover bit-reverses the tables into its own planes once
         return effsc.InnerProductProverLSB(addrs[0], addrs[1], borrow=True, n=n, device=dev)
@@ -190,7 +268,7 @@ def make_prover(effsc, L, N, dev, spec, addrs):
             host.append(h)
         return effsc.GkrProver(host[0], host[1], host[2], device=dev)
     if kind == "ml":
-        return effsc.MultilinearProver(addrs[0], borrow=True, n=n, device=dev)
+        return effsc.MultilinearProver(addrs[0], ext=ext, input_ext=ext, borrow=True, n=n, device=dev)
     if kind == "product":
         return effsc.CoefficientProver(addrs, ext=ext, input_ext=1, borrow=True, n=n, device=dev)
     w = eq_point(L, N, dev, v, ext)  # v * ext canonical words
@@ -229,7 +307,7 @@ def bench_single(args, workload):
         log(f"bench: calibration copy of {cw * 8} bytes done")
     prover = make_prover(effsc, L, N, dev, spec, addrs)
     log(f"bench: {workload} v={v} ext={ext}, tables resident "
-        f"({sum(1 for _ in made)} x {n * (ext if spec['kind'] == 'eq' else 1) * 8 / 2**20:.0f} MiB)")
+        f"({sum(1 for _ in made)} x {n * (ext if spec['kind'] in ('eq', 'ml', 'ip') else 1) * 8 / 2**20:.0f} MiB)")
 
     def step():
         prover.reset()
@@ -322,16 +400,17 @@ def bench_single(args, workload):
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        # bounded samples: a few seconds of the serial oracle at most
-        default_cpu_v = {"ip": v, "ml": v, "product3": min(v, 22), "eq": v, "eq3": min(v, 22), "lsb": min(v, 24),
-                         "gkr": min(v, 20)}[workload]
+        # bounded samples: 11 proofs of about a second or less each
+        default_cpu_v = {"ip": min(v, 26), "ml": min(v, 20), "ml2": min(v, 18), "ip2": min(v, 18),
+                         "ml3": min(v, 18), "ip3": min(v, 18), "product3": min(v, 18), "eq": min(v, 20),
+                         "eq3": min(v, 18), "lsb": min(v, 20), "gkr": min(v, 18)}[workload]
         cv = args.cpu_v if args.cpu_v is not None else default_cpu_v
         log(f"bench: CPU baseline (oracle port of the reference prover), v={cv}")
-        rate, secs, cores, desc = cpu_baseline(workload, cv, threads)
-        cpu = {"value": rate, "unit": "evals/s", "cores": cores, "kind": "port",
-               "sample": f"one full proof at v={cv} ({secs:.2f} s): {desc}"}
+        cpu = cpu_baseline(workload, cv, threads)
 
     inputs = {"ip": "seeds 0x5EED0002/3", "ml": "seed 0x5EED0001", "product3": "seeds 0x5EED0031/32/33",
+              "ml2": "seed 0x5EED0001 (F_p^2 words)", "ip2": "seeds 0x5EED0002/3 (F_p^2 words)",
+              "ml3": "seed 0x5EED0001 (F_p^3 words)", "ip3": "seeds 0x5EED0002/3 (F_p^3 words)",
               "eq": "p seed 0x5EED0034, point seed 0x5EED0035", "eq3": "p seed 0x5EED0034, point seed 0x5EED0035",
               "lsb": "seeds 0x5EED0002/3 (LSB order)", "gkr": "add, mult, witness seeds 0x5EED0041/42/43"}
     out = {
@@ -347,7 +426,7 @@ def bench_single(args, workload):
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{workload}_v{v}" if workload in ("lsb", "gkr") else f"{workload}_msb_v{v}", "v": v, "field": "goldilocks" if ext == 1 else "goldilocks^3",
+        "config": {"workload": f"{workload}_v{v}" if workload in ("lsb", "gkr") else f"{workload}_msb_v{v}", "v": v, "field": {1: "goldilocks", 2: "goldilocks^2", 3: "goldilocks^3"}[ext],
                    "tables": spec["tables"], "transcript": "shake256-fs",
                    "input": f"uniform [0,p) counter-based, {inputs[workload]}"},
         # the reference algorithm's bytes (one fold per round) over the proof time
@@ -358,7 +437,7 @@ def bench_single(args, workload):
     prover.close()
     for p in made:
         L.effsc_hip_device_free(dev, p)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_batch(args):
@@ -433,7 +512,7 @@ def bench_batch(args):
     same = all(list(pbuf[i][:2 * v]) == [x for r in q.round_polys for x in r] and int(fbuf[i]) == q.final_value
                for i, (v, q) in enumerate(zip(vs, out)))
     alg = sum(algorithmic_bytes(v, tables=2) for v in vs)
-    print(json.dumps({"metric": "batched proofs/s (512 inner-product instances, v ~ U[12,24])",
+    res = {"metric": "batched proofs/s (512 inner-product instances, v ~ U[12,24])",
                       "value": len(vs) / T_batch, "unit": "proofs/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": T_batch * 1e3, "higher_is_better": True,
                       "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
@@ -445,10 +524,13 @@ def bench_batch(args):
                       "bit_identical_to_per_instance": same,
                       "autotune": {"chosen_units_per_workgroup": tile,
                                    "ms_per_batch_by_tile": {str(c): t for c, t in tile_ms.items()}},
-                      "gbps_reference_bytes": alg / T_batch / 1e9}), flush=True)
+                      "gbps_reference_bytes": alg / T_batch / 1e9}
     for p in provers:
         p.close()
     bp.close()
+    for t in a + b:
+        t.free()
+    return res
 
 
 def bench_stream(args):
@@ -531,7 +613,7 @@ def bench_stream(args):
         os.remove(p)
     os.rmdir(tmpdir)
     T_up, T_tot = min(times_up), min(times_total)
-    print(json.dumps({
+    return {
         "metric": "streamed Goldilocks sumcheck evals/s (InnerProductProver MSB, v=%d, tables in host files)" % v,
         "value": n / T_tot, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
         "ms_per_step": T_tot * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -544,8 +626,145 @@ def bench_stream(args):
         "resident_repeat_ms_per_proof_first_pass_reused": T_res * 1e3, "bit_identical_to_resident": True,
         "out_of_core": {"ms_per_proof": T_ooc * 1e3, "evals_per_s": n / T_ooc,
                         "host_bytes_streamed_per_proof": 2 * 2 * n * 8, "resident_device_bytes": 2 * (n // 4) * 8,
-                        "bit_identical_to_resident": True}}),
-        flush=True)
+                        "bit_identical_to_resident": True}}
+
+
+def bench_matrix(args):
+    # The reference's criterion matrix (benches/sumcheck.rs:11-190): full
+    # MultilinearProver / InnerProductProver proofs with the test transcript
+    # (SanityTranscript there, TestTranscript here) and free folds, over F64,
+    # F64Ext2 and F64Ext3 at 2^16, 2^20 and 2^24 elements; Throughput::
+    # Elements(2^v).  GPU side: inputs resident in HBM (the reference builds
+    # its Vec inside the timed routine but generates it in the untimed
+    # set-up); the median of `steps` proofs after `warmup`.  CPU side: the
+    # oracle's serial restatement of the same prover, 1 warm-up + 3 samples
+    # at min(v, 18).
+    import numpy as np
+
+    import effsc
+    from effsc import _native as N
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    L = N.lib()
+    dev = 0
+    steps, warmup = max(args.steps, 3), max(args.warmup, 1)
+    cells = []
+
+    def dev_table(words, seed):
+        p = N.u64p()
+        N.check(L.effsc_hip_device_alloc(dev, words, C.byref(p)))
+        N.check(L.effsc_hip_fill_uniform(dev, p, words, seed, 0))
+        return p
+
+    def cpu_cell(run_factory, v):
+        cv = min(v, 18)
+        st = _sampled(run_factory(cv), 3)
+        return {"v": cv, "evals_per_s": (1 << cv) / st["median_s"], "median_s": st["median_s"],
+                "p10_s": st["p10_s"], "p90_s": st["p90_s"], "cores": 1}
+
+    fields = {1: "F64", 2: "F64Ext2", 3: "F64Ext3"}
+    for kind in ("multilinear", "inner_product"):
+        for ext in (1, 2, 3):
+            for v in (16, 20, 24):
+                n = 1 << v
+                k = 1 if kind == "multilinear" else 2
+                tabs = [dev_table(n * ext, SEED_A + i) for i in range(k)]
+                addrs = [C.cast(t, C.c_void_p).value for t in tabs]
+                if k == 1:
+                    pr = effsc.MultilinearProver(addrs[0], ext=ext, input_ext=ext, borrow=True, n=n)
+                else:
+                    pr = effsc.InnerProductProver(addrs[0], addrs[1], ext=ext, input_ext=ext, borrow=True, n=n)
+
+                def proof():
+                    pr.reset()
+                    return effsc.sumcheck(pr, v, effsc.TestTranscript(7, ext=ext))
+                for _ in range(warmup):
+                    proof()
+                ts = []
+                for _ in range(steps):
+                    N.check(L.effsc_hip_synchronize(dev))
+                    t0 = time.perf_counter()
+                    proof()
+                    ts.append(time.perf_counter() - t0)
+                T = float(np.median(ts))
+                pr.close()
+                for t in tabs:
+                    L.effsc_hip_device_free(dev, t)
+
+                def factory(cv, kind=kind, ext=ext):
+                    m = 1 << cv
+                    tb = [O.uniform(m * ext, SEED_A + i) for i in range(1 if kind == "multilinear" else 2)]
+
+                    def run():
+                        p = (O.Prover.multilinear(tb[0], ext=ext) if kind == "multilinear"
+                             else O.Prover.inner_product(tb[0], tb[1], ext=ext))
+                        t0 = time.perf_counter()
+                        p.sumcheck(cv, O.Transcript.rng(7))
+                        return time.perf_counter() - t0
+                    return run
+                cpu = None if args.no_cpu else cpu_cell(factory, v)
+                alg = algorithmic_bytes(v, tables=k, in_words=ext, work_words=ext)
+                cell = {"group": f"{kind}/{fields[ext]}", "size": f"2^{v}", "gpu_ms": T * 1e3,
+                        "gpu_evals_per_s": n / T, "gpu_gbs_reference_bytes": alg / T / 1e9,
+                        "hbm_frac": alg / T / 1e9 / HBM_PEAK_GBS, "cpu": cpu}
+                if cpu:
+                    cell["gpu_over_cpu"] = cell["gpu_evals_per_s"] / cpu["evals_per_s"]
+                log(f"bench: matrix {cell['group']} {cell['size']}: {T * 1e3:.3f} ms, "
+                    f"{n / T / 1e9:.2f} G evals/s" + (f", CPU {cpu['evals_per_s'] / 1e6:.1f} M evals/s" if cpu else ""))
+                cells.append(cell)
+    for ext in (1, 2, 3):
+        for v in (16, 20, 24):
+            n = 1 << v
+            src = dev_table(n * ext, SEED_ML)
+            work = N.u64p()
+            N.check(L.effsc_hip_device_alloc(dev, n * ext, C.byref(work)))
+            wt = np.array([(0x1234567 + 99 * i) % O.P for i in range(ext)], dtype=np.uint64)
+            newlen = C.c_size_t()
+
+            def fold_once():
+                N.check(L.effsc_hip_copy(dev, work, src, n * ext))
+                N.check(L.effsc_hip_synchronize(dev))
+                t0 = time.perf_counter()
+                if ext == 1:
+                    N.check(L.effsc_hip_fold(dev, work, n, int(wt[0]), C.byref(newlen)))
+                else:
+                    N.check(L.effsc_hip_fold_ext(dev, ext, W_FROM_RAW7, work, n, wt.ctypes.data_as(N.u64p),
+                                                 C.byref(newlen)))
+                return time.perf_counter() - t0
+            for _ in range(warmup):
+                fold_once()
+            T = float(np.median([fold_once() for _ in range(steps)]))
+            L.effsc_hip_device_free(dev, src)
+            L.effsc_hip_device_free(dev, work)
+
+            def factory(cv, ext=ext, wt=wt):
+                m = 1 << cv
+                tb = O.uniform(m * ext, SEED_ML)
+
+                def run():
+                    p = O.Prover.multilinear(tb, ext=ext)
+                    t0 = time.perf_counter()
+                    p.finalize([int(x) for x in wt])  # fold(&mut evals, w)
+                    return time.perf_counter() - t0
+                return run
+            cpu = None if args.no_cpu else cpu_cell(factory, v)
+            alg = 8 * ext * (n + n // 2)
+            cell = {"group": f"fold/{fields[ext]}", "size": f"2^{v}", "gpu_ms": T * 1e3, "gpu_evals_per_s": n / T,
+                    "gpu_gbs_reference_bytes": alg / T / 1e9, "hbm_frac": alg / T / 1e9 / HBM_PEAK_GBS, "cpu": cpu,
+                    "timing": "one call including its launch and device synchronisation"}
+            if cpu:
+                cell["gpu_over_cpu"] = cell["gpu_evals_per_s"] / cpu["evals_per_s"]
+            log(f"bench: matrix {cell['group']} {cell['size']}: {T * 1e3:.3f} ms")
+            cells.append(cell)
+    head = next(c for c in cells if c["group"] == "inner_product/F64" and c["size"] == "2^24")
+    return {"metric": "Goldilocks sumcheck evals/s, reference criterion matrix (headline cell inner_product/F64 2^24)",
+            "value": head["gpu_evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": head["gpu_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u64", "data": "synthetic",
+            "config": {"workload": "reference_bench_matrix", "transcript": "test RNG transcript",
+                       "sizes": [16, 20, 24], "fields": ["F64", "F64Ext2", "F64Ext3"]},
+            "cells": cells, "cpu": cpu_info()}
 
 
 def serve_mix(count, seed):
@@ -642,7 +861,7 @@ def bench_serve(args):
     # requests measured alone; a server limited to --learn-slots requests at
     # once admits a burst of every request shortest-predicted first, against
     # first come first served
-    from effsc.learn import ServiceTimeModel, measure_service_times
+    from effsc.learn import ServiceTimeModel, measure_service_times, work_priority
     grid = [(k, v) for k in (MULTILINEAR, INNER_PRODUCT, PRODUCT3_EXT3) for v in range(10, 27)]
     cal = Server(hbm_budget=int(args.budget_gib * 2**30), max_active=1)
     samples = measure_service_times(cal, grid, lambda i, k: effsc.Transcript(b"cal", ext=EXT[k]))
@@ -652,6 +871,9 @@ def bench_serve(args):
     fit_loss = model.fit(samples, steps=3000, lr=1e-2)
     t_fit = time.perf_counter() - t_fit
     prio = [model.priority(k, v) for k, v, _ in reqs]
+    # the non-learned ordering on the same features: shortest work first,
+    # work = 2^v x tables x words per element
+    heur = [work_priority(k, v) for k, v, _ in reqs]
     lim = Server(hbm_budget=int(args.budget_gib * 2**30), max_active=args.learn_slots)
 
     def burst(prios):
@@ -662,6 +884,7 @@ def bench_serve(args):
     burst([0.0] * len(reqs))  # warm-up
     fifo, T_fifo = burst([0.0] * len(reqs))
     lrn, T_lrn = burst(prio)
+    swf, T_swf = burst(heur)
     lim.close()
 
     def lat_stats(rs):
@@ -670,8 +893,10 @@ def bench_serve(args):
     learned = {"slots": args.learn_slots, "calibration_requests": len(samples), "fit_loss_log_ms": fit_loss,
                "train_s": t_fit, "fifo": dict(lat_stats(fifo), proofs_per_s=len(reqs) / T_fifo),
                "learned": dict(lat_stats(lrn), proofs_per_s=len(reqs) / T_lrn),
-               "identical_proofs": all(a.proof == b.proof for a, b in zip(fifo, lrn))}
-    log(f"bench: learned admission mean {learned['learned']['mean_ms']:.2f} ms vs FIFO {learned['fifo']['mean_ms']:.2f} ms")
+               "heuristic_shortest_work_first": dict(lat_stats(swf), proofs_per_s=len(reqs) / T_swf),
+               "identical_proofs": all(a.proof == b.proof == c.proof for a, b, c in zip(fifo, lrn, swf))}
+    log(f"bench: admission mean latency: learned {learned['learned']['mean_ms']:.2f} ms, shortest work first "
+        f"{learned['heuristic_shortest_work_first']['mean_ms']:.2f} ms, FIFO {learned['fifo']['mean_ms']:.2f} ms")
     # 3. baseline: the same requests one by one, per-request device-resident proofs
     provers = []
     for (k, v, _), a in zip(reqs, addrs):
@@ -702,7 +927,7 @@ def bench_serve(args):
         [[norm(y) for y in r] for r in q.round_polys] == [[norm(y) for y in r] for r in s.proof.round_polys]
         and norm(q.final_value) == norm(s.proof.final_value) for q, s in zip(single, served))
     alg = sum(algorithmic_bytes(v, tables=K[k], in_words=1, work_words=EXT[k]) for k, v, _ in reqs)
-    print(json.dumps({
+    res = {
         "metric": "served proofs/s and p50/p99 latency (Poisson arrivals, 40% ML / 40% IP / 20% P3-ext3, "
                   "v log-uniform [10,26])",
         "value": sat_rate, "unit": "proofs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -718,10 +943,11 @@ def bench_serve(args):
                     "p99_ms_by_kind": {str(k): float(np.percentile(x, 99)) for k, x in by_kind.items()}},
         "per_instance_launches_proofs_per_s": len(reqs) / T_single, "speedup_vs_per_instance": T_single / T_sat,
         "bit_identical_to_per_instance": identical, "poisson_equals_saturated": same_as_sat,
-        "server": st, "gbps_reference_bytes": alg / T_sat / 1e9, "learned_admission": learned}), flush=True)
+        "server": st, "gbps_reference_bytes": alg / T_sat / 1e9, "learned_admission": learned}
     for tsl in tabs:
         for t in tsl:
             t.free()
+    return res
 
 
 def main():
@@ -732,7 +958,8 @@ def main():
     ap.add_argument("--v", type=int, default=None, help="number of variables (default: the workload's)")
     ap.add_argument("--cpu-v", type=int, default=None, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serve", "stream"], default="ip",
+    ap.add_argument("--no-extras", action="store_true", help="ip: skip the batched and served parts of the line")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serve", "stream", "matrix"], default="ip",
                     help="ip: the v=28 inner-product proof (default, the headline line)")
     ap.add_argument("--count", type=int, default=512, help="batch instances / served requests")
     ap.add_argument("--load", type=float, default=0.7, help="serve: offered load, fraction of saturated proofs/s")
@@ -751,12 +978,40 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "serve":
-        return bench_serve(args)
-    if args.workload == "stream":
-        return bench_stream(args)
-    return bench_single(args, args.workload)
+        out = bench_batch(args)
+    elif args.workload == "serve":
+        out = bench_serve(args)
+    elif args.workload == "stream":
+        out = bench_stream(args)
+    elif args.workload == "matrix":
+        out = bench_matrix(args)
+    else:
+        out = bench_single(args, args.workload)
+        if args.workload == "ip" and out["config"]["v"] == 28 and not args.no_extras:
+            # the rest of the headline metric: batched proofs/s (512
+            # instances, autotuned tiles) and served p50/p99 + proofs/s
+            out["batched"] = summarize_batch(bench_batch(extras_args(args)))
+            out["served"] = summarize_serve(bench_serve(extras_args(args)))
+    print(json.dumps(out), flush=True)
+
+
+def extras_args(args):
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.count = 5, 1, 512
+    return a
+
+
+def summarize_batch(r):
+    keep = ("metric", "value", "unit", "ms_per_step", "steps", "config", "per_instance_launches_proofs_per_s",
+            "per_instance_ms", "speedup", "bit_identical_to_per_instance", "autotune", "gbps_reference_bytes")
+    return {k: r[k] for k in keep if k in r}
+
+
+def summarize_serve(r):
+    keep = ("metric", "value", "unit", "config", "saturated_proofs_per_s", "poisson",
+            "per_instance_launches_proofs_per_s", "speedup_vs_per_instance", "bit_identical_to_per_instance",
+            "poisson_equals_saturated", "learned_admission")
+    return {k: r[k] for k in keep if k in r}
 
 
 if __name__ == "__main__":

@@ -27,6 +27,12 @@ def features(kind, v):
             (v / 26.0) ** 2, math.log2(work) / 40.0, work / 2.0 ** 30, 1.0]
 
 
+def work_priority(kind, v):
+    """The non-learned ordering on the same features: log of the request's
+    work, 2^v x tables x words per element (shortest work first)."""
+    return math.log(_WORK[kind] * float(1 << v))
+
+
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 

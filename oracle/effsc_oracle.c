@@ -1028,25 +1028,129 @@ static double now_s(void) {
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return ts.tv_sec + ts.tv_nsec * 1e-9;
 }
-double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, uint64_t *round_polys,
-                   uint64_t *challenges, uint64_t *fin) {
+/* compute_sumcheck_polynomial (src/inner_product_sumcheck.rs:88-141) with
+ * the rayon split of its pairs: q(0) = sum a_lo b_lo, q(inf) = sum
+ * (a_hi - a_lo)(b_hi - b_lo) over the paired prefix, plus the unpaired tail
+ * (hi = 0) on both values. */
+static void bench_eval_par(const uint64_t *a, const uint64_t *b, size_t n, int threads, uint64_t *q0o, uint64_t *qio) {
+    if (n <= 1 || threads <= 1) {
+        bench_eval(a, b, n, q0o, qio);
+        return;
+    }
+    size_t half = next_pow2(n) >> 1, paired = n - half;
+    uint64_t q0 = 0, qi = 0;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) if (paired > 1024)
+#endif
+    {
+        uint64_t s0 = 0, si = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 1024)
+#endif
+        for (size_t i = 0; i < paired; i++) {
+            uint64_t al = a[i], ah = a[half + i], bl = b[i], bh = b[half + i];
+            s0 = m_add(s0, m_mul(al, bl));
+            si = m_add(si, m_mul(m_sub(ah, al), m_sub(bh, bl)));
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            q0 = m_add(q0, s0);
+            qi = m_add(qi, si);
+        }
+    }
+    uint64_t tail = 0;
+    for (size_t i = paired; i < half; i++) tail = m_add(tail, m_mul(a[i], b[i]));
+    *q0o = m_add(q0, tail);
+    *qio = m_add(qi, tail);
+}
+
+/* fused_fold_and_compute_polynomial (src/inner_product_sumcheck.rs:199-293):
+ * one sweep over the quadruples (k, k + L/4, k + L/2, k + 3L/4) folds both
+ * tables with w into [0, L/2) and accumulates the next round's
+ * (q(0), q(inf)); the reference splits the quadruples with rayon joins down
+ * to leaves of 1024 (workload_size: 32 KiB of 8-byte elements / 4).  Small
+ * or non-power-of-two tables take the unfused fold + evaluate. */
+static void bench_fused_round(uint64_t *a, uint64_t *b, size_t *len, uint64_t w, int threads, uint64_t *q0o,
+                              uint64_t *qio) {
+    size_t l = *len;
+    if (l < 4 || (l & (l - 1))) {
+        size_t lb = l;
+        bench_fold(a, len, w, threads);
+        bench_fold(b, &lb, w, threads);
+        bench_eval_par(a, b, *len, threads, q0o, qio);
+        return;
+    }
+    size_t q = l / 4;
+    uint64_t c0 = 0, c2 = 0;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) if (threads > 1 && q > 1024)
+#endif
+    {
+        uint64_t s0 = 0, s2 = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 1024)
+#endif
+        for (size_t k = 0; k < q; k++) {
+            uint64_t x0 = a[k], x1 = a[k + q], x2 = a[k + 2 * q], x3 = a[k + 3 * q];
+            uint64_t y0 = b[k], y1 = b[k + q], y2 = b[k + 2 * q], y3 = b[k + 3 * q];
+            uint64_t na_lo = m_add(x0, m_mul(m_sub(x2, x0), w)), na_hi = m_add(x1, m_mul(m_sub(x3, x1), w));
+            uint64_t nb_lo = m_add(y0, m_mul(m_sub(y2, y0), w)), nb_hi = m_add(y1, m_mul(m_sub(y3, y1), w));
+            a[k] = na_lo;
+            a[k + q] = na_hi;
+            b[k] = nb_lo;
+            b[k + q] = nb_hi;
+            s0 = m_add(s0, m_mul(na_lo, nb_lo));
+            s2 = m_add(s2, m_mul(m_sub(na_hi, na_lo), m_sub(nb_hi, nb_lo)));
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            c0 = m_add(c0, s0);
+            c2 = m_add(c2, s2);
+        }
+    }
+    *len = l / 2;
+    *q0o = c0;
+    *qio = c2;
+}
+
+double or_bench_ip_ex(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, int fused,
+                      const uint8_t *domsep, size_t domsep_len, uint64_t *round_polys, uint64_t *challenges,
+                      uint64_t *fin) {
     for (size_t i = 0; i < n; i++) {
         a[i] = m_to(a[i]);
         b[i] = m_to(b[i]);
     }
-    static const char dom[] = "effsc-hip/bench/ip";
-    or_transcript *t = or_tr_new_shake((const uint8_t *)dom, sizeof(dom) - 1);
+    or_transcript *t = or_tr_new_shake(domsep, domsep_len);
     double t0 = now_s();
     size_t la = n, lb = n;
     uint64_t prev = 0;
     int have = 0;
     for (size_t r = 0; r < num_rounds; r++) {
-        if (have) {
-            bench_fold(a, &la, prev, threads);
-            bench_fold(b, &lb, prev, threads);
-        }
         uint64_t q[2];
-        bench_eval(a, b, la, &q[0], &q[1]);
+        if (fused) {
+            /* inner_product_sumcheck_partial (src/inner_product_sumcheck.rs:305-360):
+             * the round-(i-1) fold fused into the round-i evaluation */
+            if (have) {
+                bench_fused_round(a, b, &la, prev, threads, &q[0], &q[1]);
+                lb = la;
+            } else {
+                bench_eval_par(a, b, la, threads, &q[0], &q[1]);
+            }
+        } else {
+            /* InnerProductProver::round (src/provers/inner_product.rs:61-113):
+             * two parallel folds, then the serial evaluate */
+            if (have) {
+                bench_fold(a, &la, prev, threads);
+                bench_fold(b, &lb, prev, threads);
+            }
+            bench_eval(a, b, la, &q[0], &q[1]);
+        }
         q[0] = m_from(q[0]);
         q[1] = m_from(q[1]);
         or_tr_send(t, 1, &q[0]);
@@ -1067,4 +1171,11 @@ double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int th
     double el_s = now_s() - t0;
     or_tr_free(t);
     return el_s;
+}
+
+double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, uint64_t *round_polys,
+                   uint64_t *challenges, uint64_t *fin) {
+    static const char dom[] = "effsc-hip/bench/ip";
+    return or_bench_ip_ex(a, b, n, num_rounds, threads, 0, (const uint8_t *)dom, sizeof(dom) - 1, round_polys,
+                          challenges, fin);
 }

@@ -102,6 +102,14 @@ int or_sumcheck_verify(int ext, uint64_t w, const uint64_t *claimed_sum, size_t 
  * src/provers/inner_product.rs:94-103 does. Returns wall seconds. */
 double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads,
                    uint64_t *round_polys, uint64_t *challenges, uint64_t *fin);
+/* The same proof with the transcript's domain separator given, and with
+ * fused = 1 the reference's faster legacy path instead
+ * (inner_product_sumcheck_partial + fused_fold_and_compute_polynomial,
+ * src/inner_product_sumcheck.rs:199-360: fold and evaluate in one parallel
+ * sweep per round).  Both produce the InnerProductProver wire. */
+double or_bench_ip_ex(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, int fused,
+                      const uint8_t *domsep, size_t domsep_len, uint64_t *round_polys, uint64_t *challenges,
+                      uint64_t *fin);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ def lib():
             "or_sumcheck": (None, [vp, sz, vp, u64p, u64p, u64p]),
             "or_sumcheck_verify": (C.c_int, [C.c_int, u64, u64p, sz, sz, vp, u64p, sz, u64p, u64p]),
             "or_bench_ip": (C.c_double, [u64p, u64p, sz, sz, C.c_int, u64p, u64p, u64p]),
+            "or_bench_ip_ex": (C.c_double, [u64p, u64p, sz, sz, C.c_int, C.c_int, C.c_char_p, sz, u64p, u64p,
+                                            u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -279,13 +281,16 @@ def verify(claimed_sum, degree, num_rounds, transcript, proof_words, ext=1, w=W_
     return st, ch, [int(x) for x in fin]
 
 
-def bench_ip(a, b, num_rounds, threads=1):
-    """Times the reference-structured inner-product proof; a, b are consumed."""
+def bench_ip(a, b, num_rounds, threads=1, fused=False, domsep=b"effsc-hip/bench/ip"):
+    """Times a reference-structured inner-product proof; a, b are consumed.
+    fused=False: InnerProductProver's round (two parallel folds, serial
+    evaluate); fused=True: the reference's legacy fused fold + evaluate."""
     a = np.ascontiguousarray(a, dtype=np.uint64)
     b = np.ascontiguousarray(b, dtype=np.uint64)
     polys = np.zeros(2 * max(num_rounds, 1), dtype=np.uint64)
     chals = np.zeros(max(num_rounds, 1), dtype=np.uint64)
     fin = np.zeros(1, dtype=np.uint64)
-    t = lib().or_bench_ip(a.ctypes.data_as(u64p), b.ctypes.data_as(u64p), len(a), num_rounds, threads,
-                          polys.ctypes.data_as(u64p), chals.ctypes.data_as(u64p), fin.ctypes.data_as(u64p))
+    t = lib().or_bench_ip_ex(a.ctypes.data_as(u64p), b.ctypes.data_as(u64p), len(a), num_rounds, threads,
+                             1 if fused else 0, domsep, len(domsep), polys.ctypes.data_as(u64p),
+                             chals.ctypes.data_as(u64p), fin.ctypes.data_as(u64p))
     return t, polys, chals, int(fin[0])

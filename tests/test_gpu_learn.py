@@ -9,7 +9,7 @@ import pytest
 
 import mlp as M
 from effsc import Transcript
-from effsc.learn import INPUTS, Network, ServiceTimeModel, features, measure_service_times
+from effsc.learn import INPUTS, Network, ServiceTimeModel, features, measure_service_times, work_priority
 from effsc.serve import INNER_PRODUCT, MULTILINEAR, PRODUCT3_EXT3, Server
 
 pytestmark = pytest.mark.gpu
@@ -84,8 +84,12 @@ def test_learned_admission_beats_fifo_with_identical_proofs():
         _run_burst(srv, reqs[:4], [0.0] * 4)  # warm-up
         fifo = _run_burst(srv, reqs, [0.0] * len(reqs))
         learned = _run_burst(srv, reqs, [model.priority(k, v) for k, v in reqs])
+        # the non-learned ordering on the same features: shortest work first
+        swf = _run_burst(srv, reqs, [work_priority(k, v) for k, v in reqs])
     lat_f = np.mean([r.latency for r in fifo])
     lat_l = np.mean([r.latency for r in learned])
-    say(f"mean latency FIFO {lat_f * 1e3:.3f} ms, learned {lat_l * 1e3:.3f} ms")
-    assert all(a.proof == b.proof for a, b in zip(fifo, learned))
+    lat_h = np.mean([r.latency for r in swf])
+    say(f"mean latency FIFO {lat_f * 1e3:.3f} ms, shortest work first {lat_h * 1e3:.3f} ms, "
+        f"learned {lat_l * 1e3:.3f} ms")
+    assert all(a.proof == b.proof == c.proof for a, b, c in zip(fifo, learned, swf))
     assert lat_l < 0.85 * lat_f

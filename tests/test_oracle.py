@@ -377,3 +377,33 @@ def test_base_coefficient_form_of_first_fold():
         got = [F.add(F.add(F.lift(e[t][0]), F.scale(c, e[t][1])), F.add(F.scale(c2, e[t][2]), F.scale(c3, e[t][3])))
                for t in range(3)]
         assert got == want
+
+
+# ------------------------------------------------------- threaded CPU baselines
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("threads", [1, 8])
+@pytest.mark.parametrize("v", [1, 2, 5, 11, 16])
+def test_bench_ip_equals_prover_and_reference_order(v, threads, fused):
+    # the timed CPU baselines (InnerProductProver round structure, and the
+    # legacy fused path of src/inner_product_sumcheck.rs:199-360) must produce
+    # the oracle prover's proof: they check the full-size GPU proofs
+    n = 1 << v
+    a, b = O.uniform(n, 0x77 + v), O.uniform(n, 0x78 + v)
+    rp, ch, fin = O.Prover.inner_product(a, b).sumcheck(v, O.Transcript.shake(b"effsc-hip/bench/ip"))
+    _, polys, chals, f = O.bench_ip(a.copy(), b.copy(), v, threads, fused=fused)
+    assert [int(x) for x in polys[:2 * v]] == [x[0] for r in rp for x in r]
+    assert [int(x) for x in chals[:v]] == [c[0] for c in ch]
+    assert [f] == fin
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_bench_ip_matches_brute_force_golden(golden, fused):
+    for case in golden["proofs"]:
+        if case["kind"] != "ip" or case["ext"] != 1 or case["n"] != 1 << case["v"] or case["transcript"] != "shake":
+            continue
+        v = case["v"]
+        a, b = O.uniform(case["n"], case["seeds"][0]), O.uniform(case["n"], case["seeds"][1])
+        _, polys, chals, f = O.bench_ip(a, b, v, 4, fused=fused, domsep=case["domsep"].encode())
+        assert [[[int(polys[2 * r])], [int(polys[2 * r + 1])]] for r in range(v)] == case["round_polys"]
+        assert [[int(c)] for c in chals[:v]] == case["challenges"]
+        assert [f] == case["final"]
