@@ -335,8 +335,68 @@ int round_twostep(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   return EFFSC_HIP_OK;
 }
 
+// ------------------------------------------------ native pair-split rounds
+// MultilinearProverLSB over a table of any length (src/provers/
+// multilinear_lsb.rs:59-126): the round value is s0 = sum of the
+// even-indexed elements (an unpaired last element included), the fold is
+// new[k] = f[2k] + w (f[2k+1] - f[2k]) for k < n/2 (an odd trailing element
+// is dropped).  Out of place: thread k reads 2k, 2k+1 and writes k.
+template <class FI, class FO>
+__global__ __launch_bounds__(kBlock) void k_lsb_fold(View<FI> in, View<FO> out, FO c, size_t half, uint64_t w) {
+  for (size_t k = gtid(); k < half; k += (size_t)gridDim.x * blockDim.x)
+    out.st(k, fold1<FO>(in.ld(2 * k), in.ld(2 * k + 1), c, w));
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_lsb_even_sum(View<F> t, size_t n, Reduce r) {
+  F acc[1] = {gl::zero<F>()};
+  for (size_t k = gtid(); 2 * k < n; k += (size_t)gridDim.x * blockDim.x) acc[0] = gl::add(acc[0], t.ld(2 * k));
+  finish<F, 1>(acc, r);
+}
+
+int lsb_native_fold(effsc_hip_prover* h, const uint64_t* chal) {
+  const size_t L = h->len;
+  if (L <= 1) return EFFSC_HIP_OK;
+  const size_t half = L / 2;
+  const int fi = src_field(h);
+  uint64_t* src = src_planes(h)[0];
+  const size_t sst = src_stride(h);
+  uint64_t* dst = h->in_work ? h->work_alt[0] : h->work[0];
+  const unsigned g = blocks_for(h, half);
+  TRY(dispatch(fi, h->ext, 1, [&](auto op) -> int {
+    using FI = typename decltype(op)::In;
+    using FO = typename decltype(op)::Out;
+    hipLaunchKernelGGL((k_lsb_fold<FI, FO>), dim3(g), dim3(kBlock), 0, h->stream, View<FI>{src, sst},
+                       View<FO>{dst, h->work_stride}, from_words<FO>(chal), half, h->w);
+    HIPCHK(hipGetLastError());
+    return EFFSC_HIP_OK;
+  }));
+  if (h->in_work) std::swap(h->work[0], h->work_alt[0]);
+  h->len = half;
+  h->in_work = true;
+  return EFFSC_HIP_OK;
+}
+
+int lsb_native_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
+  if (chal) TRY(lsb_native_fold(h, chal));
+  const int E = h->ext;
+  std::fill(out, out + E, 0);
+  if (h->len == 0) return EFFSC_HIP_OK;
+  const int fi = src_field(h);
+  const unsigned g = blocks_for(h, (h->len + 1) / 2);
+  TRY(with_field(fi, [&](auto f) -> int {
+    using F = decltype(f);
+    hipLaunchKernelGGL(k_lsb_even_sum<F>, dim3(g), dim3(kBlock), 0, h->stream, View<F>{src_planes(h)[0], src_stride(h)},
+                       h->len, h->red);
+    HIPCHK(hipGetLastError());
+    return reduce<F, 1>(h, g);
+  }));
+  return fetch_result(h, fi, out, 1);
+}
+
 int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
   if (h->kind == K_EQ) return eq_round(h, chal, out);
+  if (h->lsb_native) return lsb_native_round(h, chal, out);
   if (h->kind == K_GKR) return gkr_round(h, chal, out);
   if (h->twostep) return round_twostep(h, chal, out);
   bool evaluated = false;
@@ -349,6 +409,7 @@ int round_impl(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out) {
 
 int finalize_impl(effsc_hip_prover* h, const uint64_t* chal) {
   if (h->kind == K_EQ) return eq_finalize(h, chal);
+  if (h->lsb_native) return lsb_native_fold(h, chal);
   h->have_next = false;
   TRY(materialize(h));
   bool evaluated = false;
@@ -400,6 +461,7 @@ void destroy(effsc_hip_prover* h) {
   for (int i = 0; i < kMaxK; i++) {
     if (h->own_in && h->in[i]) (void)hipFree(h->in[i]);
     if (h->work[i]) (void)hipFree(h->work[i]);
+    if (h->work_alt[i]) (void)hipFree(h->work_alt[i]);
   }
   if (h->red.partials) (void)hipFree(h->red.partials);
   if (h->red.result) (void)hipFree(h->red.result);
@@ -563,10 +625,24 @@ int new_lsb(const effsc_hip_config* cfg, int kind, int K, int deg, const uint64_
             effsc_hip_prover** out) {
   if (out) *out = nullptr;
   if (!cfg) return fail(EFFSC_HIP_EINVAL, "null config");
-  if (!is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "LSB provers take power-of-two tables");
   effsc_hip_config c2 = *cfg;
   c2.flags &= ~EFFSC_HIP_BORROW_INPUT;  // the reordered tables are the prover's own
   effsc_hip_prover* h = nullptr;
+  if (!is_pow2(n)) {
+    // the reference's IP and coefficient LSB provers panic on such lengths;
+    // MultilinearProverLSB drops the unpaired element at each fold
+    if (kind != K_ML) return fail(EFFSC_HIP_EINVAL, "LSB inner-product and product provers take power-of-two tables");
+    TRY(new_product_like(&c2, kind, K, deg, tables, n, &h));
+    h->twostep = false;
+    h->lsb_native = true;
+    int rc = alloc_dev(&h->work_alt[0], h->work_stride * h->ext);
+    if (rc != EFFSC_HIP_OK) {
+      destroy(h);
+      return rc;
+    }
+    *out = h;
+    return EFFSC_HIP_OK;
+  }
   TRY(new_product_like(&c2, kind, K, deg, tables, n, &h));
   int rc = EFFSC_HIP_OK;
   for (int i = 0; i < K && rc == EFFSC_HIP_OK; i++) {
@@ -639,7 +715,6 @@ int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* ta
                           effsc_hip_prover** out) {
   if (out) *out = nullptr;
   if (!tables) return fail(EFFSC_HIP_EINVAL, "null tables");
-  if (n && !is_pow2(n)) return fail(EFFSC_HIP_EINVAL, "product prover tables must have power-of-two length");
   return new_product_like(cfg, K_PRODUCT, k, k, tables, n, out);
 }
 
@@ -737,7 +812,7 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
   HIPCHK(hipStreamSynchronize(h->stream));
   // LSB provers: element i of the reference's table is element rev(i) here
   int lv = 0;
-  while (h->lsb && ((size_t)1 << lv) < L) lv++;
+  while (h->lsb && !h->lsb_native && ((size_t)1 << lv) < L) lv++;
   for (size_t i = 0; i < L; i++) {
     const size_t j = (h->lsb && lv) ? (size_t)(gl_brev64(i) >> (64 - lv)) : i;
     for (int c = 0; c < E; c++) {

@@ -94,7 +94,7 @@ namespace effsc_hip {
 namespace pv {
 
 bool device_loop_applies(const effsc_hip_prover* h, size_t num_rounds) {
-  if (h->kind == K_EQ || h->kind == K_GKR || num_rounds == 0 || num_rounds > kMaxRounds) return false;
+  if (h->kind == K_EQ || h->kind == K_GKR || h->lsb_native || num_rounds == 0 || num_rounds > kMaxRounds) return false;
   if (!is_pow2(h->len)) return false;
   size_t logl = 0;
   while (((size_t)1 << logl) < h->len) logl++;

@@ -401,9 +401,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) 
 #pragma unroll
   for (int i = 0; i < K; i++) acc[i] = gl::zero<FO>();
   const size_t step = (size_t)gridDim.x * blockDim.x * VEC;
+  // the table loop stays rolled for extension outputs: unrolled, the compiler
+  // hoisted every table's loads, and the one-quadruple F_p^3 form spilled at
+  // the two-wave cap of 256 VGPRs
+  constexpr int UT = FO::D >= 2 ? 1 : K;
   for (size_t k = gtid() * VEC; k < q; k += step) {
     FO p[VEC][K];
-#pragma unroll
+#pragma unroll UT
     for (int i = 0; i < K; i++) {
       FI x0[VEC], x1[VEC], x2[VEC], x3[VEC];
       if constexpr (VEC == 2) {

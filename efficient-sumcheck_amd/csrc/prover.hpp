@@ -83,6 +83,11 @@ struct effsc_hip_prover {
   // LSB (pair-split) provers hold their tables bit-reversed and run the MSB
   // path; post-state reads reverse back (src/streams/memory/core.rs:20-47)
   bool lsb = false;
+  // MultilinearProverLSB over a table whose length is not a power of two
+  // (src/provers/multilinear_lsb.rs:59-126): no bit reversal; native
+  // pair-split rounds ping-pong between work and work_alt
+  bool lsb_native = false;
+  uint64_t* work_alt[effsc_hip::kMaxK] = {};
   // Provers built from streams: the first two-round pass (round 0 values and
   // round 1 coefficients over the input) was accumulated during the upload
   // and its reduced values live in d_pass0 (the input is never written, so
@@ -237,6 +242,8 @@ int reduce(effsc_hip_prover* h, unsigned nblocks) {
 // fold: fold only.
 template <class FI, class FO, int K>
 struct Ops {
+  using In = FI;
+  using Out = FO;
   static int eval(effsc_hip_prover* h, uint64_t* const* planes, size_t stride, size_t half, size_t paired);
   static int fused(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t q,
                    const uint64_t* cptr = nullptr);

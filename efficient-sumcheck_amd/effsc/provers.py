@@ -263,7 +263,8 @@ def _lsb(cls_fn, tables, ext, input_ext, repr, w, device, borrow, n):
 
 class MultilinearProverLSB(NativeProver):
     """MultilinearProverLSB (src/provers/multilinear_lsb.rs): pairs (2k, 2k+1),
-    degree 1; power-of-two tables."""
+    degree 1; any length (an unpaired last element counts in the round value
+    and is dropped by the fold, as in the reference)."""
 
     def __init__(self, evals, *, ext=1, input_ext=None, repr=CANONICAL, w=W_FROM_RAW7, device=0, borrow=False,
                  n=None):

@@ -90,7 +90,10 @@ int effsc_hip_inner_product_new(const effsc_hip_config* cfg, const uint64_t* a, 
  * product-of-k evaluator, sum_x prod_i f_i(x) (degree k, 1 <= k <= 4) —
  * src/provers/coefficient.rs:28-40; the k = 3 evaluator is the reference's
  * TripleProductEval (tests/adversarial_verifier.rs:444-465).  All k tables
- * have length n, a power of two. */
+ * have length n, any n: each round pairs k with k + n/2 and the fold keeps
+ * n/2 elements, dropping an odd last one (msb_fold_vec,
+ * src/provers/coefficient.rs:131-140).  Tablewise tables belong to
+ * user-defined evaluators, which do not cross this ABI. */
 int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* tables, int k, size_t n,
                           effsc_hip_prover** out);
 
@@ -98,10 +101,15 @@ int effsc_hip_product_new(const effsc_hip_config* cfg, const uint64_t* const* ta
  * bound first.  MultilinearProverLSB::new (src/provers/multilinear_lsb.rs:33-36),
  * InnerProductProverLSB::new (src/provers/inner_product_lsb.rs),
  * CoefficientProverLSB::new with the product-of-k evaluator
- * (src/provers/coefficient_lsb.rs).  Tables have power-of-two length; the
- * prover bit-reverses them once into its own device planes (so
- * EFFSC_HIP_BORROW_INPUT sources are read, never kept) and post-state reads
- * return the reference's pair-split order. */
+ * (src/provers/coefficient_lsb.rs).  For power-of-two tables the prover
+ * bit-reverses them once into its own device planes (so
+ * EFFSC_HIP_BORROW_INPUT sources are read, never kept) and runs the MSB path;
+ * post-state reads return the reference's pair-split order.  The multilinear
+ * prover also takes any other length, as the reference does
+ * (src/provers/multilinear_lsb.rs:59-126): the round value sums the
+ * even-indexed elements, the fold drops an unpaired last element; the
+ * inner-product and product provers reject such lengths (EINVAL), where the
+ * reference panics. */
 int effsc_hip_multilinear_lsb_new(const effsc_hip_config* cfg, const uint64_t* evals, size_t n,
                                   effsc_hip_prover** out);
 int effsc_hip_inner_product_lsb_new(const effsc_hip_config* cfg, const uint64_t* a, const uint64_t* b, size_t n,

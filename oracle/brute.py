@@ -259,3 +259,73 @@ def prove_lsb(F, tables, v, transcript, degree, lift=True):
             acc = F.mul(acc, mle(F, T, zr))
         return acc
     return prove(F, summand, v, degree, transcript)
+
+
+def _wire(F, evals, degree):
+    if degree <= 1:
+        return [evals[0]]
+    return [evals[0], _lead_coeff(F, evals, degree)] + [evals[i] for i in range(2, degree)]
+
+
+def prove_halving(F, tables, num_rounds, transcript, lift=True):
+    """CoefficientProver with the product evaluator over tables of any length
+    (src/provers/coefficient.rs:66-140, 272-329), from its definition: round
+    j pairs k with k + h, h = floor(L_j / 2), for k < h (an odd last element
+    takes no part); the round polynomial is
+        g_j(t) = sum_{k < h} prod_i ((1 - t) T_i[k] + t T_i[k + h]),
+    evaluated at t = 0..K and sent as g(0), its leading coefficient, g(2), ...;
+    binding r keeps T_i[k] <- (1 - r) T_i[k] + r T_i[k + h] for k < h."""
+    Ts = [[F.lift(x) if lift else tuple(x) for x in t] for t in tables]
+    degree = len(Ts)
+    polys, chals = [], []
+    for _ in range(num_rounds):
+        h = len(Ts[0]) // 2
+        evals = []
+        for t in range(degree + 1):
+            tt = F.lift(t)
+            omt = F.sub(F.one(), tt)
+            acc = F.zero()
+            for k in range(h):
+                term = F.one()
+                for T in Ts:
+                    term = F.mul(term, F.add(F.mul(omt, T[k]), F.mul(tt, T[k + h])))
+                acc = F.add(acc, term)
+            evals.append(acc)
+        wire = _wire(F, evals, degree)
+        for x in wire:
+            transcript.send(x)
+        polys.append(wire)
+        r = transcript.challenge(F.d)
+        chals.append(r)
+        omr = F.sub(F.one(), r)
+        if len(Ts[0]) > 1:  # a table of one element is left as it is
+            Ts = [[F.add(F.mul(omr, T[k]), F.mul(r, T[k + h])) for k in range(h)] for T in Ts]
+    final = F.zero()
+    if len(Ts[0]) == 1:
+        final = F.one()
+        for T in Ts:
+            final = F.mul(final, T[0])
+    return polys, chals, final
+
+
+def prove_ml_pairs(F, table, num_rounds, transcript, lift=True):
+    """MultilinearProverLSB over a table of any length
+    (src/provers/multilinear_lsb.rs:59-126, 168-192), from its definition:
+    the round value is the sum of the even-indexed elements (an unpaired last
+    element included), and binding r keeps T[k] <- (1 - r) T[2k] + r T[2k+1]
+    for k < floor(L / 2) (a one-element table stays)."""
+    T = [F.lift(x) if lift else tuple(x) for x in table]
+    polys, chals = [], []
+    for _ in range(num_rounds):
+        s0 = F.zero()
+        for i in range(0, len(T), 2):
+            s0 = F.add(s0, T[i])
+        transcript.send(s0)
+        polys.append([s0])
+        r = transcript.challenge(F.d)
+        chals.append(r)
+        omr = F.sub(F.one(), r)
+        if len(T) > 1:  # a table of one element is left as it is
+            T = [F.add(F.mul(omr, T[2 * k]), F.mul(r, T[2 * k + 1])) for k in range(len(T) // 2)]
+    final = T[0] if len(T) == 1 else F.zero()
+    return polys, chals, final, T

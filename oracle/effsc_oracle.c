@@ -526,6 +526,11 @@ or_prover *or_prover_new_product_lsb(int ext, uint64_t w, int k, const uint64_t 
     if (p) p->lsb = 1;
     return p;
 }
+or_prover *or_prover_new_product_lsb_ext(int ext, uint64_t w, int k, const uint64_t *const *tables, size_t n) {
+    or_prover *p = or_prover_new_product_ext(ext, w, k, tables, n);
+    if (p) p->lsb = 1;
+    return p;
+}
 
 int or_prover_degree(const or_prover *p) { return p->deg; }
 

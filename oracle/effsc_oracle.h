@@ -73,6 +73,8 @@ or_prover *or_prover_new_eq(int ext, uint64_t w, const uint64_t *point, size_t v
 or_prover *or_prover_new_ml_lsb(int ext, uint64_t w, const uint64_t *evals, size_t n);
 or_prover *or_prover_new_ip_lsb(int ext, uint64_t w, const uint64_t *a, const uint64_t *b, size_t n);
 or_prover *or_prover_new_product_lsb(int ext, uint64_t w, int k, const uint64_t *const *tables, size_t n);
+/* Same, each table already holding n extension elements (ext words each). */
+or_prover *or_prover_new_product_lsb_ext(int ext, uint64_t w, int k, const uint64_t *const *tables, size_t n);
 /* GkrProver: add and mult hold n*n elements over (b, c), wit holds n
  * elements W(x); tables 0..3 are add, mult, w_b, w_c (ext words each). */
 or_prover *or_prover_new_gkr(int ext, uint64_t w, const uint64_t *add, const uint64_t *mult, const uint64_t *wit,

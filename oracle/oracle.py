@@ -61,6 +61,7 @@ def lib():
             "or_prover_new_ip": (vp, [C.c_int, u64, u64p, u64p, sz]),
             "or_prover_new_product": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_new_product_ext": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
+            "or_prover_new_product_lsb_ext": (vp, [C.c_int, u64, C.c_int, C.POINTER(u64p), sz]),
             "or_prover_new_eq": (vp, [C.c_int, u64, u64p, sz, u64p, sz]),
             "or_prover_new_gkr": (vp, [C.c_int, u64, u64p, u64p, u64p, sz]),
             "or_prover_new_ml_lsb": (vp, [C.c_int, u64, u64p, sz]),
@@ -202,10 +203,13 @@ class Prover:
         return cls(lib().or_prover_new_ip_lsb(ext, w, pa, pb, len(A) // ext), ext)
 
     @classmethod
-    def product_lsb(cls, tables, ext=1, w=W_FROM_RAW7):
-        """CoefficientProverLSB with the product evaluator; base tables lifted."""
+    def product_lsb(cls, tables, ext=1, w=W_FROM_RAW7, tables_in_ext=False):
+        """CoefficientProverLSB with the product evaluator; base tables lifted
+        (or tables already in the extension with tables_in_ext)."""
         arrs = [_arr(t) for t in tables]
         ptrs = (u64p * len(arrs))(*[p for _, p in arrs])
+        if tables_in_ext:
+            return cls(lib().or_prover_new_product_lsb_ext(ext, w, len(arrs), ptrs, len(arrs[0][0]) // ext), ext)
         return cls(lib().or_prover_new_product_lsb(ext, w, len(arrs), ptrs, len(arrs[0][0])), ext)
 
     @classmethod

@@ -77,16 +77,14 @@ def make(kind, ext, input_ext, tabs):
         return (InnerProductProverLSB(tabs[0], tabs[1], ext=ext, input_ext=input_ext),
                 O.Prover.inner_product_lsb(lift(tabs[0], ext // input_ext), lift(tabs[1], ext // input_ext), ext=ext))
     k = int(kind[1:])
-    assert input_ext == 1
-    return CoefficientProverLSB(tabs[:k], ext=ext, input_ext=1), O.Prover.product_lsb(tabs[:k], ext=ext)
+    return (CoefficientProverLSB(tabs[:k], ext=ext, input_ext=input_ext),
+            O.Prover.product_lsb(tabs[:k], ext=ext, tables_in_ext=(input_ext == ext and ext > 1)))
 
 
 @pytest.mark.parametrize("kind", ["ml", "ip", "p3"])
 @pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3), (2, 1), (2, 2)])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 6, 10, 13])
 def test_lsb_parity(kind, ext, input_ext, v):
-    if kind == "p3" and input_ext != 1:
-        pytest.skip("product tables are base-field inputs")
     n = 1 << v
     k = {"ml": 1, "ip": 2}.get(kind) or 3
     tabs = [O.uniform(n * input_ext, 7000 + 10 * v + i) for i in range(k)]
@@ -126,6 +124,33 @@ def test_lsb_verifier_accepts():
     assert res.final_claim == proof.final_value
 
 
-def test_lsb_rejects_non_power_of_two():
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (2, 2), (3, 1), (3, 3)])
+@pytest.mark.parametrize("n", [0, 3, 5, 6, 7, 11, 12, 100, 1000, 4097])
+def test_lsb_multilinear_any_length(n, ext, input_ext):
+    # src/provers/multilinear_lsb.rs:59-126: the round value counts an
+    # unpaired last element, the fold drops it; native pair-split rounds
+    rounds = max(1, (n - 1).bit_length()) if n > 1 else 1
+    t = O.uniform(n * input_ext, 7700 + n)
+    say(f"lsb any length n={n} ext={ext}/{input_ext} rounds={rounds}")
+    g = MultilinearProverLSB(t, ext=ext, input_ext=input_ext)
+    o = O.Prover.multilinear_lsb(lift(t, ext // input_ext), ext=ext)
+    pg = sumcheck(g, rounds, Transcript(b"lsb-any", ext=ext))
+    po = o.sumcheck(rounds, O.Transcript.shake(b"lsb-any"))
+    assert tuple(as_oracle(pg, ext)) == tuple(po)
+    assert np.array_equal(g.table_words(0), o.table(0))
+    # partial rounds, then the rest, on a fresh pair
+    if rounds >= 3:
+        g2 = MultilinearProverLSB(t, ext=ext, input_ext=input_ext)
+        o2 = O.Prover.multilinear_lsb(lift(t, ext // input_ext), ext=ext)
+        tg, to = TestTranscript(3, ext=ext), O.Transcript.rng(3)
+        assert tuple(as_oracle(sumcheck(g2, 2, tg), ext)) == tuple(o2.sumcheck(2, to))
+        assert np.array_equal(g2.table_words(0), o2.table(0))
+
+
+def test_lsb_products_reject_other_lengths():
+    # the reference's InnerProductProverLSB / CoefficientProverLSB panic on
+    # these lengths
     with pytest.raises(N.NativeError):
-        MultilinearProverLSB(O.uniform(12, 1))
+        InnerProductProverLSB([1, 2, 3], [4, 5, 6])
+    with pytest.raises(N.NativeError):
+        CoefficientProverLSB([O.uniform(12, 1), O.uniform(12, 2)])

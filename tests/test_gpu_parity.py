@@ -136,6 +136,16 @@ def test_product_parity(k, ext, input_ext, v):
     check_proof(f"p{k}", ext, input_ext, 1 << v, v, 500 + 10 * k + v)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("ext,input_ext", [(1, 1), (3, 1), (3, 3), (2, 1), (2, 2)])
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 12, 13, 100, 1000, 4097])
+def test_product_non_pow2_parity(k, ext, input_ext, n):
+    # msb_fold_vec (src/provers/coefficient.rs:131-140): half = n/2, an odd
+    # last element is dropped; the rounds run on the per-round path
+    rounds = max(1, (n - 1).bit_length() - 1)
+    check_proof(f"p{k}", ext, input_ext, n, rounds, 3000 + 10 * k + n, rounds=rounds)
+
+
 @pytest.mark.parametrize("ext", [1, 2, 3])
 @pytest.mark.parametrize("v", [0, 1, 2, 3, 5, 8, 13])
 def test_eq_factored_parity(ext, v):
@@ -417,7 +427,7 @@ def test_errors_are_reported_not_raised_natively():
     with pytest.raises(ValueError):
         InnerProductProver([1, 2, 3], [1, 2])
     with pytest.raises(N.NativeError):
-        CoefficientProver([[1, 2, 3]])  # non-power-of-two product table
+        CoefficientProver([[1, 2]] * 5)  # more tables than the built-in evaluators take
     with pytest.raises(N.NativeError):
         EqFactoredProver([1, 2], [1, 2, 3, 4, 5])  # more than 2^v evaluations
 

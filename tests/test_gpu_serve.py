@@ -95,6 +95,35 @@ def test_large_requests_and_test_transcript():
             assert tuple(words(res, EXT[k])) == tuple(oracle_proof(k, v, s, rng_seed=5 + i))
 
 
+def test_served_large_requests_under_poisson_arrivals():
+    # the served configuration's upper range: v = 22..26 of every kind,
+    # arriving as a Poisson process while the earlier ones run, against the
+    # oracle (the inner products through its threaded restatement of the
+    # reference's fused path)
+    rng = random.Random(0x5EED0007)
+    reqs = [(MULTILINEAR, 26), (INNER_PRODUCT, 25), (PRODUCT3_EXT3, 22), (MULTILINEAR, 23), (INNER_PRODUCT, 22),
+            (PRODUCT3_EXT3, 24), (INNER_PRODUCT, 26), (MULTILINEAR, 22)]
+    reqs = [(k, v, [rng.getrandbits(40) for _ in range(3)], b"served-large-%d" % i) for i, (k, v) in enumerate(reqs)]
+    with Server(hbm_budget=24 << 30, max_active=4) as srv:
+        ids = []
+        for i, (k, v, seeds, ds) in enumerate(reqs):
+            say(f"submit large {i} kind={k} v={v}")
+            ids.append(srv.submit(k, v, Transcript(ds, ext=EXT[k]), seeds=seeds))
+            time.sleep(rng.expovariate(1000.0))
+        results = [srv.wait(r) for r in ids]
+    for i, ((k, v, seeds, ds), res) in enumerate(zip(reqs, results)):
+        say(f"check large {i} kind={k} v={v}")
+        if k == INNER_PRODUCT:
+            n = 1 << v
+            _, polys, chals, fin = O.bench_ip(O.uniform(n, seeds[0]), O.uniform(n, seeds[1]), v, 16, fused=True,
+                                              domsep=ds)
+            assert [w for r in res.proof.round_polys for w in r] == [int(x) for x in polys], f"request {i}"
+            assert res.proof.challenges == [int(x) for x in chals]
+            assert res.proof.final_value == fin
+        else:
+            assert tuple(words(res, EXT[k])) == tuple(oracle_proof(k, v, seeds, domsep=ds)), f"request {i}"
+
+
 def test_borrowed_device_tables():
     v = 13
     a, b = DeviceTable.uniform(1 << v, 31), DeviceTable.uniform(1 << v, 32)

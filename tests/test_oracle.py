@@ -407,3 +407,57 @@ def test_bench_ip_matches_brute_force_golden(golden, fused):
         assert [[[int(polys[2 * r])], [int(polys[2 * r + 1])]] for r in range(v)] == case["round_polys"]
         assert [[int(c)] for c in chals[:v]] == case["challenges"]
         assert [f] == case["final"]
+
+
+# ------------------------------------------------ general coefficient shapes
+def _pylist(words, ext):
+    return [int(x) for x in words] if ext == 1 else [tuple(int(x) for x in words[i * ext:(i + 1) * ext])
+                                                    for i in range(len(words) // ext)]
+
+
+@pytest.mark.parametrize("ext", [1, 2, 3])
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 12, 13])
+def test_product_non_pow2_matches_definition(n, k, ext):
+    # msb_fold_vec halves with len / 2 and drops an odd last element
+    # (src/provers/coefficient.rs:131-140)
+    rounds = max(1, (n - 1).bit_length() - 1) if n > 3 else 1
+    tabs = [O.uniform(n, 400 + 10 * k + n + i) for i in range(k)]
+    F = B.Ext(ext)
+    rp, ch, fin = O.Prover.product(tabs, ext=ext).sumcheck(rounds, O.Transcript.shake(b"halve"))
+    brp, bch, bfin = B.prove_halving(F, [[int(x) for x in t] for t in tabs], rounds, B.ShakeTranscript(b"halve"))
+    assert [[list(x) for x in r] for r in brp] == rp
+    assert [list(c) for c in bch] == ch
+    assert list(bfin) == fin
+
+
+@pytest.mark.parametrize("ext", [1, 3])
+@pytest.mark.parametrize("n", [1, 3, 5, 6, 7, 11, 12, 100])
+def test_lsb_multilinear_any_length_matches_definition(n, ext):
+    # src/provers/multilinear_lsb.rs: s0 counts an unpaired last element, the
+    # fold drops it
+    rounds = max(1, (n - 1).bit_length())
+    t = O.uniform(n, 500 + n)
+    F = B.Ext(ext)
+    o = O.Prover.multilinear_lsb(t if ext == 1 else _lift(t, ext), ext=ext)
+    rp, ch, fin = o.sumcheck(rounds, O.Transcript.shake(b"lsb-odd"))
+    brp, bch, bfin, bT = B.prove_ml_pairs(F, [int(x) for x in t], rounds, B.ShakeTranscript(b"lsb-odd"))
+    assert [[list(x) for x in r] for r in brp] == rp
+    assert [list(c) for c in bch] == ch
+    assert list(bfin) == fin
+    assert _pylist(o.table(0), ext) == [x[0] if ext == 1 else x for x in bT]
+
+
+@pytest.mark.parametrize("ext", [2, 3])
+@pytest.mark.parametrize("v", [1, 3, 4])
+def test_product_lsb_extension_inputs_match_brute_force(v, ext):
+    n = 1 << v
+    tabs = [O.uniform(n * ext, 800 + v + i) for i in range(3)]
+    o = O.Prover.product_lsb(tabs, ext=ext, tables_in_ext=True)
+    rp, ch, fin = o.sumcheck(v, O.Transcript.shake(b"lsb-ext"))
+    F = B.Ext(ext)
+    el = [[tuple(int(x) for x in t[i * ext:(i + 1) * ext]) for i in range(n)] for t in tabs]
+    brp, bch, bfin = B.prove_lsb(F, el, v, B.ShakeTranscript(b"lsb-ext"), 3, lift=False)
+    assert [[list(x) for x in r] for r in brp] == rp
+    assert [list(c) for c in bch] == ch
+    assert list(bfin) == fin
