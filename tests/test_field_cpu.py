@@ -1,0 +1,23 @@
+"""The device field arithmetic (csrc/field.hpp) compiled for the host and
+checked against independent big-integer definitions: Goldilocks product,
+128-bit reduction, add/sub, the F_p^2 and F_p^3 prepared-multiplier products,
+the F_p^3 fold form and Karatsuba's F_p^3 form, on every pair of edge
+operands and 2 * 10^6 random ones (tests/cpp/field_check.cpp)."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "field_check")
+
+
+def test_device_field_arithmetic_on_the_cpu():
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/field_check is not built (make -C tests/cpp)")
+    r = subprocess.run([BIN, "2000000"], capture_output=True, text=True, timeout=300)
+    out = {d["name"]: d for d in map(json.loads, r.stdout.splitlines())}
+    assert r.returncode == 0, r.stdout
+    assert out["base"]["ok"] and out["base"]["cases"] >= 2000000
+    assert out["fp2"]["ok"] and out["fp3"]["ok"]
