@@ -109,7 +109,7 @@ int ensure_dev_bufs(effsc_hip_prover* h) {
   h->d_chals = blk + kMaxRounds * kMaxK * 3;
   h->d_fin = h->d_chals + kMaxRounds * 3;
   h->d_fs = reinterpret_cast<DevFs*>(h->d_fin + 4);
-  TRY(alloc_dev(&h->d_cur, 4));
+  TRY(alloc_dev(&h->d_cur, 8));  // two pending challenges of up to 3 words
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), kProofWords * 8, hipHostMallocDefault));
   return EFFSC_HIP_OK;
 }
@@ -147,8 +147,10 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   size_t r = 0;
   int p = 0;
   bool tail_used = false;
-  if (h->twostep) {
-    // two rounds per pass while the tables are large
+  const bool ext2 = ext_twostep(h);
+  if (h->twostep || ext2) {
+    // two rounds per pass while the tables are large (base field: pass2.hpp;
+    // extension fields: pass_ext.hip)
     while (num_rounds - r >= 2) {
       size_t L = h->len >> p;
       if (L <= tail_max(h) || L < 4) break;
@@ -163,6 +165,9 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
         if (p != 2) return fail(EFFSC_HIP_ESTATE, "out-of-core pass order");
         TRY(ooc_fold_pass(h, L / 4, &g));
         red = Reduce{h->ooc_parts, h->red.result};
+      } else if (ext2) {
+        TRY(launch_pass_ext(h, p, L / 4, p ? h->d_cur : nullptr));
+        g = h->last_grid;
       } else {
         TRY(launch_pass(h, p, L / 4, p ? h->d_cur : nullptr));
         g = h->last_grid;
@@ -171,7 +176,9 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
         h->len = L;
         h->in_work = true;
       }
-      if (K == 1)
+      if (ext2)
+        TRY(launch_reduce_fs2_ext(h, red, g, r));
+      else if (K == 1)
         hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs,
                            h->d_polys + r, h->d_chals + r, h->d_cur);
       else
@@ -210,11 +217,12 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
     size_t M = h->len;
     uint64_t* const* src = src_planes(h);
     size_t sst = src_stride(h);
-    TRY(dispatch(1, 1, K, [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, M / 2, M / 2, h->d_cur); }));
+    TRY(dispatch(src_field(h), E, K,
+                 [&](auto op) { return decltype(op)::fold(h, src, sst, nullptr, M / 2, M / 2, h->d_cur); }));
     h->len = M / 2;
     h->in_work = true;
     p = 1;
-    cur = h->d_cur + 1;
+    cur = h->d_cur + E;
   }
   size_t rounds_left = num_rounds - r;
   size_t M = h->len;

@@ -231,8 +231,10 @@ __device__ __forceinline__ Fp3 readlane(Fp3 a, int l) {
 // i, and broadcasts with readlane.
 template <class F, int K>
 __device__ __forceinline__ void block_reduce(F* acc) {
-  __shared__ uint64_t lds[kMaxWaves][kMaxK * 3];
-  static_assert(K * F::D <= kMaxK * 3, "block_reduce: too many words");
+  // up to 24 words per wave: the pass values of the two-round passes over
+  // F_p^3 (8 values of 3 words)
+  __shared__ uint64_t lds[kMaxWaves][24];
+  static_assert(K * F::D <= 24, "block_reduce: too many words");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if constexpr (K == 1) {
 #pragma unroll
@@ -298,7 +300,8 @@ template <class F, int K>
 __device__ __forceinline__ void sum_partials(const uint64_t* partials, unsigned nblocks, F* s) {
   // two blocks per step for F_p^3 sums of three or more values: four spilled
   // registers in the 1024-thread reduction kernels
-  constexpr int U = (F::D == 3 && K >= 3) ? 2 : 4;
+  // (one block per step when the values span more than 12 words)
+  constexpr int U = (F::D * K > 12) ? 1 : (F::D == 3 && K >= 3) ? 2 : 4;
   const unsigned st = blockDim.x;
   unsigned b = threadIdx.x;
   for (; b + (U - 1) * st < nblocks; b += U * st) {

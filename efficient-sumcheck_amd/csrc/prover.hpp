@@ -328,6 +328,11 @@ struct DevModeGuard {
   ~DevModeGuard() { h->dev_mode = false; }
 };
 
+// two-round passes over extension fields in the device loop (pass_ext.hip)
+bool ext_twostep(const effsc_hip_prover* h);
+int launch_pass_ext(effsc_hip_prover* h, int p, size_t q, const uint64_t* cptr);
+int launch_reduce_fs2_ext(effsc_hip_prover* h, const Reduce& red, unsigned g, size_t round);
+
 // EqFactoredProver (eq.hip)
 int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);
 int eq_finalize(effsc_hip_prover* h, const uint64_t* chal);

@@ -110,3 +110,37 @@ def test_fp2_large_parity(kind, input_ext):
     assert tuple(as_oracle(pg, 2)) == tuple(po)
     for i in range(k):
         assert np.array_equal(g.table_words(i), o.table(i))
+
+
+@pytest.mark.parametrize("env", ["EFFSC_HIP_ONE_ROUND", "EFFSC_HIP_EXT3_PASS"])
+@pytest.mark.parametrize("kind,ext,input_ext", [("ml", 2, 2), ("ip", 2, 1), ("ml", 3, 1), ("ip", 3, 3)])
+@pytest.mark.parametrize("v", [2, 5, 9, 14, 17])
+def test_extension_device_loop_forms(kind, ext, input_ext, v, env, monkeypatch):
+    # the extension two-round passes (default over F_p^2, EFFSC_HIP_EXT3_PASS
+    # over F_p^3) and one round per launch give the same proofs
+    monkeypatch.setenv(env, "1")
+    monkeypatch.setenv("EFFSC_HIP_TAIL", "64")
+    n = 1 << v
+    k = 2 if kind == "ip" else 1
+    tabs = [O.uniform(n * input_ext, 0xE70 + v + i) for i in range(k)]
+    say(f"ext device loop {env} kind={kind} ext={ext}/{input_ext} v={v}")
+    lifted = [t if input_ext == ext else lift(t, ext) for t in tabs]
+    if kind == "ip":
+        g = InnerProductProver(tabs[0], tabs[1], ext=ext, input_ext=input_ext)
+        o = O.Prover.inner_product(lifted[0], lifted[1], ext=ext)
+    else:
+        g = MultilinearProver(tabs[0], ext=ext, input_ext=input_ext)
+        o = O.Prover.multilinear(lifted[0], ext=ext)
+    pg = sumcheck(g, v, Transcript(b"ext-forms", ext=ext))
+    po = o.sumcheck(v, O.Transcript.shake(b"ext-forms"))
+    assert tuple(as_oracle(pg, ext)) == tuple(po)
+    for i in range(k):
+        assert np.array_equal(g.table_words(i), o.table(i))
+    # a partial device run, then the rest
+    g2 = InnerProductProver(tabs[0], tabs[1], ext=ext, input_ext=input_ext) if kind == "ip" else \
+        MultilinearProver(tabs[0], ext=ext, input_ext=input_ext)
+    o2 = O.Prover.inner_product(lifted[0], lifted[1], ext=ext) if kind == "ip" else O.Prover.multilinear(lifted[0], ext=ext)
+    r1 = max(1, v // 2)
+    t1, u1 = Transcript(b"ext-part", ext=ext), O.Transcript.shake(b"ext-part")
+    assert tuple(as_oracle(sumcheck(g2, r1, t1), ext)) == tuple(o2.sumcheck(r1, u1))
+    assert tuple(as_oracle(sumcheck(g2, v - r1, t1), ext)) == tuple(o2.sumcheck(v - r1, u1))
