@@ -36,6 +36,53 @@ __device__ __forceinline__ F chain(const View<FI>& t, size_t e, size_t L, const 
   }
 }
 
+// Two consecutive level-L elements e, e + 1 (e even, 16-byte aligned planes).
+template <int P, class FI, class F>
+__device__ __forceinline__ void chain2(const View<FI>& t, size_t e, size_t L, const F& c1, const F& c2, uint64_t w,
+                                       F& o0, F& o1) {
+  if constexpr (P == 0) {
+    FI x0, x1;
+    t.ld2(e, x0, x1);
+    o0 = gl::lift_to<F>(x0);
+    o1 = gl::lift_to<F>(x1);
+  } else {
+    FI x0[2], x1[2], x2[2], x3[2];
+    t.ld2(e, x0[0], x0[1]);
+    t.ld2(e + L, x1[0], x1[1]);
+    t.ld2(e + 2 * L, x2[0], x2[1]);
+    t.ld2(e + 3 * L, x3[0], x3[1]);
+    o0 = fold1<F>(fold1<F>(x0[0], x2[0], c1, w), fold1<F>(x1[0], x3[0], c1, w), c2, w);
+    o1 = fold1<F>(fold1<F>(x0[1], x2[1], c1, w), fold1<F>(x1[1], x3[1], c1, w), c2, w);
+  }
+}
+
+// The multilinear pass with two units per thread step and 16-byte loads and
+// stores (q even, strides even).
+template <class FI, class F, int P>
+__global__ __launch_bounds__(kBlock) void k_pass_ext_ml2(View<FI> src, View<F> dst, const uint64_t* cptr, size_t q,
+                                                         uint64_t w, Reduce r) {
+  F c1 = gl::zero<F>(), c2 = gl::zero<F>();
+  if constexpr (P > 0) {
+    c1 = gl::load_words<F>(cptr);
+    c2 = gl::load_words<F>(cptr + F::D);
+  }
+  const size_t L = 4 * q;
+  F s[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
+  for (size_t u = gtid() * 2; u < q; u += (size_t)gridDim.x * blockDim.x * 2) {
+    F E0[4], E1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) chain2<P>(src, u + j * q, L, c1, c2, w, E0[j], E1[j]);
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) dst.st2(u + j * q, E0[j], E1[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) s[j] = gl::add(s[j], gl::add(E0[j], E1[j]));
+  }
+  F v[3] = {gl::add(s[0], s[1]), s[0], gl::sub(s[2], s[0])};
+  finish<F, 3>(v, r);
+}
+
 // Raw sums -> pass values (pass2.hpp layout: K values of round s, then
 // value i's coefficient j at K + i (K + 1) + j).
 template <class F, int K>
@@ -192,9 +239,14 @@ template <class FI, class F, int K, int P>
 int launch_pass_ext_t(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   const Views<FI, K> in = views<FI, K>(src_planes(h), src_stride(h));
   const Views<F, K> out = views<F, K>(h->work, h->work_stride);
-  const unsigned g = blocks_for(h, q);
+  const bool vec = K == 1 && q % 2 == 0 && src_stride(h) % 2 == 0 && h->work_stride % 2 == 0;
+  const unsigned g = blocks_for(h, vec ? q / 2 : q);
   TRY(prof_begin(h));
-  hipLaunchKernelGGL((k_pass_ext<FI, F, K, P>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q, h->w, h->red);
+  if (vec)
+    hipLaunchKernelGGL((k_pass_ext_ml2<FI, F, P>), dim3(g), dim3(kBlock), 0, h->stream, in.t[0], out.t[0], cptr, q,
+                       h->w, h->red);
+  else
+    hipLaunchKernelGGL((k_pass_ext<FI, F, K, P>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q, h->w, h->red);
   HIPCHK(hipGetLastError());
   const double L = 4.0 * (double)q;
   // read the stored level (2^P L elements), write level L when P > 0
@@ -207,13 +259,14 @@ int launch_pass_ext_t(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
 
 }  // namespace
 
-// F_p^2 only: over F_p^3 these passes measured slower than one round per
-// launch (v = 24: inner product 1.354 against 1.131 ms, multilinear 0.666
-// against 0.539 ms; the F_p^3 passes run at two waves per SIMD with 8-byte
-// loads), over F_p^2 faster (v = 26: 2.116 against 2.409 ms, multilinear 0.876
-// against 1.064 ms).  EFFSC_HIP_EXT3_PASS=1 selects them over F_p^3 as well.
+// Used over F_p^2, and for the multilinear prover over F_p^3 (measured with
+// the SHAKE transcript, v = 26 over F_p^2: inner product 2.41 -> 2.12 ms,
+// multilinear 1.06 -> 0.88 ms; v = 24 over F_p^3: multilinear 0.669 -> 0.644
+// ms).  The F_p^3 inner-product pass (8 F_p^3 products per unit at two waves
+// per SIMD) measured 1.349 against 1.307 ms for one round per launch, so it
+// runs only with EFFSC_HIP_EXT3_PASS=1.
 bool ext_twostep(const effsc_hip_prover* h) {
-  const bool field_ok = h->ext == 2 || (h->ext == 3 && std::getenv("EFFSC_HIP_EXT3_PASS"));
+  const bool field_ok = h->ext == 2 || (h->ext == 3 && (h->K == 1 || std::getenv("EFFSC_HIP_EXT3_PASS")));
   return (h->kind == K_ML || h->kind == K_IP) && field_ok && h->K <= 2 && !h->lsb_native && !h->ooc &&
          !std::getenv("EFFSC_HIP_ONE_ROUND");
 }
