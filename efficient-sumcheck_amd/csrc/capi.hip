@@ -469,6 +469,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->eq_full) (void)hipFree(h->eq_full);
+  if (h->eq_rp) (void)hipFree(h->eq_rp);
   if (h->d_pass0) (void)hipFree(h->d_pass0);
   if (h->ooc_parts) (void)hipFree(h->ooc_parts);
   if (h->gkr_b) (void)hipFree(h->gkr_b);

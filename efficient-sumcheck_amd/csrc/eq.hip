@@ -202,11 +202,9 @@ __global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs(Reduce r, unsigne
                                                                uint64_t* chal_cur, View<F> ft, size_t fold_half,
                                                                uint64_t w) {
   __shared__ uint64_t chal_sh[3];
-  F s[2] = {gl::zero<F>(), gl::zero<F>()};
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < 2; i++) s[i] = gl::add(s[i], gl::load_words<F>(r.partials + ((size_t)b * 2 + i) * F::D));
-  block_reduce<F, 2>(s);
+  __shared__ uint64_t sh[kReduceBlock];
+  sum_partials_words<2 * F::D>(r.partials, nblocks, sh);
+  F s[2] = {gl::load_words<F>(sh), gl::load_words<F>(sh + F::D)};
   if (right) {
     const F sc = eql.ld(0);
     s[0] = gl::mul(s[0], sc, w);
@@ -339,15 +337,17 @@ __global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs2(Reduce r, unsign
                                                                 uint64_t* chals, uint64_t* cur, uint64_t* t0,
                                                                 size_t h0, uint64_t* t1, size_t h1) {
   constexpr int NV = pass_values<2>();
+  __shared__ uint64_t sh[kReduceBlock];
   __shared__ uint64_t xy[3];
   LaneFs fs;
   if (threadIdx.x < 64) fs.load(t);
-  Fp sv[NV];
+  sum_partials_words<NV>(r.partials, nblocks, sh);
+  if (threadIdx.x < 64) {
+    Fp sv[NV];
 #pragma unroll
-  for (int i = 0; i < NV; i++) sv[i] = gl::zero<Fp>();
-  sum_partials<Fp, NV>(r.partials, nblocks, sv);
-  block_reduce<Fp, NV>(sv);
-  if (threadIdx.x < 64) emit_two_rounds<2>(sv, fs, t, poly, chals, xy);
+    for (int i = 0; i < NV; i++) sv[i] = Fp{sh[i]};
+    emit_two_rounds<2>(sv, fs, t, poly, chals, xy);
+  }
   __syncthreads();
   const uint64_t x = xy[0], y = xy[1];
   if (threadIdx.x < 2) cur[threadIdx.x] = xy[threadIdx.x];
@@ -668,7 +668,8 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
   // rounds on the device until p is small enough for one workgroup; eq
   // tables folded inside the reduction when small, else by k_fold
   const size_t tail = h->ext == 3 ? ((size_t)1 << 10) : h->ext == 2 ? ((size_t)1 << 11) : ((size_t)1 << 12);
-  const size_t fold_in_reduce = (size_t)1 << 14;
+  size_t fold_in_reduce = (size_t)1 << 14;
+  if (const char* e = std::getenv("EFFSC_HIP_EQ_FOLD_LIMIT")) fold_in_reduce = std::strtoull(e, nullptr, 10);
   size_t L = h->len, j = 0;
   bool have = false;
   View<F> eql{h->eq_l, h->eq_l_stride}, eqr{h->eq_r, h->eq_r_stride};
@@ -793,6 +794,70 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         h->in_work = true;
         have = true;
         cur_in = h->d_cur + 1;
+      }
+    }
+  }
+  if constexpr (F::D >= 2 && std::is_same<FI, F>::value) {
+    if (!std::getenv("EFFSC_HIP_ONE_ROUND") && !std::getenv("EFFSC_HIP_EQ_DIRECT")) {
+      // two rounds per pass while both variables of a pass are left ones
+      // (split-value form, q a multiple of 2^m) and the level evaluated is
+      // above the tail
+      int pend = 0;
+      bool prepped = false;
+      while (j + 2 <= v) {
+        const size_t Lv = L >> pend;
+        if (Lv <= tail || Lv < 8) break;
+        size_t m_log = 0;
+        while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
+        const size_t q = Lv / 4;
+        // a wave per quarter, 128 units per wave step; at most grid_max
+        // workgroups (8 extension values each in the partials buffer)
+        const size_t gmax = (size_t)h->grid_max;
+        size_t chunk = 128;
+        while (q / chunk > gmax) chunk *= 2;
+        if (q % ((size_t)1 << m_log) != 0 || chunk > ((size_t)1 << m_log) || q % chunk != 0) break;
+        if (!prepped) {
+          TRY(eq_prep_r_ext(h));  // eq_r is fixed while left variables are bound
+          prepped = true;
+        }
+        const unsigned g = (unsigned)(q / chunk);
+        TRY(eq_pass_ext(h, pend, q, chunk, (unsigned)m_log, g));
+        if (pend) {
+          L = Lv;
+          h->len = Lv;
+          h->in_work = true;
+        }
+        const size_t h0 = h->eq_l_len >> 1, h1 = h0 >> 1;
+        const bool in_reduce = h0 <= fold_in_reduce;
+        TRY(eq_reduce_fs2_ext(h, g, j, in_reduce ? h0 : 0, in_reduce ? h1 : 0));
+        if (!in_reduce) {
+          Views<F, 1> t1{};
+          t1.t[0] = eql;
+          hipLaunchKernelGGL((k_fold<F, F, 1>), dim3(blocks_for(h, h0)), dim3(kBlock), 0, h->stream, t1, t1,
+                             gl::zero<F>(), h->d_cur, h0, h0, h->w);
+          HIPCHK(hipGetLastError());
+          hipLaunchKernelGGL((k_fold<F, F, 1>), dim3(blocks_for(h, h1)), dim3(kBlock), 0, h->stream, t1, t1,
+                             gl::zero<F>(), h->d_cur + E, h1, h1, h->w);
+          HIPCHK(hipGetLastError());
+        }
+        h->eq_l_len = h1;
+        j += 2;
+        pend = 2;
+      }
+      if (pend == 2) {
+        // the older pending challenge folds p with a multi-block fold; the
+        // newer one stays pending for the rounds below
+        Views<F, 1> a{}, b{};
+        a.t[0] = View<F>{src_planes(h)[0], src_stride(h)};
+        b.t[0] = View<F>{h->work[0], h->work_stride};
+        hipLaunchKernelGGL((k_fold<F, F, 1>), dim3(blocks_for(h, L / 2)), dim3(kBlock), 0, h->stream, a, b,
+                           gl::zero<F>(), h->d_cur, L / 2, L / 2, h->w);
+        HIPCHK(hipGetLastError());
+        L /= 2;
+        h->len = L;
+        h->in_work = true;
+        have = true;
+        cur_in = h->d_cur + E;
       }
     }
   }

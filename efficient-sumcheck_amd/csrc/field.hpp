@@ -188,6 +188,151 @@ GLHD Fp3 mul(Fp3 a, const Fp3Pre& b) {
 // challenge) is prepared once, outside the loop.
 GLHD Fp3 mul(Fp3 a, Fp3 b, uint64_t w) { return mul(b, prepare(a, w)); }
 
+// a + b as a word congruent to a + b (mod p), not necessarily below p (a, b
+// in [0, p)): a product operand only.  A wrapped sum a + b - 2^64 + eps stays
+// below 2^64 - eps.
+GLHD uint64_t add_lazy(uint64_t a, uint64_t b) {
+  const uint64_t s = a + b;
+  return s < a ? s + EPS : s;
+}
+
+// A sum of 64 x 64-bit products kept in 32-bit columns with carry counts:
+//   value = s0 + 2^32 s1 + 2^64 (s2 + k0) + 2^96 k1 + 2^128 k2,
+// s0 += x0 y0, s1 += x0 y1 + x1 y0, s2 += x1 y1 (xi, yi the 32-bit halves)
+// and each k counts the carries out of its column.  On gfx950 one product is
+// four v_mad_u64_u32 with the column as addend and carry out, and four
+// v_addc_co_u32 into the counts; at most 2^31 products per sum.
+struct AccW {
+  uint64_t s0, s1, s2;
+  uint32_t k0, k1, k2;
+};
+GLHD void zero(AccW& a) { a = {0, 0, 0, 0, 0, 0}; }
+GLHD void macw(AccW& a, uint64_t x, uint64_t y) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // every carry is read three instructions after the mad that writes it
+  uint64_t c0, c1, c2, c3;
+  asm("v_mad_u64_u32 %0, %6, %10, %12, %0\n\t"
+      "v_mad_u64_u32 %1, %7, %10, %13, %1\n\t"
+      "v_mad_u64_u32 %1, %8, %11, %12, %1\n\t"
+      "v_mad_u64_u32 %2, %9, %11, %13, %2\n\t"
+      "v_addc_co_u32_e64 %3, %6, %3, 0, %6\n\t"
+      "v_addc_co_u32_e64 %4, %7, %4, 0, %7\n\t"
+      "v_addc_co_u32_e64 %4, %8, %4, 0, %8\n\t"
+      "v_addc_co_u32_e64 %5, %9, %5, 0, %9"
+      : "+v"(a.s0), "+v"(a.s1), "+v"(a.s2), "+v"(a.k0), "+v"(a.k1), "+v"(a.k2), "=&s"(c0), "=&s"(c1), "=&s"(c2),
+        "=&s"(c3)
+      : "v"(x0), "v"(x1), "v"(y0), "v"(y1));
+#else
+  auto col = [](uint64_t& s, uint32_t& k, uint64_t p) {
+    s += p;
+    k += s < p;
+  };
+  col(a.s0, a.k0, (uint64_t)x0 * y0);
+  col(a.s1, a.k1, (uint64_t)x0 * y1);
+  col(a.s1, a.k1, (uint64_t)x1 * y0);
+  col(a.s2, a.k2, (uint64_t)x1 * y1);
+#endif
+}
+// 2^128 = -2^32 (mod p)
+GLHD uint64_t reduce(const AccW& a) {
+  typedef unsigned __int128 u128;
+  const uint64_t sl = a.s1 << 32, lo = a.s0 + sl;
+  const u128 mid = (u128)a.s2 + a.k0 + (a.s1 >> 32) + (uint64_t)(lo < sl) + ((u128)a.k1 << 32);  // < 2^66
+  return sub(reduce128(lo, (uint64_t)mid), ((uint64_t)a.k2 + (uint64_t)(mid >> 64)) << 32);
+}
+
+// a b accumulated coefficient-wise into r (b prepared: mod X^d - w).
+GLHD void mac_pre(AccW* r, Fp2 a, const Fp2Pre& b) {
+  macw(r[0], a.c0, b.b0);
+  macw(r[0], a.c1, b.wb1);
+  macw(r[1], a.c0, b.b1);
+  macw(r[1], a.c1, b.b0);
+}
+GLHD void mac_pre(AccW* r, Fp3 a, const Fp3Pre& b) {
+  macw(r[0], a.c0, b.b0);
+  macw(r[0], a.c1, b.wb2);
+  macw(r[0], a.c2, b.wb1);
+  macw(r[1], a.c0, b.b1);
+  macw(r[1], a.c1, b.b0);
+  macw(r[1], a.c2, b.wb2);
+  macw(r[2], a.c0, b.b2);
+  macw(r[2], a.c1, b.b1);
+  macw(r[2], a.c2, b.b0);
+}
+
+// Sums of products a r (a, r in F_p^2 or F_p^3) by Karatsuba without any
+// reduction in the sum: each partial-product class accumulates in its own
+// column sum (AccW) and the classes are combined once, in value().  r comes
+// prepared (kdot_prep): its coefficients and their pairwise sums.
+//   F_p^2: m0 = a0 r0, m1 = a1 r1, m2 = (a0 + a1)(r0 + r1);
+//          a r = (m0 + w m1) + (m2 - m0 - m1) X.
+//   F_p^3: m0..m2 = ai ri, m3 = (a0 + a1)(r0 + r1), m4 = (a0 + a2)(r0 + r2),
+//          m5 = (a1 + a2)(r1 + r2); with s1 = m3 - m0 - m1,
+//          s2 = m4 - m0 - m2 + m1, s3 = m5 - m1 - m2:
+//          a r = (m0 + w s3) + (s1 + w m2) X + s2 X^2.
+// 3 (F_p^2) or 6 (F_p^3) products per term against 4 or 9 in the schoolbook
+// form.
+template <class F>
+struct KDot;
+template <>
+struct KDot<Fp2> {
+  static constexpr int R = 3;  // prepared words per multiplier
+  AccW m[3];
+  GLHD void zero() {
+#pragma unroll
+    for (int i = 0; i < 3; i++) gl::zero(m[i]);
+  }
+  GLHD void add_term(Fp2 a, const uint64_t* r) {
+    macw(m[0], a.c0, r[0]);
+    macw(m[1], a.c1, r[1]);
+    macw(m[2], add_lazy(a.c0, a.c1), r[2]);
+  }
+  GLHD Fp2 value(uint64_t w) const {
+    const uint64_t m0 = gl::reduce(m[0]), m1 = gl::reduce(m[1]), m2 = gl::reduce(m[2]);
+    return {add(m0, mul(w, m1)), sub(sub(m2, m0), m1)};
+  }
+};
+template <>
+struct KDot<Fp3> {
+  static constexpr int R = 6;
+  AccW m[6];
+  GLHD void zero() {
+#pragma unroll
+    for (int i = 0; i < 6; i++) gl::zero(m[i]);
+  }
+  GLHD void add_term(Fp3 a, const uint64_t* r) {
+    macw(m[0], a.c0, r[0]);
+    macw(m[1], a.c1, r[1]);
+    macw(m[2], a.c2, r[2]);
+    macw(m[3], add_lazy(a.c0, a.c1), r[3]);
+    macw(m[4], add_lazy(a.c0, a.c2), r[4]);
+    macw(m[5], add_lazy(a.c1, a.c2), r[5]);
+  }
+  GLHD Fp3 value(uint64_t w) const {
+    uint64_t v[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = gl::reduce(m[i]);
+    const uint64_t s1 = sub(sub(v[3], v[0]), v[1]);
+    const uint64_t s2 = add(sub(sub(v[4], v[0]), v[2]), v[1]);
+    const uint64_t s3 = sub(sub(v[5], v[1]), v[2]);
+    return {add(v[0], mul(w, s3)), add(s1, mul(w, v[2])), s2};
+  }
+};
+GLHD void kdot_prep(Fp2 r, uint64_t* o) {
+  o[0] = r.c0;
+  o[1] = r.c1;
+  o[2] = add(r.c0, r.c1);
+}
+GLHD void kdot_prep(Fp3 r, uint64_t* o) {
+  o[0] = r.c0;
+  o[1] = r.c1;
+  o[2] = r.c2;
+  o[3] = add(r.c0, r.c1);
+  o[4] = add(r.c0, r.c2);
+  o[5] = add(r.c1, r.c2);
+}
+
 // Mixed products: extension element times base element.
 GLHD Fp2 mul(Fp2 a, Fp b, uint64_t) { return {mul(a.c0, b.v), mul(a.c1, b.v)}; }
 GLHD Fp2 mul(Fp a, Fp2 b, uint64_t) { return {mul(a.v, b.c0), mul(a.v, b.c1)}; }
@@ -276,5 +421,42 @@ template <>
 GLHD Fp2 load_words<Fp2>(const uint64_t* w) { return {canon(w[0]), canon(w[1])}; }
 template <>
 GLHD Fp3 load_words<Fp3>(const uint64_t* w) { return {canon(w[0]), canon(w[1]), canon(w[2])}; }
+
+// Two pending folds of four elements x0..x3 (x_k at offset k L of the level
+// 4L table) with challenges c1 (the older) and c2:
+//   fold(fold(x0, x2, c1), fold(x1, x3, c1), c2)
+//     = x0 (1 - c1 - c2 + c1 c2) + x1 (c2 - c1 c2) + x2 (c1 - c1 c2) + x3 c1 c2,
+// four products by prepared constants summed in one column sum per
+// coefficient and reduced once: no subtraction per element.
+template <class F>
+struct Fold2Pre {
+  decltype(prepare(F{}, uint64_t{})) e[4];
+};
+template <class F>
+GLHD Fold2Pre<F> prepare_fold2(F c1, F c2, uint64_t w) {
+  const F c12 = mul(c1, c2, w);
+  Fold2Pre<F> p;
+  p.e[0] = prepare(add(sub(sub(one<F>(), c1), c2), c12), w);
+  p.e[1] = prepare(sub(c2, c12), w);
+  p.e[2] = prepare(sub(c1, c12), w);
+  p.e[3] = prepare(c12, w);
+  return p;
+}
+template <class F>
+GLHD F fold2(F x0, F x1, F x2, F x3, const Fold2Pre<F>& p, uint64_t) {
+  AccW r[F::D];
+#pragma unroll
+  for (int i = 0; i < F::D; i++) zero(r[i]);
+  mac_pre(r, x0, p.e[0]);
+  mac_pre(r, x1, p.e[1]);
+  mac_pre(r, x2, p.e[2]);
+  mac_pre(r, x3, p.e[3]);
+  F o;
+  uint64_t wd[3];
+#pragma unroll
+  for (int i = 0; i < F::D; i++) wd[i] = reduce(r[i]);
+  o = load_words<F>(wd);
+  return o;
+}
 
 }  // namespace gl

@@ -182,25 +182,15 @@ __device__ __forceinline__ F uniform_elem(const F& a) {
   return gl::load_words<F>(wd);
 }
 
-// Sums the pass values of `nblocks` workgroups and runs the two transcript
-// steps by wave 0: round s values, challenge x, round s + 1 values (the pass
-// polynomials at x), challenge y.  poly receives 2 K elements, chals (x, y),
-// cur (x, y): the pending folds of the next pass.
+// The two transcript steps of a pass from its summed values s (wave 0,
+// uniform values): round s values, challenge x; then round s + 1 values (the
+// pass polynomials at x), challenge y.  poly receives 2 K elements, chals
+// (x, y), cur (x, y): the pending folds of the next pass; xy (shared memory)
+// x after the first step and y after the second.
 template <class F, int K>
-__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2_ext(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
-                                                                 uint64_t* chals, uint64_t* cur, uint64_t w) {
-  constexpr int NV = pass_values<K>(), D = F::D;
-  LaneFs fs;
-  if (threadIdx.x < 64) fs.load(t);
-  F s[NV];
-#pragma unroll
-  for (int i = 0; i < NV; i++) s[i] = gl::zero<F>();
-  sum_partials<F, NV>(r.partials, nblocks, s);
-  block_reduce<F, NV>(s);
-  if (threadIdx.x >= 64) return;
+__device__ __forceinline__ F emit_first_ext(const F* s, LaneFs& fs, uint64_t* poly, uint64_t* xy) {
+  constexpr int D = F::D;
   const bool lane0 = (threadIdx.x & 63) == 0;
-#pragma unroll
-  for (int i = 0; i < NV; i++) s[i] = uniform_elem(s[i]);
   uint64_t wd[3];
 #pragma unroll
   for (int i = 0; i < K; i++) {
@@ -212,16 +202,39 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2_ext(Reduce r, unsig
   }
   uint64_t x[3];
   fs.challenge(D, x);
-  const F X = gl::load_words<F>(x);
+  if (lane0)
+#pragma unroll
+    for (int c = 0; c < D; c++) xy[c] = x[c];
+  return gl::load_words<F>(x);
+}
+template <class F, int K>
+__device__ __forceinline__ void emit_second_ext(const F* s, const F& X, LaneFs& fs, DevFs* t, uint64_t* poly,
+                                                uint64_t* chals, uint64_t* cur, uint64_t w, uint64_t* xy) {
+  constexpr int D = F::D;
+  const int lane = threadIdx.x & 63;
+  const bool lane0 = lane == 0;
+  // lane i < K evaluates value i; the others repeat value 0
+  F cf[K + 1];
+#pragma unroll
+  for (int j = 0; j <= K; j++) {
+    cf[j] = s[K + j];
+#pragma unroll
+    for (int i = 1; i < K; i++) cf[j] = select(lane == i, s[K + i * (K + 1) + j], cf[j]);
+  }
+  F mine = cf[K];
+#pragma unroll
+  for (int j = K - 1; j >= 0; j--) mine = gl::add(gl::mul(mine, X, w), cf[j]);
+  uint64_t wd[3];
 #pragma unroll
   for (int i = 0; i < K; i++) {
-    gl::store_words(eval_next<F, K>(s, i, X, w), wd);
+    gl::store_words(readlane(mine, i), wd);
     if (lane0)
 #pragma unroll
       for (int c = 0; c < D; c++) poly[(K + i) * D + c] = wd[c];
     fs.send(D, wd);
   }
-  uint64_t y[3];
+  uint64_t y[3], x[3];
+  gl::store_words(X, x);
   fs.challenge(D, y);
   if (lane0) {
 #pragma unroll
@@ -230,9 +243,230 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2_ext(Reduce r, unsig
       chals[D + c] = y[c];
       cur[c] = x[c];
       cur[D + c] = y[c];
+      if (xy) xy[D + c] = y[c];
     }
   }
   fs.store(t);
+}
+
+// The summed pass values (sum_partials_words) in wave 0.
+template <class F, int NV>
+__device__ __forceinline__ void load_values(const uint64_t* sh, F* s) {
+#pragma unroll
+  for (int i = 0; i < NV; i++) s[i] = gl::load_words<F>(sh + i * F::D);
+}
+
+// Sums the pass values of `nblocks` workgroups and runs the two transcript
+// steps.
+template <class F, int K>
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2_ext(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
+                                                                 uint64_t* chals, uint64_t* cur, uint64_t w) {
+  constexpr int NV = pass_values<K>();
+  __shared__ uint64_t sh[kReduceBlock];
+  __shared__ uint64_t xy[6];
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
+  sum_partials_words<NV * F::D>(r.partials, nblocks, sh);
+  if (threadIdx.x >= 64) return;
+  F s[NV];
+  load_values<F, NV>(sh, s);
+  const F X = emit_first_ext<F, K>(s, fs, poly, xy);
+  emit_second_ext<F, K>(s, X, fs, t, poly, chals, cur, w, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// EqFactored over F_p^2 / F_p^3 in two-round passes (src/provers/eq_factored.rs:
+// 139-172, the left phase).  The proof is the inner product of p with
+// EQ(x) = eq_l[x >> m] eq_r[x & (2^m - 1)] (m = log2 |eq_r|); while both
+// variables of a pass are left variables, q = |level| / 4 is a multiple of
+// 2^m and unit u + j q (j < 4) has EQ = L_j R(u) with L_j = eq_l[(u >> m) +
+// j (q >> m)] and R(u) = eq_r[u & (2^m - 1)].  A workgroup owns a chunk of
+// units inside one block of 2^m, so its L_j are fixed, and the eight raw sums
+// of the inner-product pass (pass values of K = 2) are L-combinations times
+// T-combinations, T_j = sum over the chunk of A_j R:
+//   L0 T0, L1 T1, (L2 - L0)(T2 - T0), (L3 - L1)(T3 - T1), L2 T2,
+//   (L1 - L0)(T1 - T0), (L3 - L2)(T3 - T2), (L3 - L2 - L1 + L0)(T3 - T2 - T1 + T0).
+// Each T_j is a lazy Karatsuba sum (gl::KDot) against eq_r in its prepared
+// form (gl::kdot_prep), so an element costs 6 F_p^3 (3 F_p^2) word products
+// and no reduction; eq_r does not change while the left variables are bound.
+// p enters with P pending folds (the previous pass's challenges), applied on
+// the fly, and the folded level is written (in place is safe: element e is
+// written only by the thread that reads e, e + L, e + 2 L, e + 3 L).
+// ---------------------------------------------------------------------------
+
+// lo + c (hi - lo) with a prepared challenge
+template <class F, class Pre>
+__device__ __forceinline__ F fold_pre(const F& lo, const F& hi, const Pre& c) {
+  return gl::add(gl::mul(gl::sub(hi, lo), c), lo);
+}
+
+// The two pending folds' constants (gl::prepare_fold2) from the challenges
+// at cptr (c1 then c2), kept in scalar registers.
+template <class F>
+__device__ __forceinline__ gl::Fold2Pre<F> uniform_fold2(const uint64_t* cptr, uint64_t w) {
+  gl::Fold2Pre<F> p = gl::prepare_fold2(gl::load_words<F>(cptr), gl::load_words<F>(cptr + F::D), w);
+  constexpr int NW = sizeof(p.e[0]) / 8;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint64_t* wd = reinterpret_cast<uint64_t*>(&p.e[k]);
+#pragma unroll
+    for (int i = 0; i < NW; i++) wd[i] = uniform64(wd[i]);
+  }
+  return p;
+}
+
+// The level-4L elements behind level-L elements e, e + 1 (e even): x[k] at
+// e + k L (P = 2), or the level-L pair itself (P = 0).
+template <int P, class F>
+struct Quad2 {
+  F x[P > 0 ? 4 : 1][2];
+};
+template <int P, class F>
+__device__ __forceinline__ void quad_load(const View<F>& t, size_t e, size_t L, Quad2<P, F>& d) {
+#pragma unroll
+  for (int k = 0; k < (P > 0 ? 4 : 1); k++) t.ld2(e + k * L, d.x[k][0], d.x[k][1]);
+}
+// The two pending folds (gl::fold2) of a loaded quad.
+template <int P, class F>
+__device__ __forceinline__ void quad_fold(const Quad2<P, F>& d, const gl::Fold2Pre<F>& p, uint64_t w, F& o0, F& o1) {
+  if constexpr (P == 0) {
+    o0 = d.x[0][0];
+    o1 = d.x[0][1];
+  } else {
+    o0 = gl::fold2(d.x[0][0], d.x[1][0], d.x[2][0], d.x[3][0], p, w);
+    o1 = gl::fold2(d.x[0][1], d.x[1][1], d.x[2][1], d.x[3][1], p, w);
+  }
+}
+
+// eq_r -> its prepared form, KDot<F>::R planes of stride st.
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_eq_prep_r(View<F> eqr, size_t n, uint64_t* rp, size_t st) {
+  constexpr int R = gl::KDot<F>::R;
+  for (size_t b = gtid(); b < n; b += (size_t)gridDim.x * blockDim.x) {
+    uint64_t o[R];
+    gl::kdot_prep(eqr.ld(b), o);
+#pragma unroll
+    for (int i = 0; i < R; i++) rp[i * st + b] = o[i];
+  }
+}
+
+// One pass over units [blockIdx.x chunk, + chunk) of level L = 4 q: wave j
+// (kBlock = 4 waves) owns quarter j, two units per lane step (16-byte loads
+// and stores; q, chunk and the strides even, chunk a multiple of 128), so
+// each lane keeps one lazy sum.  The eight raw products are formed by lanes
+// 0..7 of wave 0, one each.
+template <class F, int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P == 0 ? 4 : 2, 8))) void k_eq_pass_ext(
+    View<F> src, View<F> dst, const uint64_t* cptr, size_t q, size_t chunk, View<F> eql, const uint64_t* rp,
+    size_t rst, unsigned m_log, uint64_t w, Reduce r) {
+  constexpr int R = gl::KDot<F>::R, D = F::D, NV = pass_values<2>();
+  static_assert(kBlock == 256, "one wave per quarter");
+  __shared__ uint64_t tsh[4][3], rsh[8][3];
+  gl::Fold2Pre<F> pf{};
+  if constexpr (P > 0) pf = uniform_fold2<F>(cptr, w);
+  const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
+  const size_t u0 = (size_t)blockIdx.x * chunk;
+  const unsigned j = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  gl::KDot<F> acc;
+  acc.zero();
+  // the next step's table loads are issued before this step's arithmetic
+  size_t u = u0 + 2 * lane;
+  const size_t uend = u0 + chunk;
+  Quad2<P, F> cur;
+  quad_load<P>(src, u + j * q, L, cur);
+  for (;;) {
+    const size_t un = u + 128;
+    const bool more = un < uend;  // the same for every lane: chunk is a multiple of 128
+    Quad2<P, F> nxt;
+    if (more) quad_load<P>(src, un + j * q, L, nxt);
+    F e0, e1;
+    quad_fold<P>(cur, pf, w, e0, e1);
+    if constexpr (P > 0) dst.st2(u + j * q, e0, e1);
+    const size_t b = u & mask;
+    uint64_t ra[R], rb[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(rp + i * rst + b);
+      ra[i] = v.x;
+      rb[i] = v.y;
+    }
+    acc.add_term(e0, ra);
+    acc.add_term(e1, rb);
+    if (!more) break;
+    cur = nxt;
+    u = un;
+  }
+  F t = acc.value(w);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) t = gl::add(t, shfl_xor(t, m));
+  if (lane == 0) gl::store_words(t, tsh[j]);
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int i = threadIdx.x;
+    const size_t b = u0 >> m_log, qs = q >> m_log;
+    F l[4], T[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      l[k] = eql.ld(b + k * qs);
+      T[k] = gl::load_words<F>(tsh[k]);
+    }
+    // raw product i: the same combination of the L_k and of the T_k
+    auto d = [](const F* v, int a, int b) { return gl::sub(v[a], v[b]); };
+    F x, y;
+    switch (i) {
+      case 0: x = l[0]; y = T[0]; break;
+      case 1: x = l[1]; y = T[1]; break;
+      case 2: x = d(l, 2, 0); y = d(T, 2, 0); break;
+      case 3: x = d(l, 3, 1); y = d(T, 3, 1); break;
+      case 4: x = l[2]; y = T[2]; break;
+      case 5: x = d(l, 1, 0); y = d(T, 1, 0); break;
+      case 6: x = d(l, 3, 2); y = d(T, 3, 2); break;
+      default:
+        x = gl::sub(d(l, 3, 1), d(l, 2, 0));
+        y = gl::sub(d(T, 3, 1), d(T, 2, 0));
+    }
+    gl::store_words(gl::mul(x, y, w), rsh[i]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    F raw[8], vals[NV];
+#pragma unroll
+    for (int i = 0; i < 8; i++) raw[i] = gl::load_words<F>(rsh[i]);
+    values_from_raw<F, 2>(raw, vals);
+#pragma unroll
+    for (int i = 0; i < NV; i++) gl::store_words(vals[i], r.partials + ((size_t)blockIdx.x * NV + i) * D);
+  }
+}
+
+// Sums the EQ pass values and runs the two transcript steps, then folds eq_l
+// (both variables of the pass are left ones): half h0 with x, by waves
+// 1..15 while wave 0 runs the second transcript step, then half h1 with y
+// (0 skips a fold).
+template <class F>
+__global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs2_ext(Reduce r, unsigned nblocks, DevFs* t,
+                                                                    uint64_t* poly, uint64_t* chals, uint64_t* cur,
+                                                                    View<F> tab, size_t h0, size_t h1, uint64_t w) {
+  constexpr int NV = pass_values<2>(), D = F::D;
+  __shared__ uint64_t sh[kReduceBlock];
+  __shared__ uint64_t xy[6];
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t);
+  sum_partials_words<NV * D>(r.partials, nblocks, sh);
+  F s[NV], X;
+  if (threadIdx.x < 64) {
+    load_values<F, NV>(sh, s);
+    X = emit_first_ext<F, 2>(s, fs, poly, xy);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    emit_second_ext<F, 2>(s, X, fs, t, poly, chals, cur, w, xy);
+  } else {
+    const auto x = gl::prepare(gl::load_words<F>(xy), w);
+    for (size_t k = threadIdx.x - 64; k < h0; k += blockDim.x - 64) tab.st(k, fold_pre(tab.ld(k), tab.ld(k + h0), x));
+  }
+  __syncthreads();
+  const auto y = gl::prepare(gl::load_words<F>(xy + D), w);
+  for (size_t k = threadIdx.x; k < h1; k += blockDim.x) tab.st(k, fold_pre(tab.ld(k), tab.ld(k + h1), y));
 }
 
 template <class FI, class F, int K, int P>
@@ -306,6 +540,76 @@ int launch_reduce_fs2_ext(effsc_hip_prover* h, const Reduce& red, unsigned g, si
       else
         hipLaunchKernelGGL((k_reduce_fs2_ext<F, 2>), dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs, poly,
                            chals, h->d_cur, h->w);
+      HIPCHK(hipGetLastError());
+      return EFFSC_HIP_OK;
+    }
+  });
+}
+
+int eq_prep_r_ext(effsc_hip_prover* h) {
+  const int R = h->ext == 2 ? gl::KDot<Fp2>::R : gl::KDot<Fp3>::R;
+  const size_t words = (size_t)R * h->eq_r_stride;
+  if (h->eq_rp_words < words) {
+    if (h->eq_rp) (void)hipFree(h->eq_rp);
+    h->eq_rp = nullptr;
+    h->eq_rp_words = 0;
+    TRY(alloc_dev(&h->eq_rp, words));
+    h->eq_rp_words = words;
+  }
+  return with_field(h->ext, [&](auto fo) -> int {
+    using F = decltype(fo);
+    if constexpr (F::D == 1) {
+      return fail(EFFSC_HIP_EINVAL, "extension passes need an extension field");
+    } else {
+      hipLaunchKernelGGL(k_eq_prep_r<F>, dim3(blocks_for(h, h->eq_r_len)), dim3(kBlock), 0, h->stream,
+                         View<F>{h->eq_r, h->eq_r_stride}, h->eq_r_len, h->eq_rp, h->eq_r_stride);
+      HIPCHK(hipGetLastError());
+      return EFFSC_HIP_OK;
+    }
+  });
+}
+
+int eq_pass_ext(effsc_hip_prover* h, int p, size_t q, size_t chunk, unsigned m_log, unsigned g) {
+  if (p != 0 && p != 2) return fail(EFFSC_HIP_EINVAL, "extension passes take 0 or 2 pending folds");
+  if (q % 2 || chunk % 128 || src_stride(h) % 2 || h->work_stride % 2 || h->eq_r_stride % 2)
+    return fail(EFFSC_HIP_EINVAL, "eq pass: odd sizes");
+  return with_field(h->ext, [&](auto fo) -> int {
+    using F = decltype(fo);
+    if constexpr (F::D == 1) {
+      return fail(EFFSC_HIP_EINVAL, "extension passes need an extension field");
+    } else {
+      const View<F> src{src_planes(h)[0], src_stride(h)}, dst{h->work[0], h->work_stride};
+      const View<F> eql{h->eq_l, h->eq_l_stride};
+      TRY(prof_begin(h));
+      if (p == 0)
+        hipLaunchKernelGGL((k_eq_pass_ext<F, 0>), dim3(g), dim3(kBlock), 0, h->stream, src, dst, h->d_cur, q, chunk,
+                           eql, h->eq_rp, h->eq_r_stride, m_log, h->w, h->red);
+      else
+        hipLaunchKernelGGL((k_eq_pass_ext<F, 2>), dim3(g), dim3(kBlock), 0, h->stream, src, dst, h->d_cur, q, chunk,
+                           eql, h->eq_rp, h->eq_r_stride, m_log, h->w, h->red);
+      HIPCHK(hipGetLastError());
+      // p read at level 2^P L and level L written when P > 0; word products:
+      // R per level-L element (Karatsuba) and 3 prepared folds of D^2 each
+      const double L = 4.0 * (double)q;
+      const double bytes = 8.0 * F::D * (p ? 5.0 * L : L);
+      const double muls = L * ((F::D == 2 ? 3.0 : 6.0) + (p ? 3.0 * F::D * F::D : 0.0));
+      TRY(prof_end(h, 3, bytes, muls));
+      return EFFSC_HIP_OK;
+    }
+  });
+}
+
+int eq_reduce_fs2_ext(effsc_hip_prover* h, unsigned g, size_t round, size_t h0, size_t h1) {
+  const int E = h->ext;
+  uint64_t* poly = h->d_polys + round * 2 * E;
+  uint64_t* chals = h->d_chals + round * E;
+  return with_field(E, [&](auto fo) -> int {
+    using F = decltype(fo);
+    if constexpr (F::D == 1) {
+      return fail(EFFSC_HIP_EINVAL, "extension passes need an extension field");
+    } else {
+      hipLaunchKernelGGL(k_eq_reduce_fs2_ext<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, poly,
+                         chals, h->d_cur, View<F>{h->eq_l, h->eq_l_stride}, h0, h1, h->w);
       HIPCHK(hipGetLastError());
       return EFFSC_HIP_OK;
     }

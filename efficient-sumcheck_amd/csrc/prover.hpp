@@ -80,6 +80,10 @@ struct effsc_hip_prover {
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
   uint64_t* eq_point = nullptr;  // canonical point, v * ext words (rebuilds the eq tables on reset)
   uint64_t* eq_full = nullptr;   // EQ = eq_l x eq_r at the register-tail level (at most 2^14 words)
+  // eq_r with its coefficient sums for Karatsuba products (extension-field
+  // passes): 3 planes over F_p^2, 6 over F_p^3, stride eq_r_stride
+  uint64_t* eq_rp = nullptr;
+  size_t eq_rp_words = 0;
   // LSB (pair-split) provers hold their tables bit-reversed and run the MSB
   // path; post-state reads reverse back (src/streams/memory/core.rs:20-47)
   bool lsb = false;
@@ -332,6 +336,13 @@ struct DevModeGuard {
 bool ext_twostep(const effsc_hip_prover* h);
 int launch_pass_ext(effsc_hip_prover* h, int p, size_t q, const uint64_t* cptr);
 int launch_reduce_fs2_ext(effsc_hip_prover* h, const Reduce& red, unsigned g, size_t round);
+// Two-round EqFactored passes over F_p^2 and F_p^3 while both variables of a
+// pass are left variables (split-value form): the prepared eq_r table, one
+// pass over p (p pending folds, q units, chunk units per workgroup), and the
+// two transcript rounds with the folds of eq_l (halves h0 then h1; 0 skips).
+int eq_prep_r_ext(effsc_hip_prover* h);
+int eq_pass_ext(effsc_hip_prover* h, int p, size_t q, size_t chunk, unsigned m_log, unsigned g);
+int eq_reduce_fs2_ext(effsc_hip_prover* h, unsigned g, size_t round, size_t h0, size_t h1);
 
 // EqFactoredProver (eq.hip)
 int eq_round(effsc_hip_prover* h, const uint64_t* chal, uint64_t* out);

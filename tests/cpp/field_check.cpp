@@ -115,5 +115,86 @@ int main(int argc, char** argv) {
     }
     printf("{\"name\": \"fp%d\", \"ok\": true, \"cases\": %ld}\n", d, n);
   }
+  // lazy Karatsuba dot products (gl::KDot): sums of 1..4096 terms a_i r_i
+  // against the sum of schoolbook products, edge-valued and random terms
+  for (int d = 2; d <= 3; d++) {
+    long n = 0;
+    for (int wi = 0; wi < 2; wi++) {
+      const uint64_t w = ws[wi];
+      for (long t = 0; t < 200 + randoms / 20000; t++) {
+        const long terms = t < 64 ? 1 + t : 1 + (long)(rnd() % 4096);
+        uint64_t ref[3] = {0, 0, 0}, got[3] = {0, 0, 0};
+        gl::KDot<gl::Fp2> k2;
+        gl::KDot<gl::Fp3> k3;
+        k2.zero();
+        k3.zero();
+        for (long i = 0; i < terms; i++) {
+          uint64_t a[3], r[3], pr[3];
+          for (int c = 0; c < 3; c++) {
+            // every third term takes edge components (p - 1 maximises the carries)
+            a[c] = (t + i) % 3 == 0 ? specials[(t * 7 + i * 3 + c) % ns] : rnd();
+            r[c] = (t + i) % 3 == 1 ? specials[(t * 5 + i + c * 11) % ns] : rnd();
+          }
+          school(d, w, a, r, pr);
+          for (int c = 0; c < d; c++) ref[c] = radd(ref[c], pr[c]);
+          uint64_t rp[6];
+          if (d == 2) {
+            gl::kdot_prep(gl::Fp2{r[0], r[1]}, rp);
+            k2.add_term(gl::Fp2{a[0], a[1]}, rp);
+          } else {
+            gl::kdot_prep(gl::Fp3{r[0], r[1], r[2]}, rp);
+            k3.add_term(gl::Fp3{a[0], a[1], a[2]}, rp);
+          }
+        }
+        if (d == 2)
+          gl::store_words(k2.value(w), got);
+        else
+          gl::store_words(k3.value(w), got);
+        for (int c = 0; c < d; c++)
+          if (got[c] != ref[c]) fail(d == 2 ? "kdot2" : "kdot3", t);
+        n++;
+      }
+    }
+    printf("{\"name\": \"kdot%d\", \"ok\": true, \"sums\": %ld}\n", d, n);
+  }
+  // two pending folds in one lazy sum (gl::fold2) against the nested
+  // schoolbook folds lo + c (hi - lo)
+  for (int d = 2; d <= 3; d++) {
+    long n = 0;
+    for (int wi = 0; wi < 2; wi++) {
+      const uint64_t w = ws[wi];
+      for (long i = 0; i < (long)ns * ns + randoms / 8; i++) {
+        uint64_t x[4][3], c1[3], c2[3];
+        for (int c = 0; c < 3; c++) {
+          for (int k = 0; k < 4; k++) x[k][c] = i < ns * ns && (k + c) % 2 ? specials[(i + 3 * k + c) % ns] : rnd();
+          c1[c] = i < ns * ns ? specials[(i / ns + c) % ns] : rnd();
+          c2[c] = i < ns * ns ? specials[(i % ns + 2 * c) % ns] : rnd();
+        }
+        auto fold = [&](const uint64_t* lo, const uint64_t* hi, const uint64_t* c, uint64_t* out) {
+          uint64_t df[3], pr[3];
+          for (int k = 0; k < 3; k++) df[k] = rsub(hi[k], lo[k]);
+          school(d, w, c, df, pr);
+          for (int k = 0; k < d; k++) out[k] = radd(pr[k], lo[k]);
+        };
+        uint64_t a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, ref[3] = {0, 0, 0}, got[3] = {0, 0, 0};
+        fold(x[0], x[2], c1, a);
+        fold(x[1], x[3], c1, b);
+        fold(a, b, c2, ref);
+        if (d == 2) {
+          auto e = [&](int k) { return gl::Fp2{x[k][0], x[k][1]}; };
+          const auto p = gl::prepare_fold2(gl::Fp2{c1[0], c1[1]}, gl::Fp2{c2[0], c2[1]}, w);
+          gl::store_words(gl::fold2(e(0), e(1), e(2), e(3), p, w), got);
+        } else {
+          auto e = [&](int k) { return gl::Fp3{x[k][0], x[k][1], x[k][2]}; };
+          const auto p = gl::prepare_fold2(gl::Fp3{c1[0], c1[1], c1[2]}, gl::Fp3{c2[0], c2[1], c2[2]}, w);
+          gl::store_words(gl::fold2(e(0), e(1), e(2), e(3), p, w), got);
+        }
+        for (int c = 0; c < d; c++)
+          if (got[c] != ref[c]) fail(d == 2 ? "fold2_fp2" : "fold2_fp3", i);
+        n++;
+      }
+    }
+    printf("{\"name\": \"fold2_fp%d\", \"ok\": true, \"cases\": %ld}\n", d, n);
+  }
   return 0;
 }

@@ -153,7 +153,8 @@ def test_eq_factored_parity(ext, v):
 
 
 @pytest.mark.parametrize("ext,v", [(1, 13), (1, 14), (1, 15), (1, 17), (1, 18), (1, 20), (1, 23), (3, 11), (3, 12),
-                                   (3, 15), (3, 17), (3, 18), (2, 11), (2, 12), (2, 14), (2, 17)])
+                                   (3, 15), (3, 17), (3, 18), (3, 20), (3, 21), (2, 11), (2, 12), (2, 14), (2, 17),
+                                   (2, 19), (2, 20)])
 def test_eq_factored_device_loop(ext, v, monkeypatch):
     # large p: multi-workgroup rounds with in-reduction eq folds, then the
     # one-workgroup tail; against the oracle and the per-round host loop
@@ -161,15 +162,24 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     # split-value form while both variables of a pass are left ones, else
     # computed on the fly (v = 23 has a pass across the left/right split,
     # v_L = 11), then the register tail over p and EQ from 2^11..2^14
-    # elements; over F_p^3 one round per launch, split-value in the left
-    # phase, then the F_p^3 register tail over p and EQ from 2^10 elements;
-    # EFFSC_HIP_EQ_DIRECT computes EQ per element in every pass,
+    # elements; over F_p^2 and F_p^3 two rounds per pass in the split-value
+    # form while both variables of a pass are left ones (lazy Karatsuba sums
+    # against the prepared eq_r; from v = 17 on), then one round per launch,
+    # then the F_p^3 register tail over p and EQ from 2^10 elements (F_p^2:
+    # the one-workgroup eq tail);
+    # EFFSC_HIP_EQ_DIRECT computes EQ per element in every pass (over the
+    # extensions: one round per launch in the pair form),
     # EFFSC_HIP_EQ_TAIL keeps the one-workgroup eq tail,
-    # EFFSC_HIP_ONE_ROUND keeps one round per launch
+    # EFFSC_HIP_ONE_ROUND keeps one round per launch,
+    # EFFSC_HIP_EQ_FOLD_LIMIT=0 folds the eq tables in their own launches
+    # instead of inside the transcript kernel
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.setenv("EFFSC_HIP_EQ_DIRECT", "1")
     check_proof("eq", ext, ext, 1 << v, v, 950 + v)
     monkeypatch.delenv("EFFSC_HIP_EQ_DIRECT")
+    monkeypatch.setenv("EFFSC_HIP_EQ_FOLD_LIMIT", "0")
+    check_proof("eq", ext, ext, 1 << v, v, 950 + v)
+    monkeypatch.delenv("EFFSC_HIP_EQ_FOLD_LIMIT")
     if ext >= 2:
         monkeypatch.setenv("EFFSC_HIP_EQ_TAIL", "1")
         check_proof("eq", ext, ext, 1 << v, v, 950 + v, transcript="rng")
