@@ -684,15 +684,18 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         size_t m_log = 0;
         while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
         const size_t q = Lv / 4;
-        const size_t gmax = (size_t)h->grid_max * kPassGridScale;
         // split-value form while q is a multiple of 2^m: chunks of at least
-        // 512 units (one step of the workgroup), inside one block of 2^m
+        // 512 units (one step of the workgroup), inside one block of 2^m, and
+        // at most grid_max workgroups (measured v = 24, first pass: 8192
+        // one-step workgroups 38.9 us, 2048 four-step workgroups 29.1 us)
+        const size_t gmax = (size_t)h->grid_max;
         size_t chunk = 2 * kBlock;
         while (q / chunk > gmax) chunk *= 2;
         const bool sv = q % ((size_t)1 << m_log) == 0 && chunk <= ((size_t)1 << m_log) && q % chunk == 0 &&
                         !std::getenv("EFFSC_HIP_EQ_DIRECT");
         const unsigned g = sv ? (unsigned)(q / chunk)
-                              : (unsigned)std::max<size_t>(1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax));
+                              : (unsigned)std::max<size_t>(
+                                    1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax * kPassGridScale));
         const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
         const bool nt = 8.0 * (double)L >= kNtMinBytes;
         TRY(prof_begin(h));

@@ -8,6 +8,84 @@
 namespace effsc_hip {
 namespace pv {
 
+// Lane I's value in every quad of lanes (DPP quad_perm [I, I, I, I]).
+template <int I>
+__device__ __forceinline__ gl::Fp2 quad_bcast(const gl::Fp2& x) {
+  constexpr int ctrl = I | (I << 2) | (I << 4) | (I << 6);
+  return {dpp64<ctrl>(x.c0), dpp64<ctrl>(x.c1)};
+}
+template <int I>
+__device__ __forceinline__ gl::Fp3 quad_bcast(const gl::Fp3& x) {
+  constexpr int ctrl = I | (I << 2) | (I << 4) | (I << 6);
+  return {dpp64<ctrl>(x.c0), dpp64<ctrl>(x.c1), dpp64<ctrl>(x.c2)};
+}
+
+// Fold + evaluate of an extension-field product of K = 3 or 4 tables with
+// the work of a quadruple split over a quad of lanes: lane j folds table j
+// (both elements of the folded pair, written as by k_fused_seq), the quad
+// exchanges the folded pairs by DPP, and lane j evaluates point j (0,
+// infinity, 2, 3) of the pair's product.  A lane's dependent chain is two
+// folds and K - 1 products instead of 2 K folds and K (K - 1) products, for
+// the latency-bound small levels.  Partials: K values per workgroup, as
+// finish<F, K>.
+template <class F, int K>
+__global__ __launch_bounds__(kBlock) void k_fused_split(Views<F, K> in, Views<F, K> out, F c, const uint64_t* cptr,
+                                                        size_t q, uint64_t w, Reduce r) {
+  static_assert(K >= 3 && K <= 4, "a lane per table and per point within a quad");
+  if (cptr) c = gl::load_words<F>(cptr);
+  const auto cp = gl::prepare(c, w);
+  const unsigned j = threadIdx.x & 3;
+  View<F> tin = in.t[0], tout = out.t[0];
+#pragma unroll
+  for (int i = 1; i < K; i++)
+    if (j == (unsigned)i) {
+      tin = in.t[i];
+      tout = out.t[i];
+    }
+  F acc = gl::zero<F>();
+  const size_t step = ((size_t)gridDim.x * blockDim.x) >> 2;
+  for (size_t g = gtid() >> 2; g < q; g += step) {
+    F lo = gl::zero<F>(), hi = gl::zero<F>();
+    if (j < (unsigned)K) {
+      const F x0 = tin.ld(g), x1 = tin.ld(g + q), x2 = tin.ld(g + 2 * q), x3 = tin.ld(g + 3 * q);
+      lo = gl::add(gl::mul(gl::sub(x2, x0), cp), x0);
+      hi = gl::add(gl::mul(gl::sub(x3, x1), cp), x1);
+      tout.st(g, lo);
+      tout.st(g + q, hi);
+    }
+    F prod = gl::zero<F>();
+    auto factor = [&](const F& l, const F& h) {
+      const F d = gl::sub(h, l);
+      F x = j == 0 ? l : j == 1 ? d : gl::add(h, d);
+      if (j == 3) x = gl::add(x, d);
+      return x;
+    };
+    prod = factor(quad_bcast<0>(lo), quad_bcast<0>(hi));
+    prod = gl::mul(prod, factor(quad_bcast<1>(lo), quad_bcast<1>(hi)), w);
+    prod = gl::mul(prod, factor(quad_bcast<2>(lo), quad_bcast<2>(hi)), w);
+    if constexpr (K == 4) prod = gl::mul(prod, factor(quad_bcast<3>(lo), quad_bcast<3>(hi)), w);
+    if (j < (unsigned)K) acc = gl::add(acc, prod);
+  }
+  // point j: lanes j, j + 4, ... of each wave, then the waves
+#pragma unroll
+  for (int m = 32; m >= 4; m >>= 1) acc = gl::add(acc, shfl_xor(acc, m));
+  __shared__ uint64_t ps[kBlock / 64][4][3];
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < 4) {
+    uint64_t wd[3] = {0, 0, 0};
+    gl::store_words(acc, wd);
+#pragma unroll
+    for (int k = 0; k < 3; k++) ps[wave][lane][k] = wd[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)K) {
+    F s = gl::zero<F>();
+#pragma unroll
+    for (int v = 0; v < kBlock / 64; v++) s = gl::add(s, gl::load_words<F>(ps[v][threadIdx.x]));
+    gl::store_words(s, r.partials + ((size_t)blockIdx.x * K + threadIdx.x) * F::D);
+  }
+}
+
 template <class FI, class FO, int K>
 struct OpsImpl {
   static int eval(effsc_hip_prover* h, uint64_t* const* planes, size_t stride, size_t half, size_t paired) {
@@ -40,6 +118,10 @@ struct OpsImpl {
     // against 28.7 us at 2^17; from 2^18 on two per thread are faster.
     if constexpr (FO::D >= 2 && K >= 3)
       vec = vec && q >= ((size_t)1 << 18) && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
+    // below 2^17 quadruples the quadruple's work split over a quad of lanes
+    // (k_fused_split; EFFSC_HIP_FSEQ_SCALAR keeps one quadruple per thread)
+    const bool split = FO::D >= 2 && K >= 3 && K <= 4 && std::is_same<FI, FO>::value && !vec &&
+                       q < ((size_t)1 << 17) && !std::getenv("EFFSC_HIP_FSEQ_SCALAR");
     TRY(prof_begin(h));
     if constexpr (std::is_same<FI, Fp>::value && std::is_same<FO, Fp3>::value && K == 3) {
       // base tables folded into F_p^3: sums of base-field coefficients of
@@ -64,12 +146,21 @@ struct OpsImpl {
         hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in, out,
                            c, cptr, q, h->w, h->red);
     } else if constexpr (FO::D >= 2 && K >= 3) {
-      if (vec)
+      if (vec) {
         hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 2>), dim3(blocks_for(h, q / 2)), dim3(kBlock), 0, h->stream, in,
                            out, c, cptr, q, h->w, h->red);
-      else
+      } else if constexpr (std::is_same<FI, FO>::value && K <= 4) {
+        if (split) {
+          hipLaunchKernelGGL((k_fused_split<FO, K>), dim3(blocks_for(h, 4 * q)), dim3(kBlock), 0, h->stream, in, out,
+                             c, cptr, q, h->w, h->red);
+        } else {
+          hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in,
+                             out, c, cptr, q, h->w, h->red);
+        }
+      } else {
         hipLaunchKernelGGL((k_fused_seq<FI, FO, K, 1>), dim3(blocks_for(h, q)), dim3(kBlock), 0, h->stream, in, out,
                            c, cptr, q, h->w, h->red);
+      }
     } else if (vec) {
       unsigned g = blocks_for(h, q / 2);
       hipLaunchKernelGGL((k_fused<FI, FO, K, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, c, cptr, q, h->w, h->red);
@@ -82,7 +173,7 @@ struct OpsImpl {
     // per quadruple: 2 folded elements per table, one evaluated pair
     TRY(prof_end(h, 0, 8.0 * K * (4.0 * q * FI::D + 2.0 * q * FO::D),
                  (double)q * (2.0 * K * fold_cost<FI, FO>() + eval_products(K) * mul_cost<FO>())));
-    return reduce<FO, K>(h, vec ? blocks_for(h, q / 2) : blocks_for(h, q));
+    return reduce<FO, K>(h, vec ? blocks_for(h, q / 2) : split ? blocks_for(h, 4 * q) : blocks_for(h, q));
   }
   static int fold(effsc_hip_prover* h, uint64_t* const* src, size_t src_stride, const uint64_t* chal, size_t half,
                   size_t paired, const uint64_t* cptr = nullptr) {
