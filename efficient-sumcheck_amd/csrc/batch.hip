@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_bpass(const BInst* insts, const BChu
   const BInst& in = insts[ch.inst];
   const size_t L = (size_t)1 << (in.v - 2 * s), q = L / 4;
   const uint64_t c1 = s ? cur[2 * ch.inst] : 0, c2 = s ? cur[2 * ch.inst + 1] : 0;
+  const uint64_t c12 = gl::mul(c1, c2);
   // the stored level: the input for steps 0 and 1, else level 2s - 2
   const uint64_t* src[2] = {s <= 1 ? in.in[0] : in.work[0], s <= 1 ? in.in[1] : in.work[1]};
   PassAcc<K> acc;
@@ -87,9 +88,9 @@ __global__ __launch_bounds__(kBlock) void k_bpass(const BInst* insts, const BChu
           // only in the next step, after the other instances' traffic: both
           // bypass the caches (measured 12.0 against 12.2 ms for 512 instances)
           if (s == 0)
-            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, c12, E0[i][j], E1[i][j]);
           else
-            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, c12, E0[i][j], E1[i][j]);
         }
       if (s > 0) {
 #pragma unroll
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_bpass(const BInst* insts, const BChu
       for (int i = 0; i < K; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2) : fold_chain<2>(src[i], u + j * q, L, c1, c2);
+          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2, c12)
+                          : fold_chain<2>(src[i], u + j * q, L, c1, c2, c12);
       if (s > 0) {
 #pragma unroll
         for (int i = 0; i < K; i++)

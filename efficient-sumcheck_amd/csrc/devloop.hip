@@ -19,44 +19,35 @@ int launch_pass_kp(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   Views<Fp, K> out = views<Fp, K>(h->work, h->work_stride);
   const size_t L = 4 * q;
   bool vec = (q % 2 == 0) && (sst % 2 == 0) && (h->work_stride % 2 == 0);
-  // a pass streams the whole table once: four workgroups per CU-slot of
-  // grid_max keep more loads in flight (measured 5.3 against 5.05 TB/s for
-  // the v = 28 inner-product read pass)
+  const unsigned gmax = (unsigned)h->grid_max * kPassGridScale;
   const size_t units = vec ? q / 2 : q;
-  unsigned g = (unsigned)std::max<size_t>(
-      1, std::min<size_t>((units + kBlock - 1) / kBlock, (size_t)h->grid_max * kPassGridScale));
+  unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((units + kBlock - 1) / kBlock, gmax));
   TRY(prof_begin(h));
   // fold passes over planes of 128 MiB and more stream their source and the
   // written level past the caches (nontemporal loads and stores): neither is
   // reused before it would be evicted; measured 2.75 against 2.85 ms for the
   // v = 28 inner product, no gain below that size
   const bool nt = vec && P > 0 && 8.0 * (double)(L << P) >= kNtMinBytes;
-  // the read pass (P = 0) gives each workgroup one contiguous chunk of units;
-  // the fold passes keep the grid-stride order.  Measured for v = 28: the
-  // chunked read pass took the proof from 2.52-2.62 to 2.38 ms, chunked fold
-  // passes were slower (2.66 ms)
-  size_t chunk = 0;
-  unsigned gpb = 1;
-  // groups of 8..2048 workgroups per chunk for the read pass, or 2..128 for
-  // the fold passes, measured within noise of these two choices
-  const unsigned G = P == 0 ? g : 1;
-  if (vec && G > 1) {
-    // G groups of gpb workgroups; each group strides over its own chunk
-    const unsigned groups = std::min<unsigned>((unsigned)G, g);
-    gpb = std::max<unsigned>(1, g / groups);
-    const size_t per_step = (size_t)kBlock * 2 * gpb;
-    chunk = std::max<size_t>(1, (q + per_step * groups - 1) / (per_step * groups)) * per_step;
-    g = (unsigned)((q + chunk - 1) / chunk) * gpb;
+  // wave chunks (k_pass): C consecutive 128-unit steps per wave, C enough to
+  // cover the table with gmax workgroups and at most 8 (read pass) or 4 (fold
+  // passes); the workgroups are as many as the chunks fill
+  size_t span = 0;
+  if (vec) {
+    const size_t steps = (q + 127) / 128, waves = (size_t)gmax * (kBlock / 64);
+    const size_t C = std::min<size_t>(P == 0 ? 8 : 4, std::max<size_t>(1, (steps + waves - 1) / waves));
+    span = C * 128;
+    const size_t chunks = (q + span - 1) / span;
+    g = (unsigned)std::max<size_t>(1, std::min<size_t>((chunks + kBlock / 64 - 1) / (kBlock / 64), gmax));
   }
   if (nt)
     hipLaunchKernelGGL((k_pass<K, P, 2, 3>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1],
-                       cptr, q, h->red, chunk, gpb);
+                       cptr, q, h->red, span);
   else if (vec)
     hipLaunchKernelGGL((k_pass<K, P, 2>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
-                       q, h->red, chunk, gpb);
+                       q, h->red, span);
   else
     hipLaunchKernelGGL((k_pass<K, P, 1>), dim3(g), dim3(kBlock), 0, h->stream, in, out, h->fpend[0], h->fpend[1], cptr,
-                       q, h->red);
+                       q, h->red, (size_t)0);
   HIPCHK(hipGetLastError());
   double bytes = 8.0 * K * ((double)(L << P) + (P > 0 ? (double)L : 0.0));
   // per unit: 4 level-L elements per table, each 0, 1 or 3 fold products

@@ -230,7 +230,7 @@ template <int P, bool NT>
 __global__ __launch_bounds__(kBlock) void k_eq_pass(const uint64_t* src, uint64_t* dst, const uint64_t* cptr, size_t q,
                                                     const uint64_t* eql, const uint64_t* eqr, unsigned m_log,
                                                     Reduce r) {
-  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0;
+  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0, c12 = P == 2 ? gl::mul(c1, c2) : 0;
   const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
   PassAcc<2> acc;
 #pragma unroll
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_eq_pass(const uint64_t* src, uint64_
     uint64_t E0[2][4], E1[2][4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, E0[0][j], E1[0][j]);
+      fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, c12, E0[0][j], E1[0][j]);
       const size_t x = u + j * q;
       E0[1][j] = gl::mul(eql[x >> m_log], eqr[x & mask]);
       E1[1][j] = gl::mul(eql[(x + 1) >> m_log], eqr[(x + 1) & mask]);
@@ -280,7 +280,7 @@ template <int P, bool NT>
 __global__ __launch_bounds__(kBlock) void k_eq_pass_sv(const uint64_t* src, uint64_t* dst, const uint64_t* cptr,
                                                        size_t q, size_t chunk, const uint64_t* eql,
                                                        const uint64_t* eqr, unsigned m_log, Reduce r) {
-  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0;
+  const uint64_t c1 = P ? cptr[0] : 0, c2 = P ? cptr[1] : 0, c12 = P == 2 ? gl::mul(c1, c2) : 0;
   const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
   Acc128 T[4];
 #pragma unroll
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_eq_pass_sv(const uint64_t* src, uint
   for (size_t u = u0 + threadIdx.x * 2; u < u0 + chunk; u += (size_t)blockDim.x * 2) {
     uint64_t E0[4], E1[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, E0[j], E1[j]);
+    for (int j = 0; j < 4; j++) fold_chain2<P, NT || P == 0>(src, u + j * q, L, c1, c2, c12, E0[j], E1[j]);
     if constexpr (P > 0) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -603,10 +603,10 @@ bool eq_device_applies(const effsc_hip_prover* h, size_t num_rounds) {
 __global__ __launch_bounds__(kBlock) void k_eq_tail_prep(const uint64_t* src, uint64_t* dst, const uint64_t* cptr,
                                                          size_t Lt, const uint64_t* eql, const uint64_t* eqr,
                                                          unsigned m_log, uint64_t* eqf) {
-  const uint64_t c1 = cptr[0], c2 = cptr[1];
+  const uint64_t c1 = cptr[0], c2 = cptr[1], c12 = gl::mul(c1, c2);
   const size_t mask = ((size_t)1 << m_log) - 1;
   for (size_t e = gtid(); e < Lt; e += (size_t)gridDim.x * blockDim.x) {
-    dst[e] = fold_chain<2>(src, e, Lt, c1, c2);
+    dst[e] = fold_chain<2>(src, e, Lt, c1, c2, c12);
     eqf[e] = gl::mul(eql[e >> m_log], eqr[e & mask]);
   }
 }

@@ -114,33 +114,81 @@ GLHD Fp2 sub(Fp2 a, Fp2 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1)}; }
 GLHD Fp3 add(Fp3 a, Fp3 b) { return {add(a.c0, b.c0), add(a.c1, b.c1), add(a.c2, b.c2)}; }
 GLHD Fp3 sub(Fp3 a, Fp3 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1), sub(a.c2, b.c2)}; }
 
-// Sum of up to three 64x64 products without intermediate reduction:
-// value = lo + 2^64 hi + 2^128 top, top <= 2.
+// A sum of a few 64x64 products and words, reduced once:
+//   value = w0 + 2^32 w1 + 2^64 w2 + 2^96 w3 + 2^128 top.
+// Adding a 128-bit product is one carry chain of five 32-bit adds on gfx950
+// (the compiler's 64-bit form compares for every carry: 24 against 15
+// instructions per product).  The reduction treats H = top:w3 as one word:
+// with 2^64 = 2^32 - 1 and 2^96 = -1 (mod p), 2^96 H + 2^64 w2 is congruent
+// to w2 (2^32 - 1) - H, which is reduce128's step with hh widened to 64 bits;
+// it stays exact while H <= p, i.e. top <= 2^32 - 2.
 struct Acc3 {
-  uint64_t lo, hi;
-  uint32_t top;
+  uint32_t w0, w1, w2, w3, top;
 };
+GLHD Acc3 wide(uint64_t x) { return {(uint32_t)x, (uint32_t)(x >> 32), 0, 0, 0}; }
+GLHD void add_wide(Acc3& a, uint64_t lo, uint64_t hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // a carry written to VCC is read two wait states later (s_nop 1), as the
+  // compiler schedules its own add-with-carry pairs on gfx950
+  asm("v_add_co_u32 %0, vcc, %0, %5\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %1, vcc, %1, %6, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %2, vcc, %2, %7, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %3, vcc, %3, %8, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %4, vcc, %4, 0, vcc"
+      : "+v"(a.w0), "+v"(a.w1), "+v"(a.w2), "+v"(a.w3), "+v"(a.top)
+      : "v"((uint32_t)lo), "v"((uint32_t)(lo >> 32)), "v"((uint32_t)hi), "v"((uint32_t)(hi >> 32))
+      : "vcc");
+#else
+  const uint64_t alo = ((uint64_t)a.w1 << 32) | a.w0, ahi = ((uint64_t)a.w3 << 32) | a.w2;
+  const uint64_t s = alo + lo;
+  const uint64_t c = s < lo;
+  const uint64_t h = ahi + hi;
+  uint32_t t = h < hi;
+  const uint64_t h2 = h + c;
+  t += h2 < h;
+  a = {(uint32_t)s, (uint32_t)(s >> 32), (uint32_t)h2, (uint32_t)(h2 >> 32), a.top + t};
+#endif
+}
 GLHD Acc3 prod(uint64_t x, uint64_t y) {
-  Acc3 a;
-  mul128(x, y, a.lo, a.hi);
-  a.top = 0;
-  return a;
+  uint64_t lo, hi;
+  mul128(x, y, lo, hi);
+  return {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32), 0};
 }
 GLHD void mac(Acc3& a, uint64_t x, uint64_t y) {
-  uint64_t plo, phi;
-  mul128(x, y, plo, phi);
-  uint64_t s = a.lo + plo;
-  uint64_t c = s < plo;
-  a.lo = s;
-  uint64_t h = a.hi + phi;
-  uint32_t t = h < phi;
-  uint64_t h2 = h + c;
-  t += h2 < h;
-  a.hi = h2;
-  a.top += t;
+  uint64_t lo, hi;
+  mul128(x, y, lo, hi);
+  add_wide(a, lo, hi);
 }
-// 2^128 = -2^32 (mod p)
-GLHD uint64_t reduce(const Acc3& a) { return sub(reduce128(a.lo, a.hi), (uint64_t)a.top << 32); }
+GLHD uint64_t reduce(const Acc3& a) {
+  const uint64_t lo = ((uint64_t)a.w1 << 32) | a.w0;
+  const uint64_t H = ((uint64_t)a.top << 32) | a.w3;
+  const uint64_t hl = a.w2;
+  uint64_t t0 = lo - H;
+  if (lo < H) t0 -= EPS;             // borrow: lo - H + p (>= 0 since H <= p)
+  const uint64_t t1 = (hl << 32) - hl;
+  uint64_t r = t0 + t1;
+  if (r < t1) r += EPS;              // carry: cannot overflow again
+  return (r >= P) ? r - P : r;
+}
+
+// Two pending folds of the half-split table in one reduction: the element
+// of the twice-folded table from x0 = T[e], x1 = T[e + L], x2 = T[e + 2L],
+// x3 = T[e + 3L] (first fold c1 over the top variable, then c2) is
+//   a + c2 (b - a), a = x0 + c1 (x2 - x0), b = x1 + c1 (x3 - x1)
+//   = x0 + c1 (x2 - x0) + c2 (x1 - x0) + c1 c2 ((x3 - x1) - (x2 - x0)),
+// three products summed unreduced (c12 = c1 c2 is prepared once).
+GLHD uint64_t fold2(uint64_t x0, uint64_t x1, uint64_t x2, uint64_t x3, uint64_t c1, uint64_t c2, uint64_t c12) {
+  const uint64_t d20 = sub(x2, x0), d10 = sub(x1, x0), dd = sub(sub(x3, x1), d20);
+  Acc3 s = wide(x0);
+  mac(s, c1, d20);
+  mac(s, c2, d10);
+  mac(s, c12, dd);
+  return reduce(s);
+}
 
 // A multiplier prepared for products mod X^2 - w: b and w b1.
 struct Fp2Pre {
@@ -207,6 +255,11 @@ struct AccW {
   uint32_t k0, k1, k2;
 };
 GLHD void zero(AccW& a) { a = {0, 0, 0, 0, 0, 0}; }
+// a word x into the low column
+GLHD void addw(AccW& a, uint64_t x) {
+  a.s0 += x;
+  a.k0 += a.s0 < x;
+}
 GLHD void macw(AccW& a, uint64_t x, uint64_t y) {
   const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -25,36 +25,16 @@
 
 namespace effsc_hip {
 
-// Unreduced sum of products: value = lo + 2^64 hi + 2^128 top (< 2^160).
+// Unreduced sum of products and words kept in 32-bit columns (gl::AccW):
+// one product is four v_mad_u64_u32 with the column as addend and carry out
+// plus four carry counts (8 instructions against 24 for a 160-bit sum with
+// compares), one word an add and a carry count.
 struct Acc128 {
-  uint64_t lo, hi;
-  uint32_t top;
-  __device__ __forceinline__ void zero() {
-    lo = hi = 0;
-    top = 0;
-  }
-  __device__ __forceinline__ void add128(uint64_t plo, uint64_t phi) {
-    uint64_t s = lo + plo;
-    uint64_t c1 = s < plo;
-    lo = s;
-    uint64_t h = hi + phi;
-    uint32_t c2 = h < phi;
-    uint64_t h2 = h + c1;
-    c2 += h2 < h;
-    hi = h2;
-    top += c2;
-  }
-  __device__ __forceinline__ void mac(uint64_t x, uint64_t y) {
-    uint64_t plo, phi;
-    gl::mul128(x, y, plo, phi);
-    add128(plo, phi);
-  }
-  __device__ __forceinline__ void add(uint64_t x) { add128(x, 0); }
-  // canonical residue: 2^128 = -2^32 (mod p)
-  __device__ __forceinline__ Fp reduce() const {
-    uint64_t r = gl::reduce128(lo, hi);
-    return {gl::add(r, gl::mul((uint64_t)top, 0xFFFFFFFE00000001ull))};
-  }
+  gl::AccW w;
+  __device__ __forceinline__ void zero() { gl::zero(w); }
+  __device__ __forceinline__ void mac(uint64_t x, uint64_t y) { gl::macw(w, x, y); }
+  __device__ __forceinline__ void add(uint64_t x) { gl::addw(w, x); }
+  __device__ __forceinline__ Fp reduce() const { return {gl::reduce(w)}; }
 };
 
 // Number of values a pass produces: K for round s, K (K+1) coefficients for
@@ -128,26 +108,25 @@ __device__ __forceinline__ void pass_values_from_raw(const Fp* s, Fp* v) {
   }
 }
 
-// Level-L element at index e from the level-M table with P pending folds.
+// Level-L element at index e from the level-M table with P pending folds
+// (c12 = c1 c2, used when P = 2).
 template <int P>
-__device__ __forceinline__ uint64_t fold_chain(const uint64_t* src, size_t e, size_t L, uint64_t c1, uint64_t c2) {
+__device__ __forceinline__ uint64_t fold_chain(const uint64_t* src, size_t e, size_t L, uint64_t c1, uint64_t c2,
+                                               uint64_t c12) {
   if constexpr (P == 0) {
     return src[e];
   } else if constexpr (P == 1) {
     uint64_t x0 = src[e], x1 = src[e + L];
     return gl::add(x0, gl::mul(c1, gl::sub(x1, x0)));
   } else {
-    uint64_t x0 = src[e], x1 = src[e + L], x2 = src[e + 2 * L], x3 = src[e + 3 * L];
-    uint64_t a = gl::add(x0, gl::mul(c1, gl::sub(x2, x0)));
-    uint64_t b = gl::add(x1, gl::mul(c1, gl::sub(x3, x1)));
-    return gl::add(a, gl::mul(c2, gl::sub(b, a)));
+    return gl::fold2(src[e], src[e + L], src[e + 2 * L], src[e + 3 * L], c1, c2, c12);
   }
 }
 
 // Vector form: elements e and e+1 (e even, 16-byte aligned planes).
 template <int P, bool NT>
 __device__ __forceinline__ void fold_chain2(const uint64_t* src, size_t e, size_t L, uint64_t c1, uint64_t c2,
-                                            uint64_t& o0, uint64_t& o1) {
+                                            uint64_t c12, uint64_t& o0, uint64_t& o1) {
   auto ld = [&](size_t i) -> v2u64 {
     if constexpr (NT)
       return __builtin_nontemporal_load(reinterpret_cast<const v2u64*>(src + i));
@@ -164,44 +143,37 @@ __device__ __forceinline__ void fold_chain2(const uint64_t* src, size_t e, size_
     o1 = gl::add(x0.y, gl::mul(c1, gl::sub(x1.y, x0.y)));
   } else {
     v2u64 x0 = ld(e), x1 = ld(e + L), x2 = ld(e + 2 * L), x3 = ld(e + 3 * L);
-    uint64_t a0 = gl::add(x0.x, gl::mul(c1, gl::sub(x2.x, x0.x)));
-    uint64_t a1 = gl::add(x0.y, gl::mul(c1, gl::sub(x2.y, x0.y)));
-    uint64_t b0 = gl::add(x1.x, gl::mul(c1, gl::sub(x3.x, x1.x)));
-    uint64_t b1 = gl::add(x1.y, gl::mul(c1, gl::sub(x3.y, x1.y)));
-    o0 = gl::add(a0, gl::mul(c2, gl::sub(b0, a0)));
-    o1 = gl::add(a1, gl::mul(c2, gl::sub(b1, a1)));
+    o0 = gl::fold2(x0.x, x1.x, x2.x, x3.x, c1, c2, c12);
+    o1 = gl::fold2(x0.y, x1.y, x2.y, x3.y, c1, c2, c12);
   }
 }
 
 // One pass: P pending folds from the level-M tables `src` to level L = 4 q,
 // written to `dst` when P > 0; accumulates the pass values.  Challenges come
 // from `cptr` (device words c1, c2) when non-null, else from c1, c2.
+//
+// Unit order: grid-stride (span == 0), or wave chunks: each wave owns
+// `span` consecutive units (a multiple of 64 VEC) at a time, so a wave reads
+// every region of the pass in runs of span/64 consecutive 1 KB lines.  For the
+// v = 28 inner product, runs of four lines took the read pass from 688 to
+// 655 us and the first fold pass from 1030 to 965 us (runs of one line are the
+// grid-stride order; longer runs measured no faster).  The chunk loop's unit
+// index goes through an empty asm statement: otherwise loop strength
+// reduction keeps one 64-bit pointer per region live across the loop (269
+// VGPRs, one wave per SIMD, for the fold pass against 122).
 template <int K, int P, int VEC, int NTM = 0>
 __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> dst, uint64_t c1, uint64_t c2,
-                                                 const uint64_t* cptr, size_t q, Reduce r, size_t chunk = 0,
-                                                 unsigned gpb = 1) {
+                                                 const uint64_t* cptr, size_t q, Reduce r, size_t span = 0) {
   if (cptr) {
     c1 = cptr[0];
     c2 = cptr[1];
   }
+  const uint64_t c12 = P == 2 ? gl::mul(c1, c2) : 0;
   const size_t L = 4 * q;
   PassAcc<K> acc;
 #pragma unroll
   for (int i = 0; i < pass_raw<K>(); i++) acc.a[i].zero();
-  // grid-stride (chunk == 0), or groups of gpb workgroups each striding over
-  // one contiguous chunk of units
-  size_t u0, uend, step;
-  if (chunk) {
-    const size_t grp = blockIdx.x / gpb, k = blockIdx.x % gpb;
-    u0 = grp * chunk + (k * blockDim.x + threadIdx.x) * VEC;
-    uend = min(q, (grp + 1) * chunk);
-    step = (size_t)gpb * blockDim.x * VEC;
-  } else {
-    u0 = gtid() * VEC;
-    uend = q;
-    step = (size_t)gridDim.x * blockDim.x * VEC;
-  }
-  for (size_t u = u0; u < uend; u += step) {
+  auto unit = [&](size_t u) {
     uint64_t E0[K][4], E1[K][4];
 #pragma unroll
     for (int i = 0; i < K; i++) {
@@ -209,10 +181,12 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
       for (int j = 0; j < 4; j++) {
         if constexpr (VEC == 2) {
           // a pass without folds reads each element exactly once: stream it
-          fold_chain2<P, P == 0 || (NTM & 1)>(src.t[i].p, u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+          fold_chain2<P, P == 0 || (NTM & 1)>(src.t[i].p, u + j * q, L, c1, c2, c12, E0[i][j], E1[i][j]);
         } else {
-          E0[i][j] = fold_chain<P>(src.t[i].p, u + j * q, L, c1, c2);
+          E0[i][j] = fold_chain<P>(src.t[i].p, u + j * q, L, c1, c2, c12);
         }
+        // one region's loads and folds at a time: bounds the live registers
+        if constexpr (P > 0) __builtin_amdgcn_sched_barrier(0);
       }
     }
     if constexpr (P > 0) {
@@ -230,6 +204,22 @@ __global__ __launch_bounds__(kBlock) void k_pass(Views<Fp, K> src, Views<Fp, K> 
     }
     pass_accumulate<K>(E0, acc);
     if constexpr (VEC == 2) pass_accumulate<K>(E1, acc);
+  };
+  if (span) {
+    const size_t lane = threadIdx.x & 63;
+    const size_t wv = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const size_t nwv = (size_t)gridDim.x * (blockDim.x >> 6);
+    for (size_t b0 = wv * span; b0 < q; b0 += nwv * span) {
+      const size_t bend = min(q, b0 + span);
+#pragma unroll 1
+      for (size_t u = b0 + VEC * lane; u < bend; u += 64 * VEC) {
+        size_t uo = u;
+        asm volatile("" : "+v"(uo));
+        unit(uo);
+      }
+    }
+  } else {
+    for (size_t u = gtid() * VEC; u < q; u += (size_t)gridDim.x * blockDim.x * VEC) unit(u);
   }
   Fp raw[pass_raw<K>()], vals[pass_values<K>()];
 #pragma unroll

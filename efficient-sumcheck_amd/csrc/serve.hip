@@ -257,7 +257,7 @@ template <int K>
 __device__ __forceinline__ void serve_pass_block(const SInst* ip, const RItem& it, uint64_t b, uint64_t* partials) {
   const uint32_t s = it.j, v = ip->v;
   const size_t L = (size_t)1 << (v - 2 * s), q = L / 4;
-  const uint64_t c1 = s ? ip->cur[0] : 0, c2 = s ? ip->cur[1] : 0;
+  const uint64_t c1 = s ? ip->cur[0] : 0, c2 = s ? ip->cur[1] : 0, c12 = gl::mul(c1, c2);
   const uint64_t* src[2] = {s <= 1 ? ip->in[0] : ip->work[0], s <= 1 ? ip->in[1] : ip->work[1]};
   const uint64_t begin = b * it.per, end = std::min<uint64_t>(begin + it.per, it.units);
   PassAcc<K> acc;
@@ -272,9 +272,9 @@ __device__ __forceinline__ void serve_pass_block(const SInst* ip, const RItem& i
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           if (s == 0)
-            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+            fold_chain2<0, true>(src[i], u + j * q, L, c1, c2, c12, E0[i][j], E1[i][j]);
           else
-            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, E0[i][j], E1[i][j]);
+            fold_chain2<2, true>(src[i], u + j * q, L, c1, c2, c12, E0[i][j], E1[i][j]);
         }
       if (s > 0) {
 #pragma unroll
@@ -293,7 +293,8 @@ __device__ __forceinline__ void serve_pass_block(const SInst* ip, const RItem& i
       for (int i = 0; i < K; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2) : fold_chain<2>(src[i], u + j * q, L, c1, c2);
+          E[i][j] = s == 0 ? fold_chain<0>(src[i], u + j * q, L, c1, c2, c12)
+                          : fold_chain<2>(src[i], u + j * q, L, c1, c2, c12);
       if (s > 0) {
 #pragma unroll
         for (int i = 0; i < K; i++)
@@ -344,7 +345,7 @@ __device__ __forceinline__ void serve_final2(SInst* ip, uint32_t s) {
   uint64_t t[2][2] = {{0, 0}, {0, 0}};
 #pragma unroll
   for (int i = 0; i < K; i++)
-    for (size_t e = 0; e < n2; e++) t[i][e] = s == 0 ? src[i][e] : fold_chain<2>(src[i], e, M / 4, x, y);
+    for (size_t e = 0; e < n2; e++) t[i][e] = s == 0 ? src[i][e] : fold_chain<2>(src[i], e, M / 4, x, y, gl::mul(x, y));
   uint64_t f[2] = {t[0][0], t[1][0]};
   if (n2 == 2) {
     // round v - 1 over the pair (t[.][0], t[.][1]): ML [h(0)], IP [h(0), h(inf)]

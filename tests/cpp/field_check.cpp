@@ -196,5 +196,56 @@ int main(int argc, char** argv) {
     }
     printf("{\"name\": \"fold2_fp%d\", \"ok\": true, \"cases\": %ld}\n", d, n);
   }
+  // base field: two pending folds in one 160-bit sum (gl::fold2 with
+  // c12 = c1 c2) against the nested folds; the 160-bit sum of a word and up
+  // to three products (gl::Acc3) against the exact sum; the column sums
+  // (gl::AccW) of up to 4096 products and words against the exact sum
+  {
+    long n = 0;
+    for (long i = 0; i < (long)ns * ns * ns + randoms / 4; i++) {
+      uint64_t x[4], c1, c2;
+      if (i < (long)ns * ns * ns) {
+        for (int k = 0; k < 4; k++) x[k] = specials[(i * (k + 1) + k) % ns];
+        c1 = specials[i % ns];
+        c2 = specials[(i / ns) % ns];
+        if ((i / ns / ns) % 2) x[3] = specials[(i / ns / ns) % ns];
+      } else {
+        for (int k = 0; k < 4; k++) x[k] = rnd();
+        c1 = rnd();
+        c2 = rnd();
+      }
+      const uint64_t a = radd(x[0], rmul(c1, rsub(x[2], x[0]))), b = radd(x[1], rmul(c1, rsub(x[3], x[1])));
+      const uint64_t ref = radd(a, rmul(c2, rsub(b, a)));
+      if (gl::fold2(x[0], x[1], x[2], x[3], c1, c2, gl::mul(c1, c2)) != ref) fail("fold2_base", i);
+      // x0 + x1 x2 + x3 c1 + c2 c2 as one Acc3
+      gl::Acc3 s = gl::wide(x[0]);
+      gl::mac(s, x[1], x[2]);
+      gl::mac(s, x[3], c1);
+      gl::mac(s, c2, c2);
+      const uint64_t sref = radd(radd(x[0], rmul(x[1], x[2])), radd(rmul(x[3], c1), rmul(c2, c2)));
+      if (gl::reduce(s) != sref) fail("acc3_sum", i);
+      n++;
+    }
+    for (long t = 0; t < 200 + randoms / 20000; t++) {
+      const long terms = t < 64 ? 1 + t : 1 + (long)(rnd() % 4096);
+      gl::AccW acc;
+      gl::zero(acc);
+      uint64_t ref = 0;
+      for (long i = 0; i < terms; i++) {
+        // edge operands on most terms: p - 1 maximises every column's carries
+        const uint64_t x = (t + i) % 4 ? specials[(t * 3 + i) % ns] : rnd();
+        const uint64_t y = (t + i) % 4 != 1 ? specials[(t + i * 5) % ns] : rnd();
+        gl::macw(acc, x, y);
+        ref = radd(ref, rmul(x, y));
+        if (i % 3 == 0) {
+          gl::addw(acc, x);
+          ref = radd(ref, x);
+        }
+      }
+      if (gl::reduce(acc) != ref) fail("accw_sum", t);
+      n++;
+    }
+    printf("{\"name\": \"fold2_base\", \"ok\": true, \"cases\": %ld}\n", n);
+  }
   return 0;
 }
