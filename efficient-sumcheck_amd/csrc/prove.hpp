@@ -49,74 +49,87 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, true);
   return ((uint64_t)hi << 32) | lo;
 }
-// t[l] ^ t[l ^ 32] in every lane.  permlane32_swap(a, b) leaves a with b's
-// rows 0,1 in its rows 2,3 and b with a's rows 2,3 in its rows 0,1; for
-// a = b = t the xor of the two results is the sum in every row.
-__device__ __forceinline__ uint64_t xor_lane32(uint64_t t) {
-  auto l = __builtin_amdgcn_permlane32_swap((uint32_t)t, (uint32_t)t, false, false);
-  auto h = __builtin_amdgcn_permlane32_swap((uint32_t)(t >> 32), (uint32_t)(t >> 32), false, false);
-  return ((uint64_t)(h[0] ^ h[1]) << 32) | (l[0] ^ l[1]);
-}
-// t[l] ^ t[l ^ 16]: permlane16_swap exchanges odd rows of a with even rows
-// of b.
-__device__ __forceinline__ uint64_t xor_lane16(uint64_t t) {
-  auto l = __builtin_amdgcn_permlane16_swap((uint32_t)t, (uint32_t)t, false, false);
-  auto h = __builtin_amdgcn_permlane16_swap((uint32_t)(t >> 32), (uint32_t)(t >> 32), false, false);
-  return ((uint64_t)(h[0] ^ h[1]) << 32) | (l[0] ^ l[1]);
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 
 // Keccak-f[1600] spread over a wave.  Interface: lane i < 25 holds state word
 // (x, y) = (i % 5, i / 5) on entry and on return (lanes >= 25: ignored on
 // entry, word 0 on return).  All 64 lanes must call.
 //
-// Inside, row y of the state occupies the 8-lane group g = y: lane 8y + x
-// holds A[x, y] for x < 5 and lanes 8y + 5..7 copies of x = 0..2, so that
-// "x + 1" and "x + 2" (mod 5) are plain row shifts; groups 5..7 are zero.
-//   theta: column sums are xors over the groups at one position: row_ror:8
-//          (groups 2r, 2r+1), then the lane ^ 32 and lane ^ 16 swaps; C[x+1]
-//          is row_shl:1, C[x-1] row_shr:1 (row_shl:4 for x = 0)
-//   rho+pi: one shuffle from the source lane, rotated by its offset
-//   chi:   row_shl:1 and row_shl:2; then row_shr:5 refreshes the copies
-// One ds_bpermute per round instead of ten (measured 4.5 against 6.6 us
-// per permutation on gfx950).  The rho rotation by the lane's offset r is
-// two v_alignbit_b32 on the halves, taken in swapped order when r = 0 or
-// r > 32 (3.62 against 3.69 us with 64-bit shifts; unrolling the 24 rounds
-// gave 3.38 us but grew every kernel that absorbs, and some call sites
-// would not unroll).
+// Inside, row y of the state occupies the 8-lane group g = y, each word as two
+// 32-bit halves (lo, hi).  Position p of a group holds x = (p + 4) mod 5:
+// p = 1..5 are x = 0..4, p = 0 a copy of x = 4 and p = 6, 7 copies of x = 0, 1,
+// so that for p = 1..5 the neighbours x - 1, x + 1, x + 2 are the positions
+// p - 1, p + 1, p + 2 (plain DPP row shifts, no selects).  Groups 5..7 are zero.
+//   theta: the column sums are one packed reduction: row_ror:8 xors the two
+//          groups of each 16-lane row; permlane32_swap of (lo, hi) and a xor
+//          leave the lo sums of rows r, r ^ 2 in rows 0, 1 and the hi sums in
+//          rows 2, 3; permlane16_swap and a xor finish both (lo parity in rows
+//          0, 1, hi parity in rows 2, 3); a last permlane32_swap spreads each
+//          half to all rows.  D = C[x - 1] ^ rotl(C[x + 1], 1) is row_shr:1 and
+//          row_shl:1 and a three-input xor (v_bitop3).
+//   rho+pi: one ds_bpermute per half from the source word's lane (every
+//          position, copies included, reads a main lane, so the copies are
+//          exact afterwards), rotated by the source's offset r as two
+//          v_alignbit_b32 on the halves, taken in swapped order when r = 0 or
+//          r > 32
+//   chi:   row_shl:1 and row_shl:2 and one v_bitop3 per half (exact for
+//          p = 0..5); iota one v_bitop3 on lane (0, 0); row_shr:5 refreshes the
+//          copy at p = 6, the only copy theta reads that chi leaves stale
+//          (p = 7 is read by chi only, and pi rewrites it)
+// About 35 VALU operations and two ds_bpermute per round, against about 60
+// with 64-bit lanes (one select per theta, two-instruction chi and iota).
 __device__ __forceinline__ uint64_t keccak_lanes(uint64_t a_in) {
   const int lane = threadIdx.x & 63;
-  const int g = lane >> 3, j = lane & 7;
+  const int g = lane >> 3, p = lane & 7;
   const bool active = g < 5;
-  const int x = j < 5 ? j : j - 5, y = active ? g : 0;
+  const int x = (p + 4) % 5, y = active ? g : 0;
   const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
-  // pi: destination (x, y) reads source (3 (y - 3x) mod 5, x)
+  // pi: destination (x, y) reads source (3 (y - 3x) mod 5, x) at its main
+  // position sx + 1
   const int sx = (3 * (y - 3 * x) % 5 + 25) % 5, sy = x;
-  const int src = 8 * sy + sx;
+  const int src4 = 4 * (8 * sy + sx + 1);
   const uint32_t r = rot[sx + 5 * sy];
   // rotl by r in (0, 32]: hi' = alignbit(hi, lo, t), lo' = alignbit(lo, hi, t)
   // with t = (32 - r) mod 32; for r in (32, 64) and r = 0 the operands swap
   const uint32_t tsh = (32 - (r & 31)) & 31;
   const bool swp = r == 0 || r > 32;
-  // masks instead of selects keep every DPP read under a full EXEC
-  const uint64_t m_iota = lane == 0 ? ~0ull : 0ull, m_x0 = j == 0 ? ~0ull : 0ull;
-  const uint64_t m_copy = j >= 5 ? ~0ull : 0ull, m_act = active ? ~0ull : 0ull;
-  uint64_t a = shfl64(a_in, x + 5 * y) & m_act;
+  const uint32_t m_iota = lane == 1 ? ~0u : 0u;
+  const bool copy = p == 6;
+  const uint64_t a0 = shfl64(a_in, x + 5 * y);
+  uint32_t lo = active ? (uint32_t)a0 : 0u, hi = active ? (uint32_t)(a0 >> 32) : 0u;
   for (int rnd = 0; rnd < 24; rnd++) {
-    const uint64_t c = xor_lane16(xor_lane32(a ^ dpp64<0x128>(a)));  // row_ror:8, rows r ^ 2, rows r ^ 1
-    const uint64_t cp = dpp64<0x101>(c);                           // row_shl:1
-    const uint64_t c4 = dpp64<0x104>(c), c1 = dpp64<0x111>(c);     // row_shl:4, row_shr:1
-    a ^= ((c4 & m_x0) | (c1 & ~m_x0)) ^ rotl64(cp, 1);
-    const uint64_t s = shfl64(a, src);
-    const uint32_t s_lo = (uint32_t)s, s_hi = (uint32_t)(s >> 32);
-    const uint32_t p = swp ? s_lo : s_hi, q = swp ? s_hi : s_lo;
-    const uint64_t b = ((uint64_t)__builtin_amdgcn_alignbit(p, q, tsh) << 32) | __builtin_amdgcn_alignbit(q, p, tsh);
-    a = b ^ (~dpp64<0x101>(b) & dpp64<0x102>(b));  // row_shl:1, row_shl:2
-    a ^= keccak_rc(rnd) & m_iota;
-    const uint64_t cpy = dpp64<0x115>(a);  // row_shr:5
-    a = ((cpy & m_copy) | (a & ~m_copy)) & m_act;
+    // theta
+    const uint32_t tl = lo ^ dpp32<0x128>(lo), th = hi ^ dpp32<0x128>(hi);  // row_ror:8
+    const auto s1 = __builtin_amdgcn_permlane32_swap(tl, th, false, false);
+    const uint32_t u = s1[0] ^ s1[1];
+    const auto s2 = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const uint32_t w = s2[0] ^ s2[1];
+    const auto s3 = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+    const uint32_t cl = s3[0], ch = s3[1];
+    const uint32_t pl = dpp32<0x101>(cl), ph = dpp32<0x101>(ch);  // row_shl:1: C[x + 1]
+    lo = __builtin_amdgcn_bitop3_b32(lo, dpp32<0x111>(cl), __builtin_amdgcn_alignbit(pl, ph, 31), 0x96);
+    hi = __builtin_amdgcn_bitop3_b32(hi, dpp32<0x111>(ch), __builtin_amdgcn_alignbit(ph, pl, 31), 0x96);
+    // rho + pi
+    const uint32_t sl = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)lo);
+    const uint32_t sh = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)hi);
+    const uint32_t P = swp ? sl : sh, Q = swp ? sh : sl;
+    const uint32_t bh = __builtin_amdgcn_alignbit(P, Q, tsh), bl = __builtin_amdgcn_alignbit(Q, P, tsh);
+    // chi: b ^ (~b[x + 1] & b[x + 2]) (truth table 0xD2), then iota
+    // a ^ (rc & m) (0x78)
+    const uint64_t rc = keccak_rc(rnd);
+    lo = __builtin_amdgcn_bitop3_b32(bl, dpp32<0x101>(bl), dpp32<0x102>(bl), 0xD2);
+    hi = __builtin_amdgcn_bitop3_b32(bh, dpp32<0x101>(bh), dpp32<0x102>(bh), 0xD2);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)rc, m_iota, 0x78);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(rc >> 32), m_iota, 0x78);
+    const uint32_t cpl = dpp32<0x115>(lo), cph = dpp32<0x115>(hi);  // row_shr:5
+    lo = active ? (copy ? cpl : lo) : 0u;
+    hi = active ? (copy ? cph : hi) : 0u;
   }
   const int i = lane < 25 ? lane : 0;
-  return shfl64(a, 8 * (i / 5) + i % 5);
+  return shfl64(((uint64_t)hi << 32) | lo, 8 * (i / 5) + i % 5 + 1);
 }
 
 // Sponge with the state spread over a wave (one word per lane); items are
