@@ -94,11 +94,12 @@ class BatchProver:
         ts = (C.c_void_p * self.count)(*[t.h for t in transcripts])
         polys, chals, fins = self.buffers()
         self.prove_raw(ts, polys, chals, fins)
+        pw, cw, fw = polys.tolist(), chals.tolist(), fins.tolist()
         out = []
         for i in range(self.count):
             f, v = self.first[i], self.v[i]
-            rp = [[int(x) for x in polys[(f + r) * K:(f + r + 1) * K]] for r in range(v)]
-            out.append(SumcheckProof(rp, [int(c) for c in chals[f:f + v]], int(fins[i])))
+            rp = [pw[(f + r) * K:(f + r + 1) * K] for r in range(v)]
+            out.append(SumcheckProof(rp, cw[f:f + v], fw[i]))
         return out
 
     def close(self):

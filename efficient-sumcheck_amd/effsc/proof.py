@@ -15,6 +15,24 @@ class SumcheckProof:
     final_value: Any = 0
 
 
+def proof_from_words(polys, chals, fin, rounds, nw, E):
+    """A proof from canonical word arrays (numpy uint64): `rounds` round
+    polynomials of `nw` values, `rounds` challenges, one final value; an
+    element is an int (E = 1) or an E-tuple.  One tolist() per array:
+    per-element slicing and int() cost as much host time as a small proof on
+    the device."""
+    vals = _elements(polys[:rounds * nw * E].tolist(), E)
+    rp = [vals[r * nw:(r + 1) * nw] for r in range(rounds)]
+    return SumcheckProof(rp, _elements(chals[:rounds * E].tolist(), E), _elements(fin[:E].tolist(), E)[0])
+
+
+def _elements(words, E):
+    if E == 1:
+        return words
+    it = iter(words)
+    return list(zip(*([it] * E)))
+
+
 @dataclass
 class SumcheckResult:
     """Verifier output (src/verifier.rs:38-46)."""

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _native as N
 from .field import CANONICAL, MONT64, from_mont, to_mont
-from .proof import SumcheckProof
+from .proof import SumcheckProof, proof_from_words
 from .provers import NativeProver
 from .transcript import NativeTranscript
 
@@ -85,7 +85,4 @@ def _native_sumcheck(prover, num_rounds, transcript, hook):
                                        chals.ctypes.data_as(N.u64p), fin.ctypes.data_as(N.u64p)))
     if errors:
         raise errors[0]
-    el = prover._elem
-    rp = [[el(polys[(r * nw + i) * E:(r * nw + i + 1) * E]) for i in range(nw)] for r in range(num_rounds)]
-    ch = [el(chals[r * E:(r + 1) * E]) for r in range(num_rounds)]
-    return SumcheckProof(rp, ch, el(fin))
+    return proof_from_words(polys, chals, fin, num_rounds, nw, E)

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .proof import SumcheckProof
+from .proof import SumcheckProof, proof_from_words
 
 MULTILINEAR = 1
 INNER_PRODUCT = 2
@@ -81,12 +81,7 @@ class Server:
         times = (C.c_double * 3)()
         N.check(N.lib().effsc_hip_server_wait(self.h, rid, polys.ctypes.data_as(N.u64p), chals.ctypes.data_as(N.u64p),
                                               fin.ctypes.data_as(N.u64p), times))
-
-        def el(words):
-            return int(words[0]) if ext == 1 else tuple(int(x) for x in words)
-        rp = [[el(polys[(r * K + i) * ext:(r * K + i + 1) * ext]) for i in range(K)] for r in range(v)]
-        ch = [el(chals[r * ext:(r + 1) * ext]) for r in range(v)]
-        return Served(SumcheckProof(rp, ch, el(fin)), times[0], times[1], times[2])
+        return Served(proof_from_words(polys, chals, fin, v, K, ext), times[0], times[1], times[2])
 
     def now(self):
         t = C.c_double()
