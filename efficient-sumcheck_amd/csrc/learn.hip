@@ -46,32 +46,34 @@ __device__ __forceinline__ void stage_params(const float* p, float* sp, int coun
   __syncthreads();
 }
 
-// Forward of one sample; keeps the activations for the backward pass.
-__device__ __forceinline__ float forward1(const Shape S, const float* sp, const float* x, float* h1, float* h2) {
-  for (int i = 0; i < S.H; i++) {
-    float a = sp[S.b1() + i];
-    for (int k = 0; k < S.D; k++) a = fmaf(sp[S.w1() + i * S.D + k], x[k], a);
-    h1[i] = tanhf(a);
-  }
-  for (int i = 0; i < S.H; i++) {
-    float a = sp[S.b2() + i];
-    for (int k = 0; k < S.H; k++) a = fmaf(sp[S.w2() + i * S.H + k], h1[k], a);
-    h2[i] = tanhf(a);
-  }
-  float y = sp[S.b3()];
-  for (int k = 0; k < S.H; k++) y = fmaf(sp[S.w3() + k], h2[k], y);
-  return y;
-}
+// Forward of one sample per thread.  The input and the first hidden layer sit
+// in LDS after the parameters, column k of thread t at k * blockDim.x + t (a
+// private array indexed by a run-time width would live in scratch); the second
+// layer is folded into the output as it is computed, in the same order.
+constexpr int kFwdBlock = 64;
 
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(Shape S, const float* params, const float* X, size_t n,
+__global__ __launch_bounds__(kFwdBlock) void k_mlp_forward(Shape S, const float* params, const float* X, size_t n,
                                                            float* Y) {
   extern __shared__ float sp[];
   stage_params(params, sp, S.count());
   const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  float x[kMaxIn], h1[kMaxHidden], h2[kMaxHidden];
-  for (int k = 0; k < S.D; k++) x[k] = X[s * S.D + k];
-  Y[s] = forward1(S, sp, x, h1, h2);
+  const int t = threadIdx.x, B = blockDim.x;
+  float* x = sp + S.count();
+  float* h1 = x + S.D * B;
+  for (int k = 0; k < S.D; k++) x[k * B + t] = X[s * S.D + k];
+  for (int i = 0; i < S.H; i++) {
+    float a = sp[S.b1() + i];
+    for (int k = 0; k < S.D; k++) a = fmaf(sp[S.w1() + i * S.D + k], x[k * B + t], a);
+    h1[i * B + t] = tanhf(a);
+  }
+  float y = sp[S.b3()];
+  for (int i = 0; i < S.H; i++) {
+    float a = sp[S.b2() + i];
+    for (int k = 0; k < S.H; k++) a = fmaf(sp[S.w2() + i * S.H + k], h1[k * B + t], a);
+    y = fmaf(sp[S.w3() + i], tanhf(a), y);
+  }
+  Y[s] = y;
 }
 
 // Row s of G: gradient of (y_s - t_s)^2 / n; E[s] = (y_s - t_s)^2 / n.
@@ -310,9 +312,10 @@ int effsc_hip_mlp_forward(effsc_hip_mlp* m, const float* x, size_t n, float* y) 
   if (!n) return EFFSC_HIP_OK;
   HIPCHK(hipSetDevice(m->dev));
   TRY(upload(m, x, nullptr, n));
-  const unsigned g = (unsigned)((n + kMlpBlock - 1) / kMlpBlock);
-  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(kMlpBlock), (size_t)m->S.count() * 4, m->stream, m->S, m->d_params,
-                     m->d_x, n, m->d_y);
+  const unsigned g = (unsigned)((n + kFwdBlock - 1) / kFwdBlock);
+  const size_t lds = ((size_t)m->S.count() + (size_t)(m->S.D + m->S.H) * kFwdBlock) * 4;  // at most 41 KiB
+  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(kFwdBlock), lds, m->stream, m->S, m->d_params, m->d_x, n,
+                     m->d_y);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(y, m->d_y, n * 4, hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));

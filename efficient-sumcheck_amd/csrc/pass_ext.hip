@@ -7,10 +7,14 @@
 // each round-(s+1) value, and the next pass folds the stored level with both
 // challenges on the fly, materialising every other level only.  Here the
 // table elements, challenges and sums are extension elements (inputs may be
-// base-field words, lifted).  The raw sums are kept reduced (one F product
-// per raw sum and unit); k_reduce_fs2_ext sums the workgroups' values and runs
-// the two transcript steps.  All outputs are the same field elements as one
-// round per launch.
+// base-field words, lifted).  The two pending folds of a level-L element are
+// one double fold with prepared challenges (fold2x: one reduction per
+// coefficient); the raw sums are kept reduced (one F product per raw sum and
+// unit); k_reduce_fs2_ext sums the workgroups' values and runs the two
+// transcript steps.  All outputs are the same field elements as one round per
+// launch.
+#include <utility>
+
 #include "prover.hpp"
 
 using namespace effsc_hip;
@@ -23,64 +27,137 @@ namespace effsc_hip {
 namespace pv {
 namespace {
 
-// Level-L element e from the level-M table (M = 2^P L) with P pending folds
-// c1 (the older) and c2.
-template <int P, class FI, class F>
-__device__ __forceinline__ F chain(const View<FI>& t, size_t e, size_t L, const F& c1, const F& c2, uint64_t w) {
-  if constexpr (P == 0) {
-    return gl::lift_to<F>(t.ld(e));
-  } else {
-    const FI x0 = t.ld(e), x1 = t.ld(e + L), x2 = t.ld(e + 2 * L), x3 = t.ld(e + 3 * L);
-    const F a = fold1<F>(x0, x2, c1, w), b = fold1<F>(x1, x3, c1, w);
-    return fold1<F>(a, b, c2, w);
-  }
+// Compile-time loop: fn(std::integral_constant<int, i>) for i < N, expanded in
+// the source.  With #pragma unroll the compiler declined the pass's unit loops
+// and kept the per-unit arrays, indexed by the loop counter, in scratch.
+template <class Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  static_for_impl(fn, std::make_integer_sequence<int, N>{});
 }
 
-// Two consecutive level-L elements e, e + 1 (e even, 16-byte aligned planes).
-template <int P, class FI, class F>
-__device__ __forceinline__ void chain2(const View<FI>& t, size_t e, size_t L, const F& c1, const F& c2, uint64_t w,
-                                       F& o0, F& o1) {
-  if constexpr (P == 0) {
-    FI x0, x1;
-    t.ld2(e, x0, x1);
-    o0 = gl::lift_to<F>(x0);
-    o1 = gl::lift_to<F>(x1);
-  } else {
-    FI x0[2], x1[2], x2[2], x3[2];
-    t.ld2(e, x0[0], x0[1]);
-    t.ld2(e + L, x1[0], x1[1]);
-    t.ld2(e + 2 * L, x2[0], x2[1]);
-    t.ld2(e + 3 * L, x3[0], x3[1]);
-    o0 = fold1<F>(fold1<F>(x0[0], x2[0], c1, w), fold1<F>(x1[0], x3[0], c1, w), c2, w);
-    o1 = fold1<F>(fold1<F>(x0[1], x2[1], c1, w), fold1<F>(x1[1], x3[1], c1, w), c2, w);
-  }
+template <class F>
+struct PreOf;
+template <>
+struct PreOf<Fp2> {
+  using T = gl::Fp2Pre;
+};
+template <>
+struct PreOf<Fp3> {
+  using T = gl::Fp3Pre;
+};
+
+// The two pending challenges of a pass (c1 the older, c2) and c1 c2, prepared
+// as multipliers.
+template <class F>
+struct Fold2C {
+  typename PreOf<F>::T c1, c2, c12;
+};
+template <class F>
+__device__ __forceinline__ Fold2C<F> fold2_prep(const uint64_t* cptr, uint64_t w) {
+  const F c1 = gl::load_words<F>(cptr), c2 = gl::load_words<F>(cptr + F::D);
+  return {gl::prepare(c1, w), gl::prepare(c2, w), gl::prepare(gl::mul(c1, c2, w), w)};
 }
 
-// The multilinear pass with two units per thread step and 16-byte loads and
-// stores (q even, strides even).
-template <class FI, class F, int P>
-__global__ __launch_bounds__(kBlock) void k_pass_ext_ml2(View<FI> src, View<F> dst, const uint64_t* cptr, size_t q,
-                                                         uint64_t w, Reduce r) {
-  F c1 = gl::zero<F>(), c2 = gl::zero<F>();
-  if constexpr (P > 0) {
-    c1 = gl::load_words<F>(cptr);
-    c2 = gl::load_words<F>(cptr + F::D);
+// The level-L element from the level-4L elements x0..x3 at e, e + L, e + 2L,
+// e + 3L folded with c1 (pairs e, e + 2L) and then c2 (pairs e, e + L):
+//   x0 + c1 (x2 - x0) + c2 (x1 - x0) + c1 c2 (x3 - x1 - x2 + x0),
+// each coefficient one unreduced column sum (gl::AccW) reduced once: 3
+// products by prepared multipliers and D reductions, against three separate
+// folds of 3 D reductions.
+__device__ __forceinline__ Fp2 fold2x(Fp2 x0, Fp2 x1, Fp2 x2, Fp2 x3, const Fold2C<Fp2>& c) {
+  const Fp2 d20 = gl::sub(x2, x0), d10 = gl::sub(x1, x0), dd = gl::sub(gl::sub(x3, x1), d20);
+  gl::AccW a[2];
+  gl::zero(a[0]);
+  gl::zero(a[1]);
+  gl::addw(a[0], x0.c0);
+  gl::addw(a[1], x0.c1);
+  gl::mac_pre(a, d20, c.c1);
+  gl::mac_pre(a, d10, c.c2);
+  gl::mac_pre(a, dd, c.c12);
+  return {gl::reduce(a[0]), gl::reduce(a[1])};
+}
+__device__ __forceinline__ Fp3 fold2x(Fp3 x0, Fp3 x1, Fp3 x2, Fp3 x3, const Fold2C<Fp3>& c) {
+  const Fp3 d20 = gl::sub(x2, x0), d10 = gl::sub(x1, x0), dd = gl::sub(gl::sub(x3, x1), d20);
+  gl::AccW a[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) gl::zero(a[k]);
+  gl::addw(a[0], x0.c0);
+  gl::addw(a[1], x0.c1);
+  gl::addw(a[2], x0.c2);
+  gl::mac_pre(a, d20, c.c1);
+  gl::mac_pre(a, d10, c.c2);
+  gl::mac_pre(a, dd, c.c12);
+  return {gl::reduce(a[0]), gl::reduce(a[1]), gl::reduce(a[2])};
+}
+// Base-field inputs: the differences are base elements, so coefficient k is
+// a column sum of three base products with coefficient k of c1, c2, c1 c2.
+template <class F>
+__device__ __forceinline__ F fold2x(Fp x0, Fp x1, Fp x2, Fp x3, const Fold2C<F>& c) {
+  const uint64_t d20 = gl::sub(x2.v, x0.v), d10 = gl::sub(x1.v, x0.v), dd = gl::sub(gl::sub(x3.v, x1.v), d20);
+  uint64_t cw1[3], cw2[3], cw12[3];
+  cw1[0] = c.c1.b0, cw1[1] = c.c1.b1, cw2[0] = c.c2.b0, cw2[1] = c.c2.b1, cw12[0] = c.c12.b0, cw12[1] = c.c12.b1;
+  if constexpr (F::D == 3) cw1[2] = c.c1.b2, cw2[2] = c.c2.b2, cw12[2] = c.c12.b2;
+  uint64_t o[3];
+#pragma unroll
+  for (int k = 0; k < F::D; k++) {
+    gl::AccW a;
+    gl::zero(a);
+    if (k == 0) gl::addw(a, x0.v);
+    gl::macw(a, d20, cw1[k]);
+    gl::macw(a, d10, cw2[k]);
+    gl::macw(a, dd, cw12[k]);
+    o[k] = gl::reduce(a);
   }
-  const size_t L = 4 * q;
-  F s[3] = {gl::zero<F>(), gl::zero<F>(), gl::zero<F>()};
-  for (size_t u = gtid() * 2; u < q; u += (size_t)gridDim.x * blockDim.x * 2) {
-    F E0[4], E1[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) chain2<P>(src, u + j * q, L, c1, c2, w, E0[j], E1[j]);
-    if constexpr (P > 0) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) dst.st2(u + j * q, E0[j], E1[j]);
+  return gl::load_words<F>(o);
+}
+
+template <bool NT, class FI>
+__device__ __forceinline__ void ld2x(const View<FI>& t, size_t i, FI& a, FI& b) {
+  if constexpr (NT)
+    t.ld2_nt(i, a, b);
+  else
+    t.ld2(i, a, b);
+}
+
+// VEC consecutive level-L elements from e (e even and 16-byte aligned planes
+// when VEC = 2), P = 0 (the stored level itself, lifted) or P = 2 pending folds.
+template <int P, int VEC, bool NT, class FI, class F>
+__device__ __forceinline__ void load_level(const View<FI>& t, size_t e, size_t L, const Fold2C<F>& c, F (&o)[VEC]) {
+  if constexpr (VEC == 2) {
+    if constexpr (P == 0) {
+      FI a, b;
+      ld2x<NT>(t, e, a, b);
+      o[0] = gl::lift_to<F>(a);
+      o[1] = gl::lift_to<F>(b);
+    } else {
+      FI x0[2], x1[2], x2[2], x3[2];
+      ld2x<NT>(t, e, x0[0], x0[1]);
+      ld2x<NT>(t, e + L, x1[0], x1[1]);
+      ld2x<NT>(t, e + 2 * L, x2[0], x2[1]);
+      ld2x<NT>(t, e + 3 * L, x3[0], x3[1]);
+      o[0] = fold2x(x0[0], x1[0], x2[0], x3[0], c);
+      o[1] = fold2x(x0[1], x1[1], x2[1], x3[1], c);
     }
-#pragma unroll
-    for (int j = 0; j < 3; j++) s[j] = gl::add(s[j], gl::add(E0[j], E1[j]));
+  } else {
+    if constexpr (P == 0)
+      o[0] = gl::lift_to<F>(t.ld(e));
+    else
+      o[0] = fold2x(t.ld(e), t.ld(e + L), t.ld(e + 2 * L), t.ld(e + 3 * L), c);
   }
-  F v[3] = {gl::add(s[0], s[1]), s[0], gl::sub(s[2], s[0])};
-  finish<F, 3>(v, r);
+}
+
+template <int VEC, bool NT, class F>
+__device__ __forceinline__ void store_level(const View<F>& t, size_t e, const F (&x)[VEC]) {
+  if constexpr (VEC == 2 && NT)
+    t.st2_nt(e, x[0], x[1]);
+  else if constexpr (VEC == 2)
+    t.st2(e, x[0], x[1]);
+  else
+    t.st(e, x[0]);
 }
 
 // Raw sums -> pass values (pass2.hpp layout: K values of round s, then
@@ -103,60 +180,64 @@ __device__ __forceinline__ void values_from_raw(const F* s, F* v) {
   }
 }
 
+// Raw sums of one unit (pass2.hpp: pass_accumulate); A from table 0, B from
+// table 1, element v of each VEC group.
+template <class F, int K, int VEC, int V>
+__device__ __forceinline__ void accumulate_unit(const F (&E)[K][4][VEC], F* s, uint64_t w) {
+  if constexpr (K == 1) {
+    s[0] = gl::add(s[0], E[0][0][V]);
+    s[1] = gl::add(s[1], E[0][1][V]);
+    s[2] = gl::add(s[2], E[0][2][V]);
+  } else {
+    const F A0 = E[0][0][V], A1 = E[0][1][V], A2 = E[0][2][V], A3 = E[0][3][V];
+    const F B0 = E[1][0][V], B1 = E[1][1][V], B2 = E[1][2][V], B3 = E[1][3][V];
+    const F dA0 = gl::sub(A2, A0), dA1 = gl::sub(A3, A1), dB0 = gl::sub(B2, B0), dB1 = gl::sub(B3, B1);
+    s[0] = gl::add(s[0], gl::mul(A0, B0, w));
+    s[1] = gl::add(s[1], gl::mul(A1, B1, w));
+    s[2] = gl::add(s[2], gl::mul(dA0, dB0, w));
+    s[3] = gl::add(s[3], gl::mul(dA1, dB1, w));
+    s[4] = gl::add(s[4], gl::mul(A2, B2, w));
+    s[5] = gl::add(s[5], gl::mul(gl::sub(A1, A0), gl::sub(B1, B0), w));
+    s[6] = gl::add(s[6], gl::mul(gl::sub(A3, A2), gl::sub(B3, B2), w));
+    s[7] = gl::add(s[7], gl::mul(gl::sub(dA1, dA0), gl::sub(dB1, dB0), w));
+  }
+}
+
 // One pass over K tables: P pending folds from the level-M tables `src`
 // (field FI) to level L = 4 q, written to `dst` (field F) when P > 0.  Unit u
-// owns E_j = T_L[u + j q], j < 4; round s pairs (E0, E2) and (E1, E3), round
-// s + 1 pairs (E0 + x (E2 - E0), E1 + x (E3 - E1)).  Raw sums per unit:
+// owns E_j = T_L[u + j q], j < 4 (VEC consecutive units per thread step);
+// round s pairs (E0, E2) and (E1, E3), round s + 1 pairs
+// (E0 + x (E2 - E0), E1 + x (E3 - E1)).  Raw sums per unit:
 // K = 1: E0, E1, E2; K = 2 (A from table 0, B from table 1):
 //   A0 B0, A1 B1, dA0 dB0, dA1 dB1, A2 B2, eA eB, (A3 - A2)(B3 - B2), fA fB
 // with dA0 = A2 - A0, dA1 = A3 - A1, eA = A1 - A0, fA = dA1 - dA0.
-// Challenges: 2 D words at cptr (c1 then c2).
-// At most 256 VGPRs (two waves per SIMD): unconstrained, the compiler issued
-// every level-M load of a unit at once and ran at one wave per SIMD.
-template <class FI, class F, int K, int P>
+// Challenges: 2 D words at cptr (c1 then c2).  One region's loads and folds
+// at a time (sched_barrier), at most 256 VGPRs (two waves per SIMD); NT
+// streams the source and the written level past the caches.
+template <class FI, class F, int K, int P, int VEC, bool NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_pass_ext(
     Views<FI, K> src, Views<F, K> dst, const uint64_t* cptr, size_t q, uint64_t w, Reduce r) {
   constexpr int NR = pass_raw<K>(), NV = pass_values<K>();
-  F c1 = gl::zero<F>(), c2 = gl::zero<F>();
-  if constexpr (P > 0) {
-    c1 = gl::load_words<F>(cptr);
-    c2 = gl::load_words<F>(cptr + F::D);
-  }
+  Fold2C<F> cc{};
+  if constexpr (P > 0) cc = fold2_prep<F>(cptr, w);
   const size_t L = 4 * q;
   F s[NR];
-#pragma unroll
-  for (int i = 0; i < NR; i++) s[i] = gl::zero<F>();
-  for (size_t u = gtid(); u < q; u += (size_t)gridDim.x * blockDim.x) {
-    F E[K][4];
-#pragma unroll
-    for (int i = 0; i < K; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) E[i][j] = chain<P>(src.t[i], u + j * q, L, c1, c2, w);
-    if constexpr (P > 0) {
-#pragma unroll
-      for (int i = 0; i < K; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) dst.t[i].st(u + j * q, E[i][j]);
-    }
-    if constexpr (K == 1) {
-      s[0] = gl::add(s[0], E[0][0]);
-      s[1] = gl::add(s[1], E[0][1]);
-      s[2] = gl::add(s[2], E[0][2]);
-    } else {
-      const F* A = E[0];
-      const F* B = E[1];
-      const F dA0 = gl::sub(A[2], A[0]), dA1 = gl::sub(A[3], A[1]), dB0 = gl::sub(B[2], B[0]),
-              dB1 = gl::sub(B[3], B[1]);
-      const F eA = gl::sub(A[1], A[0]), eB = gl::sub(B[1], B[0]);
-      s[0] = gl::add(s[0], gl::mul(A[0], B[0], w));
-      s[1] = gl::add(s[1], gl::mul(A[1], B[1], w));
-      s[2] = gl::add(s[2], gl::mul(dA0, dB0, w));
-      s[3] = gl::add(s[3], gl::mul(dA1, dB1, w));
-      s[4] = gl::add(s[4], gl::mul(A[2], B[2], w));
-      s[5] = gl::add(s[5], gl::mul(eA, eB, w));
-      s[6] = gl::add(s[6], gl::mul(gl::sub(A[3], A[2]), gl::sub(B[3], B[2]), w));
-      s[7] = gl::add(s[7], gl::mul(gl::sub(dA1, dA0), gl::sub(dB1, dB0), w));
-    }
+  static_for<NR>([&](auto i) { s[i] = gl::zero<F>(); });
+  for (size_t u0 = gtid() * VEC; u0 < q; u0 += (size_t)gridDim.x * blockDim.x * VEC) {
+    // the unit index through an empty asm: otherwise loop strength reduction
+    // keeps a 64-bit pointer per region and plane live across the loop
+    size_t u = u0;
+    asm volatile("" : "+v"(u));
+    F E[K][4][VEC];
+    static_for<K>([&](auto i) {
+      static_for<4>([&](auto j) {
+        load_level<P, VEC, NT>(src.t[i], u + j * q, L, cc, E[i][j]);
+        if constexpr (P > 0) __builtin_amdgcn_sched_barrier(0);
+      });
+    });
+    if constexpr (P > 0)
+      static_for<K>([&](auto i) { static_for<4>([&](auto j) { store_level<VEC, NT>(dst.t[i], u + j * q, E[i][j]); }); });
+    static_for<VEC>([&](auto v) { accumulate_unit<F, K, VEC, decltype(v)::value>(E, s, w); });
   }
   F v[NV];
   values_from_raw<F, K>(s, v);
@@ -473,18 +554,24 @@ template <class FI, class F, int K, int P>
 int launch_pass_ext_t(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
   const Views<FI, K> in = views<FI, K>(src_planes(h), src_stride(h));
   const Views<F, K> out = views<F, K>(h->work, h->work_stride);
-  const bool vec = K == 1 && q % 2 == 0 && src_stride(h) % 2 == 0 && h->work_stride % 2 == 0;
+  const bool vec = q % 2 == 0 && src_stride(h) % 2 == 0 && h->work_stride % 2 == 0;
+  const size_t L = 4 * q;
+  // fold passes over planes of 128 MiB and more: nontemporal loads and stores
+  const bool nt = vec && P > 0 && 8.0 * (double)(L << P) >= kNtMinBytes;
   const unsigned g = blocks_for(h, vec ? q / 2 : q);
   TRY(prof_begin(h));
-  if (vec)
-    hipLaunchKernelGGL((k_pass_ext_ml2<FI, F, P>), dim3(g), dim3(kBlock), 0, h->stream, in.t[0], out.t[0], cptr, q,
+  if (nt)
+    hipLaunchKernelGGL((k_pass_ext<FI, F, K, P, 2, true>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q, h->w,
+                       h->red);
+  else if (vec)
+    hipLaunchKernelGGL((k_pass_ext<FI, F, K, P, 2, false>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q,
                        h->w, h->red);
   else
-    hipLaunchKernelGGL((k_pass_ext<FI, F, K, P>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q, h->w, h->red);
+    hipLaunchKernelGGL((k_pass_ext<FI, F, K, P, 1, false>), dim3(g), dim3(kBlock), 0, h->stream, in, out, cptr, q,
+                       h->w, h->red);
   HIPCHK(hipGetLastError());
-  const double L = 4.0 * (double)q;
   // read the stored level (2^P L elements), write level L when P > 0
-  const double bytes = 8.0 * K * ((double)(1 << P) * L * FI::D + (P > 0 ? L * F::D : 0.0));
+  const double bytes = 8.0 * K * ((double)(1 << P) * (double)L * FI::D + (P > 0 ? (double)L * F::D : 0.0));
   const double muls = (double)q * (4.0 * K * (P == 0 ? 0.0 : 3.0 * mul_cost<F>()) + (K == 2 ? 8.0 * mul_cost<F>() : 0.0));
   TRY(prof_end(h, 3, bytes, muls));
   h->last_grid = g;
@@ -494,10 +581,11 @@ int launch_pass_ext_t(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
 }  // namespace
 
 // Used over F_p^2, and for the multilinear prover over F_p^3 (measured with
-// the SHAKE transcript, v = 26 over F_p^2: inner product 2.41 -> 2.12 ms,
-// multilinear 1.06 -> 0.88 ms; v = 24 over F_p^3: multilinear 0.669 -> 0.644
-// ms).  The F_p^3 inner-product pass (8 F_p^3 products per unit at two waves
-// per SIMD) measured 1.349 against 1.307 ms for one round per launch, so it
+// the SHAKE transcript, v = 26 over F_p^2: inner product 2.41 -> 2.12 ms with
+// three separate folds, 1.66 ms with fold2x, multilinear 1.06 -> 0.88 -> 0.73
+// ms; v = 24 over F_p^3: multilinear 0.669 -> 0.644 -> 0.471 ms).  The F_p^3
+// inner-product pass (8 F_p^3 products per unit at two waves per SIMD)
+// measured 1.071 against 1.078 ms for one round per launch: no gain, so it
 // runs only with EFFSC_HIP_EXT3_PASS=1.
 bool ext_twostep(const effsc_hip_prover* h) {
   const bool field_ok = h->ext == 2 || (h->ext == 3 && (h->K == 1 || std::getenv("EFFSC_HIP_EXT3_PASS")));
