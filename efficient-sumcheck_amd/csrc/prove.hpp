@@ -169,11 +169,13 @@ struct LaneFs {
     for (int i = 0; i < 3; i++)
       if (i < ext) absorb_word(w[i]);
   }
+  // Writes out[0 .. ext): callers read no further.
   __device__ __forceinline__ void challenge(int ext, uint64_t* out) {
     if (kind == 1) {
       rng += 0x9E3779B97F4A7C15ull;
       out[0] = gl::canon(mix64(rng));
-      out[1] = out[2] = 0;
+      if (ext > 1) out[1] = 0;
+      if (ext > 2) out[2] = 0;
       return;
     }
     absorb_word(2);
@@ -185,8 +187,8 @@ struct LaneFs {
     uint64_t w0 = shfl64(c, 0), w1 = shfl64(c, 1), w2 = shfl64(c, 2), w3 = shfl64(c, 3), w4 = shfl64(c, 4),
              w5 = shfl64(c, 5);
     out[0] = uniform64(gl::reduce128(w0, w1));
-    out[1] = ext > 1 ? uniform64(gl::reduce128(w2, w3)) : 0;
-    out[2] = ext > 2 ? uniform64(gl::reduce128(w4, w5)) : 0;
+    if (ext > 1) out[1] = uniform64(gl::reduce128(w2, w3));
+    if (ext > 2) out[2] = uniform64(gl::reduce128(w4, w5));
   }
 };
 
@@ -346,10 +348,11 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
 
 constexpr int kTailBlock = 1024;
 // Extension-field products of two or more tables need more than the 128
-// VGPRs a 1024-thread workgroup allows (they spilled there): 256 threads.
+// VGPRs a 1024-thread workgroup allows (they spilled there): 256 threads; a
+// single F_p^3 table 512 (about 140 VGPRs).
 template <class F, int K>
 constexpr int tail_block() {
-  return (F::D >= 2 && K >= 2) ? 256 : kTailBlock;
+  return (F::D >= 2 && K >= 2) ? 256 : (F::D == 3 ? 512 : kTailBlock);
 }
 
 // All remaining rounds in one workgroup.  On entry the K tables have pow2
