@@ -178,6 +178,22 @@ def cpu_baseline(workload, v_sample, threads, samples=CPU_SAMPLES):
                 "sample": f"{protocol}; best reference CPU path: {best}, scalar Montgomery arithmetic",
                 "best_path": best, "paths": paths, "cpu": cpu_info()}
 
+    if workload == "ml":
+        x = O.uniform(n, SEED_ML)
+        paths = {}
+        for name, th, desc in (
+                ("multilinear_prover_parallel", threads, "MultilinearProver with the parallel feature "
+                 "(src/provers/multilinear.rs:38-58 over src/multilinear_sumcheck.rs:84-254): rayon-split "
+                 "evaluate, fused fold + evaluate, fold"),
+                ("multilinear_prover_serial", 1, "the same prover on one thread")):
+            st = _sampled(lambda: O.bench_ml(x.copy(), v_sample, th)[0], samples)
+            paths[name] = dict(st, evals_per_s=n / st["median_s"], threads=th, path=desc)
+            log(f"bench: CPU {name}: {st['median_s'] * 1e3:.1f} ms median at v={v_sample}, {th} threads")
+        best = max(paths, key=lambda k: paths[k]["evals_per_s"])
+        return {"value": paths[best]["evals_per_s"], "unit": "evals/s", "cores": paths[best]["threads"],
+                "kind": "port", "sample": f"{protocol}; best reference CPU path: {best}, scalar Montgomery arithmetic",
+                "best_path": best, "paths": paths, "cpu": cpu_info()}
+
     def make():
         if workload == "lsb":
             return O.Prover.inner_product_lsb(O.uniform(n, SEED_A), O.uniform(n, SEED_B))

@@ -1178,6 +1178,121 @@ double or_bench_ip_ex(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int
     return el_s;
 }
 
+/* MultilinearProver over the base field as the reference runs it with the
+ * parallel feature (src/provers/multilinear.rs:38-58 over
+ * src/multilinear_sumcheck.rs:84-254): round 0 is
+ * compute_sumcheck_polynomial (the pairs split by rayon joins), later rounds
+ * fused_fold_and_compute_polynomial (one sweep over the quadruples
+ * (k, k + L/4, k + L/2, k + 3L/4), split down to leaves of 32 KiB), the last
+ * challenge folds with fold; each round sends [s0].  OpenMP static blocks of
+ * 1024 stand in for the joins (field addition is exact, so the split does
+ * not change a bit).  Non-power-of-two tables take fold + evaluate. */
+static void bench_ml_eval(const uint64_t *v, size_t n, int threads, uint64_t *s0o, uint64_t *s1o) {
+    if (n == 0) {
+        *s0o = *s1o = 0;
+        return;
+    }
+    if (n == 1) {
+        *s0o = v[0];
+        *s1o = 0;
+        return;
+    }
+    size_t half = next_pow2(n) >> 1, paired = n - half;
+    uint64_t s0 = 0, s1 = 0;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) if (threads > 1 && paired > 1024)
+#endif
+    {
+        uint64_t a0 = 0, a1 = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 1024)
+#endif
+        for (size_t i = 0; i < paired; i++) {
+            a0 = m_add(a0, v[i]);
+            a1 = m_add(a1, v[half + i]);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            s0 = m_add(s0, a0);
+            s1 = m_add(s1, a1);
+        }
+    }
+    for (size_t i = paired; i < half; i++) s0 = m_add(s0, v[i]);
+    *s0o = s0;
+    *s1o = s1;
+}
+static void bench_ml_fused(uint64_t *v, size_t *len, uint64_t w, int threads, uint64_t *s0o, uint64_t *s1o) {
+    size_t l = *len;
+    if (l < 4 || (l & (l - 1))) {
+        bench_fold(v, len, w, threads);
+        bench_ml_eval(v, *len, threads, s0o, s1o);
+        return;
+    }
+    size_t q = l / 4;
+    uint64_t s0 = 0, s1 = 0;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) if (threads > 1 && q > 1024)
+#endif
+    {
+        uint64_t a0 = 0, a1 = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(static, 1024)
+#endif
+        for (size_t k = 0; k < q; k++) {
+            uint64_t x0 = v[k], x1 = v[k + q], x2 = v[k + 2 * q], x3 = v[k + 3 * q];
+            uint64_t lo = m_add(x0, m_mul(m_sub(x2, x0), w)), hi = m_add(x1, m_mul(m_sub(x3, x1), w));
+            v[k] = lo;
+            v[k + q] = hi;
+            a0 = m_add(a0, lo);
+            a1 = m_add(a1, hi);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            s0 = m_add(s0, a0);
+            s1 = m_add(s1, a1);
+        }
+    }
+    *len = l / 2;
+    *s0o = s0;
+    *s1o = s1;
+}
+
+double or_bench_ml_ex(uint64_t *v, size_t n, size_t num_rounds, int threads, const uint8_t *domsep,
+                      size_t domsep_len, uint64_t *round_polys, uint64_t *challenges, uint64_t *fin) {
+    for (size_t i = 0; i < n; i++) v[i] = m_to(v[i]);
+    or_transcript *t = or_tr_new_shake(domsep, domsep_len);
+    double t0 = now_s();
+    size_t len = n;
+    uint64_t prev = 0;
+    int have = 0;
+    for (size_t r = 0; r < num_rounds; r++) {
+        uint64_t s0, s1;
+        if (have)
+            bench_ml_fused(v, &len, prev, threads, &s0, &s1);
+        else
+            bench_ml_eval(v, len, threads, &s0, &s1);
+        s0 = m_from(s0);
+        or_tr_send(t, 1, &s0);
+        round_polys[r] = s0;
+        uint64_t c;
+        or_tr_challenge(t, 1, &c);
+        challenges[r] = c;
+        prev = m_to(c);
+        have = 1;
+    }
+    if (have) bench_fold(v, &len, prev, threads);
+    *fin = len == 1 ? m_from(v[0]) : 0;
+    double el_s = now_s() - t0;
+    or_tr_free(t);
+    return el_s;
+}
+
 double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, uint64_t *round_polys,
                    uint64_t *challenges, uint64_t *fin) {
     static const char dom[] = "effsc-hip/bench/ip";

@@ -112,6 +112,10 @@ double or_bench_ip(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int th
 double or_bench_ip_ex(uint64_t *a, uint64_t *b, size_t n, size_t num_rounds, int threads, int fused,
                       const uint8_t *domsep, size_t domsep_len, uint64_t *round_polys, uint64_t *challenges,
                       uint64_t *fin);
+/* MultilinearProver with the reference's parallel fused kernels (base field,
+ * [s0] per round); v is consumed.  Returns the seconds of the proof. */
+double or_bench_ml_ex(uint64_t *v, size_t n, size_t num_rounds, int threads, const uint8_t *domsep,
+                      size_t domsep_len, uint64_t *round_polys, uint64_t *challenges, uint64_t *fin);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,7 @@ def lib():
             "or_bench_ip": (C.c_double, [u64p, u64p, sz, sz, C.c_int, u64p, u64p, u64p]),
             "or_bench_ip_ex": (C.c_double, [u64p, u64p, sz, sz, C.c_int, C.c_int, C.c_char_p, sz, u64p, u64p,
                                             u64p]),
+            "or_bench_ml_ex": (C.c_double, [u64p, sz, sz, C.c_int, C.c_char_p, sz, u64p, u64p, u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -297,4 +298,16 @@ def bench_ip(a, b, num_rounds, threads=1, fused=False, domsep=b"effsc-hip/bench/
     t = lib().or_bench_ip_ex(a.ctypes.data_as(u64p), b.ctypes.data_as(u64p), len(a), num_rounds, threads,
                              1 if fused else 0, domsep, len(domsep), polys.ctypes.data_as(u64p),
                              chals.ctypes.data_as(u64p), fin.ctypes.data_as(u64p))
+    return t, polys, chals, int(fin[0])
+
+
+def bench_ml(v, num_rounds, threads=1, domsep=b"effsc-hip/bench/ml"):
+    """Times a MultilinearProver proof with the reference's parallel fused
+    kernels (src/multilinear_sumcheck.rs:84-254); v is consumed."""
+    v = np.ascontiguousarray(v, dtype=np.uint64)
+    polys = np.zeros(max(num_rounds, 1), dtype=np.uint64)
+    chals = np.zeros(max(num_rounds, 1), dtype=np.uint64)
+    fin = np.zeros(1, dtype=np.uint64)
+    t = lib().or_bench_ml_ex(v.ctypes.data_as(u64p), len(v), num_rounds, threads, domsep, len(domsep),
+                             polys.ctypes.data_as(u64p), chals.ctypes.data_as(u64p), fin.ctypes.data_as(u64p))
     return t, polys, chals, int(fin[0])

@@ -409,6 +409,35 @@ def test_bench_ip_matches_brute_force_golden(golden, fused):
         assert [f] == case["final"]
 
 
+@pytest.mark.parametrize("threads", [1, 8])
+@pytest.mark.parametrize("v", [1, 2, 5, 12, 16])
+def test_bench_ml_equals_prover(v, threads):
+    # the timed CPU baseline of the multilinear workloads (the reference's
+    # parallel fused kernels, src/multilinear_sumcheck.rs:84-254) must produce
+    # the oracle prover's proof
+    n = 1 << v
+    x = O.uniform(n, 0x91 + v)
+    rp, ch, fin = O.Prover.multilinear(x).sumcheck(v, O.Transcript.shake(b"effsc-hip/bench/ml"))
+    _, polys, chals, f = O.bench_ml(x.copy(), v, threads)
+    assert [int(p) for p in polys[:v]] == [r[0][0] for r in rp]
+    assert [int(c) for c in chals[:v]] == [c[0] for c in ch]
+    assert [f] == fin
+
+
+def test_bench_ml_matches_brute_force_golden(golden):
+    seen = 0
+    for case in golden["proofs"]:
+        if case["kind"] != "ml" or case["ext"] != 1 or case["n"] != 1 << case["v"] or case["transcript"] != "shake":
+            continue
+        v = case["v"]
+        _, polys, chals, f = O.bench_ml(O.uniform(case["n"], case["seeds"][0]), v, 4, domsep=case["domsep"].encode())
+        assert [[[int(polys[r])]] for r in range(v)] == case["round_polys"]
+        assert [[int(c)] for c in chals[:v]] == case["challenges"]
+        assert [f] == case["final"]
+        seen += 1
+    assert seen == 4
+
+
 # ------------------------------------------------ general coefficient shapes
 def _pylist(words, ext):
     return [int(x) for x in words] if ext == 1 else [tuple(int(x) for x in words[i * ext:(i + 1) * ext])
