@@ -8,7 +8,7 @@
 // challenges on the fly, materialising every other level only.  Here the
 // table elements, challenges and sums are extension elements (inputs may be
 // base-field words, lifted).  The two pending folds of a level-L element are
-// one double fold with prepared challenges (fold2x: one reduction per
+// one double fold by prepared constants (gl::fold2: one reduction per
 // coefficient); the raw sums are kept reduced (one F product per raw sum and
 // unit); k_reduce_fs2_ext sums the workgroups' values and runs the two
 // transcript steps.  All outputs are the same field elements as one round per
@@ -39,80 +39,19 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
   static_for_impl(fn, std::make_integer_sequence<int, N>{});
 }
 
+// The two pending folds' constants (gl::prepare_fold2) from the challenges
+// at cptr (c1 then c2), kept in scalar registers.
 template <class F>
-struct PreOf;
-template <>
-struct PreOf<Fp2> {
-  using T = gl::Fp2Pre;
-};
-template <>
-struct PreOf<Fp3> {
-  using T = gl::Fp3Pre;
-};
-
-// The two pending challenges of a pass (c1 the older, c2) and c1 c2, prepared
-// as multipliers.
-template <class F>
-struct Fold2C {
-  typename PreOf<F>::T c1, c2, c12;
-};
-template <class F>
-__device__ __forceinline__ Fold2C<F> fold2_prep(const uint64_t* cptr, uint64_t w) {
-  const F c1 = gl::load_words<F>(cptr), c2 = gl::load_words<F>(cptr + F::D);
-  return {gl::prepare(c1, w), gl::prepare(c2, w), gl::prepare(gl::mul(c1, c2, w), w)};
-}
-
-// The level-L element from the level-4L elements x0..x3 at e, e + L, e + 2L,
-// e + 3L folded with c1 (pairs e, e + 2L) and then c2 (pairs e, e + L):
-//   x0 + c1 (x2 - x0) + c2 (x1 - x0) + c1 c2 (x3 - x1 - x2 + x0),
-// each coefficient one unreduced column sum (gl::AccW) reduced once: 3
-// products by prepared multipliers and D reductions, against three separate
-// folds of 3 D reductions.
-__device__ __forceinline__ Fp2 fold2x(Fp2 x0, Fp2 x1, Fp2 x2, Fp2 x3, const Fold2C<Fp2>& c) {
-  const Fp2 d20 = gl::sub(x2, x0), d10 = gl::sub(x1, x0), dd = gl::sub(gl::sub(x3, x1), d20);
-  gl::AccW a[2];
-  gl::zero(a[0]);
-  gl::zero(a[1]);
-  gl::addw(a[0], x0.c0);
-  gl::addw(a[1], x0.c1);
-  gl::mac_pre(a, d20, c.c1);
-  gl::mac_pre(a, d10, c.c2);
-  gl::mac_pre(a, dd, c.c12);
-  return {gl::reduce(a[0]), gl::reduce(a[1])};
-}
-__device__ __forceinline__ Fp3 fold2x(Fp3 x0, Fp3 x1, Fp3 x2, Fp3 x3, const Fold2C<Fp3>& c) {
-  const Fp3 d20 = gl::sub(x2, x0), d10 = gl::sub(x1, x0), dd = gl::sub(gl::sub(x3, x1), d20);
-  gl::AccW a[3];
+__device__ __forceinline__ gl::Fold2Pre<F> uniform_fold2(const uint64_t* cptr, uint64_t w) {
+  gl::Fold2Pre<F> p = gl::prepare_fold2(gl::load_words<F>(cptr), gl::load_words<F>(cptr + F::D), w);
+  constexpr int NW = sizeof(p.e[0]) / 8, NE = sizeof(p.e) / sizeof(p.e[0]);
 #pragma unroll
-  for (int k = 0; k < 3; k++) gl::zero(a[k]);
-  gl::addw(a[0], x0.c0);
-  gl::addw(a[1], x0.c1);
-  gl::addw(a[2], x0.c2);
-  gl::mac_pre(a, d20, c.c1);
-  gl::mac_pre(a, d10, c.c2);
-  gl::mac_pre(a, dd, c.c12);
-  return {gl::reduce(a[0]), gl::reduce(a[1]), gl::reduce(a[2])};
-}
-// Base-field inputs: the differences are base elements, so coefficient k is
-// a column sum of three base products with coefficient k of c1, c2, c1 c2.
-template <class F>
-__device__ __forceinline__ F fold2x(Fp x0, Fp x1, Fp x2, Fp x3, const Fold2C<F>& c) {
-  const uint64_t d20 = gl::sub(x2.v, x0.v), d10 = gl::sub(x1.v, x0.v), dd = gl::sub(gl::sub(x3.v, x1.v), d20);
-  uint64_t cw1[3], cw2[3], cw12[3];
-  cw1[0] = c.c1.b0, cw1[1] = c.c1.b1, cw2[0] = c.c2.b0, cw2[1] = c.c2.b1, cw12[0] = c.c12.b0, cw12[1] = c.c12.b1;
-  if constexpr (F::D == 3) cw1[2] = c.c1.b2, cw2[2] = c.c2.b2, cw12[2] = c.c12.b2;
-  uint64_t o[3];
+  for (int k = 0; k < NE; k++) {
+    uint64_t* wd = reinterpret_cast<uint64_t*>(&p.e[k]);
 #pragma unroll
-  for (int k = 0; k < F::D; k++) {
-    gl::AccW a;
-    gl::zero(a);
-    if (k == 0) gl::addw(a, x0.v);
-    gl::macw(a, d20, cw1[k]);
-    gl::macw(a, d10, cw2[k]);
-    gl::macw(a, dd, cw12[k]);
-    o[k] = gl::reduce(a);
+    for (int i = 0; i < NW; i++) wd[i] = uniform64(wd[i]);
   }
-  return gl::load_words<F>(o);
+  return p;
 }
 
 template <bool NT, class FI>
@@ -126,7 +65,8 @@ __device__ __forceinline__ void ld2x(const View<FI>& t, size_t i, FI& a, FI& b) 
 // VEC consecutive level-L elements from e (e even and 16-byte aligned planes
 // when VEC = 2), P = 0 (the stored level itself, lifted) or P = 2 pending folds.
 template <int P, int VEC, bool NT, class FI, class F>
-__device__ __forceinline__ void load_level(const View<FI>& t, size_t e, size_t L, const Fold2C<F>& c, F (&o)[VEC]) {
+__device__ __forceinline__ void load_level(const View<FI>& t, size_t e, size_t L, const gl::Fold2Pre<F>& c,
+                                           uint64_t w, F (&o)[VEC]) {
   if constexpr (VEC == 2) {
     if constexpr (P == 0) {
       FI a, b;
@@ -139,14 +79,14 @@ __device__ __forceinline__ void load_level(const View<FI>& t, size_t e, size_t L
       ld2x<NT>(t, e + L, x1[0], x1[1]);
       ld2x<NT>(t, e + 2 * L, x2[0], x2[1]);
       ld2x<NT>(t, e + 3 * L, x3[0], x3[1]);
-      o[0] = fold2x(x0[0], x1[0], x2[0], x3[0], c);
-      o[1] = fold2x(x0[1], x1[1], x2[1], x3[1], c);
+      o[0] = gl::fold2(x0[0], x1[0], x2[0], x3[0], c, w);
+      o[1] = gl::fold2(x0[1], x1[1], x2[1], x3[1], c, w);
     }
   } else {
     if constexpr (P == 0)
       o[0] = gl::lift_to<F>(t.ld(e));
     else
-      o[0] = fold2x(t.ld(e), t.ld(e + L), t.ld(e + 2 * L), t.ld(e + 3 * L), c);
+      o[0] = gl::fold2(t.ld(e), t.ld(e + L), t.ld(e + 2 * L), t.ld(e + 3 * L), c, w);
   }
 }
 
@@ -218,8 +158,8 @@ template <class FI, class F, int K, int P, int VEC, bool NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_pass_ext(
     Views<FI, K> src, Views<F, K> dst, const uint64_t* cptr, size_t q, uint64_t w, Reduce r) {
   constexpr int NR = pass_raw<K>(), NV = pass_values<K>();
-  Fold2C<F> cc{};
-  if constexpr (P > 0) cc = fold2_prep<F>(cptr, w);
+  gl::Fold2Pre<F> cc{};
+  if constexpr (P > 0) cc = uniform_fold2<F>(cptr, w);
   const size_t L = 4 * q;
   F s[NR];
   static_for<NR>([&](auto i) { s[i] = gl::zero<F>(); });
@@ -231,7 +171,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) 
     F E[K][4][VEC];
     static_for<K>([&](auto i) {
       static_for<4>([&](auto j) {
-        load_level<P, VEC, NT>(src.t[i], u + j * q, L, cc, E[i][j]);
+        load_level<P, VEC, NT>(src.t[i], u + j * q, L, cc, w, E[i][j]);
         if constexpr (P > 0) __builtin_amdgcn_sched_barrier(0);
       });
     });
@@ -381,20 +321,6 @@ __device__ __forceinline__ F fold_pre(const F& lo, const F& hi, const Pre& c) {
   return gl::add(gl::mul(gl::sub(hi, lo), c), lo);
 }
 
-// The two pending folds' constants (gl::prepare_fold2) from the challenges
-// at cptr (c1 then c2), kept in scalar registers.
-template <class F>
-__device__ __forceinline__ gl::Fold2Pre<F> uniform_fold2(const uint64_t* cptr, uint64_t w) {
-  gl::Fold2Pre<F> p = gl::prepare_fold2(gl::load_words<F>(cptr), gl::load_words<F>(cptr + F::D), w);
-  constexpr int NW = sizeof(p.e[0]) / 8;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    uint64_t* wd = reinterpret_cast<uint64_t*>(&p.e[k]);
-#pragma unroll
-    for (int i = 0; i < NW; i++) wd[i] = uniform64(wd[i]);
-  }
-  return p;
-}
 
 // The level-4L elements behind level-L elements e, e + 1 (e even): x[k] at
 // e + k L (P = 2), or the level-L pair itself (P = 0).
@@ -582,7 +508,7 @@ int launch_pass_ext_t(effsc_hip_prover* h, size_t q, const uint64_t* cptr) {
 
 // Used over F_p^2, and for the multilinear prover over F_p^3 (measured with
 // the SHAKE transcript, v = 26 over F_p^2: inner product 2.41 -> 2.12 ms with
-// three separate folds, 1.66 ms with fold2x, multilinear 1.06 -> 0.88 -> 0.73
+// three separate folds, 1.66 ms with gl::fold2, multilinear 1.06 -> 0.88 -> 0.73
 // ms; v = 24 over F_p^3: multilinear 0.669 -> 0.644 -> 0.471 ms).  The F_p^3
 // inner-product pass (8 F_p^3 products per unit at two waves per SIMD)
 // measured 1.071 against 1.078 ms for one round per launch: no gain, so it

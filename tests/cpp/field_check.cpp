@@ -191,6 +191,22 @@ int main(int argc, char** argv) {
         }
         for (int c = 0; c < d; c++)
           if (got[c] != ref[c]) fail(d == 2 ? "fold2_fp2" : "fold2_fp3", i);
+        // base-field x0..x3 (first fold of base inputs): the same folds with
+        // x lifted to (x, 0, 0)
+        uint64_t xb[4][3] = {{x[0][0], 0, 0}, {x[1][0], 0, 0}, {x[2][0], 0, 0}, {x[3][0], 0, 0}};
+        fold(xb[0], xb[2], c1, a);
+        fold(xb[1], xb[3], c1, b);
+        fold(a, b, c2, ref);
+        auto eb = [&](int k) { return gl::Fp{x[k][0]}; };
+        if (d == 2) {
+          const auto p = gl::prepare_fold2(gl::Fp2{c1[0], c1[1]}, gl::Fp2{c2[0], c2[1]}, w);
+          gl::store_words(gl::fold2(eb(0), eb(1), eb(2), eb(3), p, w), got);
+        } else {
+          const auto p = gl::prepare_fold2(gl::Fp3{c1[0], c1[1], c1[2]}, gl::Fp3{c2[0], c2[1], c2[2]}, w);
+          gl::store_words(gl::fold2(eb(0), eb(1), eb(2), eb(3), p, w), got);
+        }
+        for (int c = 0; c < d; c++)
+          if (got[c] != ref[c]) fail(d == 2 ? "fold2_base_fp2" : "fold2_base_fp3", i);
         n++;
       }
     }
