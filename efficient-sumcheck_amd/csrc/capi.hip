@@ -160,7 +160,48 @@ size_t tail_max(const effsc_hip_prover* h) {
   // pair costs a few products, an F_p^3 product-of-K pair about 12 K.
   if (h->ext == 3) return h->K >= 3 ? (size_t)1 << 10 : (size_t)1 << 12;
   if (h->ext == 2) return h->K >= 3 ? (size_t)1 << 11 : (size_t)1 << 13;
+  // base field, K <= 2: the pipelined tail takes up to 4096 (K = 2) or 8192
+  // (K = 1) elements, the register tail 2^14
+  if (h->K <= 2 && !std::getenv("EFFSC_HIP_TAIL_REG") && !std::getenv("EFFSC_HIP_TAIL_PLAIN"))
+    return h->K == 1 ? (size_t)8192 : (size_t)4096;
   return (size_t)1 << 14;
+}
+
+const DevFs* fs_in(effsc_hip_prover* h) {
+  const DevFs* t = h->fs_pending ? h->fs_pending : h->d_fs;
+  h->fs_pending = nullptr;
+  return t;
+}
+
+int fs_flush(effsc_hip_prover* h) {
+  if (h->fs_pending) {
+    HIPCHK(hipMemcpyAsync(h->d_fs, h->fs_pending, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+    h->fs_pending = nullptr;
+  }
+  return EFFSC_HIP_OK;
+}
+
+bool pipe_tail_ok(const effsc_hip_prover* h, size_t L0, size_t rounds) {
+  if (h->ext != 1 || h->K > 2 || std::getenv("EFFSC_HIP_TAIL_REG") || std::getenv("EFFSC_HIP_TAIL_PLAIN")) return false;
+  const size_t lmax = h->K == 1 ? (size_t)pipe_max<1>() : (size_t)pipe_max<2>();
+  if (L0 < 4 || L0 > lmax || (L0 & (L0 - 1)) || rounds < 1) return false;
+  size_t lg = 0;
+  while (((size_t)1 << lg) < L0) lg++;
+  return rounds <= lg;
+}
+
+int launch_tail_pipe(effsc_hip_prover* h, uint64_t* const* src, size_t sst, size_t L0, int pend, int rounds,
+                     size_t first_round, const uint64_t* cur) {
+  auto go = [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    const Views<Fp, K> in = views<Fp, K>(src, sst), work = views<Fp, K>(h->work, h->work_stride);
+    hipLaunchKernelGGL((k_tail_pipe<K>), dim3(1), dim3(kPipeThreads), 0, h->stream, in, work, (int)L0, pend, rounds,
+                       fs_in(h), h->d_fs, h->d_polys + first_round * K, h->d_chals + first_round, cur, h->d_fin);
+    HIPCHK(hipGetLastError());
+    return EFFSC_HIP_OK;
+  };
+  if (h->K == 1) return go(std::integral_constant<int, 1>{});
+  return go(std::integral_constant<int, 2>{});
 }
 
 // ------------------------------------------------------------ round logic
@@ -469,12 +510,15 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->eq_full) (void)hipFree(h->eq_full);
+  if (h->eq_l_spare) (void)hipFree(h->eq_l_spare);
+  if (h->eq_r_spare) (void)hipFree(h->eq_r_spare);
   if (h->eq_rp) (void)hipFree(h->eq_rp);
   if (h->d_pass0) (void)hipFree(h->d_pass0);
   if (h->ooc_parts) (void)hipFree(h->ooc_parts);
   if (h->gkr_b) (void)hipFree(h->gkr_b);
   if (h->h_res) (void)hipHostFree(h->h_res);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  if (h->ev_proof) (void)hipEventDestroy(h->ev_proof);
   for (uint64_t* p : {h->d_polys, h->d_cur}) if (p) (void)hipFree(p);  // d_chals, d_fin, d_fs: inside d_polys
   if (h->h_proof) (void)hipHostFree(h->h_proof);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -827,10 +871,19 @@ int effsc_hip_read_table(effsc_hip_prover* h, size_t table, uint64_t* dst, size_
 int effsc_hip_reset(effsc_hip_prover* h) {
   if (!h) return fail(EFFSC_HIP_EINVAL, "null prover");
   if (h->kind == K_EQ) {
-    // the eq tables are folded in place: rebuild them from the stored point
+    // the eq tables are folded in place: take the fresh ones a device-loop
+    // proof built behind its proof words, or rebuild them from the point
     HIPCHK(hipSetDevice(h->dev));
-    TRY(build_eq(h, h->eq_point, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len));
-    TRY(build_eq(h, h->eq_point, h->v_l, h->v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len));
+    if (h->eq_spare_ready) {
+      std::swap(h->eq_l, h->eq_l_spare);
+      std::swap(h->eq_r, h->eq_r_spare);
+      h->eq_l_len = (size_t)1 << h->v_l;
+      h->eq_r_len = (size_t)1 << (h->v - h->v_l);
+      h->eq_spare_ready = false;
+    } else {
+      TRY(build_eq(h, h->eq_point, 0, h->v_l, &h->eq_l, &h->eq_l_stride, &h->eq_l_len));
+      TRY(build_eq(h, h->eq_point, h->v_l, h->v - h->v_l, &h->eq_r, &h->eq_r_stride, &h->eq_r_len));
+    }
     h->rounds_elapsed = 0;
   }
   h->len = h->n;

@@ -102,12 +102,14 @@ int ensure_dev_bufs(effsc_hip_prover* h) {
   h->d_fs = reinterpret_cast<DevFs*>(h->d_fin + 4);
   TRY(alloc_dev(&h->d_cur, 8));  // two pending challenges of up to 3 words
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), kProofWords * 8, hipHostMallocDefault));
+  h->h_proof_pinned = true;
   return EFFSC_HIP_OK;
 }
 
 // The proof block back to the host staging (polys, challenges, final value
 // and transcript state in one copy).
 int copy_proof_out(effsc_hip_prover* h) {
+  TRY(fs_flush(h));
   HIPCHK(hipMemcpyAsync(h->h_proof, h->d_polys, kProofWords * 8, hipMemcpyDeviceToHost, h->stream));
   return EFFSC_HIP_OK;
 }
@@ -129,7 +131,10 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
     hs->pos = t->fs.sp.pos;
   }
   hs->rng = t->rng;
-  HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+  // a pinned staging is read by the first transcript kernel itself (fs_in);
+  // otherwise (a caller's host buffer) the state is uploaded now
+  h->fs_pending = hs;
+  if (!h->h_proof_pinned) TRY(fs_flush(h));
 
   // host-side pending folds are applied first; from here on the pending
   // challenges live on the device (d_cur[0..p))
@@ -167,13 +172,15 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
         h->len = L;
         h->in_work = true;
       }
-      if (ext2)
+      if (ext2) {
+        TRY(fs_flush(h));
         TRY(launch_reduce_fs2_ext(h, red, g, r));
+      }
       else if (K == 1)
-        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
                            h->d_polys + r, h->d_chals + r, h->d_cur);
       else
-        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, h->d_fs,
+        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
                            h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
       HIPCHK(hipGetLastError());
       r += 2;
@@ -197,13 +204,23 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
         fr = fi;
       }
       unsigned g = h->last_grid;
+      TRY(fs_flush(h));
       TRY(dispatch<DevOps>(fr, fr, K, [&](auto op) { return decltype(op)::reduce_fs(h, g, r); }));
       r++;
       p = 1;
     }
   }
   const uint64_t* cur = h->d_cur;
-  if (p == 2) {
+  if (p == 2 && pipe_tail_ok(h, h->len / 4, num_rounds - r)) {
+    // the pipelined tail applies both pending folds as it loads the tables
+    const size_t L0 = h->len / 4, rounds_left = num_rounds - r;
+    TRY(launch_tail_pipe(h, src_planes(h), src_stride(h), L0, 2, (int)rounds_left, r, h->d_cur));
+    h->len = L0 >> rounds_left;
+    h->in_work = true;
+    tail_used = true;
+    p = 0;
+    r = num_rounds;
+  } else if (p == 2) {
     // apply the older of two pending challenges with a multi-block fold
     size_t M = h->len;
     uint64_t* const* src = src_planes(h);
@@ -218,7 +235,9 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   size_t rounds_left = num_rounds - r;
   size_t M = h->len;
   const bool have = p == 1;
-  if (rounds_left > 0 || M <= tail_max(h)) {
+  if (tail_used) {
+    // done by the pipelined tail
+  } else if (rounds_left > 0 || M <= tail_max(h)) {
     // remaining rounds, finalize and final value in one workgroup
     int fi = src_field(h);
     bool from_input = !h->in_work;

@@ -333,14 +333,17 @@ __global__ __launch_bounds__(kBlock) void k_eq_pass_sv(const uint64_t* src, uint
 // Two transcript rounds from the pass values, then the MSB folds of the eq
 // table of round s's variable (t0, half h0) with x and of round s+1's (t1,
 // half h1) with y; t0 and t1 may be the same table (h1 = h0 / 2 then).
-__global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
-                                                                uint64_t* chals, uint64_t* cur, uint64_t* t0,
-                                                                size_t h0, uint64_t* t1, size_t h1) {
+// The sponge is loaded from t_in (t, or the pinned host staging of a starting
+// proof) and stored to t.
+__global__ __launch_bounds__(kReduceBlock) void k_eq_reduce_fs2(Reduce r, unsigned nblocks, const DevFs* t_in,
+                                                                DevFs* t, uint64_t* poly, uint64_t* chals,
+                                                                uint64_t* cur, uint64_t* t0, size_t h0, uint64_t* t1,
+                                                                size_t h1) {
   constexpr int NV = pass_values<2>();
   __shared__ uint64_t sh[kReduceBlock];
   __shared__ uint64_t xy[3];
   LaneFs fs;
-  if (threadIdx.x < 64) fs.load(t);
+  if (threadIdx.x < 64) fs.load(t_in);
   sum_partials_words<NV>(r.partials, nblocks, sh);
   if (threadIdx.x < 64) {
     Fp sv[NV];
@@ -734,7 +737,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         uint64_t* t1 = j + 1 < v_l ? h->eq_l : h->eq_r;
         const size_t h1 = *l1 >> 1;
         *l1 = h1;
-        hipLaunchKernelGGL(k_eq_reduce_fs2, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs,
+        hipLaunchKernelGGL(k_eq_reduce_fs2, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, fs_in(h), h->d_fs,
                            h->d_polys + j * 2, h->d_chals + j, h->d_cur, t0, h0, t1, h1);
         HIPCHK(hipGetLastError());
         j += 2;
@@ -747,13 +750,26 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         size_t m_log = 0;
         while (((size_t)1 << m_log) < h->eq_r_len) m_log++;
         const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
-        hipLaunchKernelGGL(k_eq_tail_prep, dim3(blocks_for(h, Lt)), dim3(kBlock), 0, h->stream, src, h->work[0],
-                           h->d_cur, Lt, h->eq_l, h->eq_r, (unsigned)m_log, h->eq_full);
-        HIPCHK(hipGetLastError());
         Views<Fp, 2> tv{};
         tv.t[0] = View<Fp>{h->work[0], 0};
         tv.t[1] = View<Fp>{h->eq_full, 0};
         const int rounds = (int)(v - j);
+        size_t lg = 0;
+        while (((size_t)1 << lg) < Lt) lg++;
+        if (Lt <= (size_t)pipe_max<2>() && rounds >= 1 && (size_t)rounds <= lg && !std::getenv("EFFSC_HIP_TAIL_REG")) {
+          // the pipelined tail over p and EQ: p's two pending folds and EQ
+          // at level Lt as it loads them
+          Views<Fp, 2> sv{};
+          sv.t[0] = View<Fp>{const_cast<uint64_t*>(src), 0};
+          sv.t[1] = View<Fp>{h->eq_full, 0};
+          hipLaunchKernelGGL((k_tail_pipe<2, true>), dim3(1), dim3(kPipeThreads), 0, h->stream, sv, tv, (int)Lt, 2,
+                             rounds, fs_in(h), h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin, h->eq_l, h->eq_r,
+                             (unsigned)m_log);
+        } else {
+        TRY(fs_flush(h));
+        hipLaunchKernelGGL(k_eq_tail_prep, dim3(blocks_for(h, Lt)), dim3(kBlock), 0, h->stream, src, h->work[0],
+                           h->d_cur, Lt, h->eq_l, h->eq_r, (unsigned)m_log, h->eq_full);
+        HIPCHK(hipGetLastError());
         switch (Lt) {
           case 2048:
             hipLaunchKernelGGL((k_tail_reg<2, 2>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds, 0,
@@ -771,11 +787,24 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
             hipLaunchKernelGGL((k_tail_reg<2, 16>), dim3(1), dim3(kTailRegThreads), 0, h->stream, tv, tv, 0, rounds,
                                0, h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin);
         }
+        }
         HIPCHK(hipGetLastError());
+        // the proof block is complete: copy it out first, then fold eq_l and
+        // eq_r for the post-state (no proof word depends on them), so the
+        // caller waits for the copy only
+        TRY(copy_proof_out(h));
+        if (!h->ev_proof) HIPCHK(hipEventCreateWithFlags(&h->ev_proof, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(h->ev_proof, h->stream));
+        h->proof_copied = true;
         const int left = j < v_l ? (int)(v_l - j) : 0;
         hipLaunchKernelGGL(k_eq_post_fold, dim3(1), dim3(kReduceBlock), 0, h->stream, h->eq_l, h->eq_l_len, h->eq_r,
                            h->eq_r_len, h->d_chals + j, rounds, left, h->d_cur);
         HIPCHK(hipGetLastError());
+        // fresh tables for the next proof, behind the copy as well
+        size_t st = 0, ln = 0;
+        TRY(build_eq(h, h->eq_point, 0, v_l, &h->eq_l_spare, &st, &ln));
+        TRY(build_eq(h, h->eq_point, v_l, v - v_l, &h->eq_r_spare, &st, &ln));
+        h->eq_spare_ready = true;
         h->len = 1;
         h->in_work = true;
         h->eq_l_len = 1;
@@ -800,6 +829,8 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
       }
     }
   }
+  // every kernel below reads the transcript from d_fs
+  TRY(fs_flush(h));
   if constexpr (F::D >= 2 && std::is_same<FI, F>::value) {
     if (!std::getenv("EFFSC_HIP_ONE_ROUND") && !std::getenv("EFFSC_HIP_EQ_DIRECT")) {
       // two rounds per pass while both variables of a pass are left ones
@@ -986,7 +1017,11 @@ int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript
     hs->pos = t->fs.sp.pos;
   }
   hs->rng = t->rng;
-  HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+  // a pinned staging is read by the first transcript kernel itself (fs_in);
+  // otherwise the state is uploaded now
+  h->fs_pending = hs;
+  h->proof_copied = false;
+  if (!h->h_proof_pinned) TRY(fs_flush(h));
   if (E == 1)
     TRY((eq_device_run<Fp, Fp>(h, num_rounds)));
   else if (E == 2)
@@ -996,8 +1031,15 @@ int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript
   uint64_t* hp = h->h_proof;
   uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
   uint64_t* hf = hc + kMaxRounds * 3;
-  TRY(copy_proof_out(h));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->proof_copied) {
+    // post-state work follows the copy on the stream; the proof is ready
+    // once the copy is
+    h->proof_copied = false;
+    HIPCHK(hipEventSynchronize(h->ev_proof));
+  } else {
+    TRY(copy_proof_out(h));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
   TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * 2 * E, round_polys);
   std::copy(hc, hc + num_rounds * E, challenges);

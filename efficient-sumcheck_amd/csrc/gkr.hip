@@ -567,6 +567,7 @@ int prove_device_gkr(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcrip
     hs->pos = t->fs.sp.pos;
   }
   hs->rng = t->rng;
+  h->fs_pending = nullptr;
   HIPCHK(hipMemcpyAsync(h->d_fs, hs, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
   bool tail_used = false;
   const int fi = src_field(h);

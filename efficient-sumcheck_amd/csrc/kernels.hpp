@@ -218,15 +218,41 @@ __device__ __forceinline__ Fp3 readlane(Fp3 a, int l) {
   return {readlane64(a.c0, l), readlane64(a.c1, l), readlane64(a.c2, l)};
 }
 
+// The wave's sums of K values, reduced transposed: at the step with lane mask
+// m, lanes with that bit set keep the upper half of their values and lanes
+// without it the lower half, and each receives its partner's copy of the half
+// it keeps.  After log2(P) steps (P = K rounded up to a power of two) every
+// lane holds one value, index lane >> (6 - log2 P), and the remaining masks
+// finish a plain butterfly: lane l returns the sum of value
+// l >> (6 - log2 P) (zero for indices >= K).  P - 1 + 6 - log2 P exchanges
+// instead of 6 K.
+template <class F, int K>
+__device__ __forceinline__ F wave_reduce_t(const F* acc) {
+  constexpr int P = pow2_ceil(K), LP = ilog2c(P);
+  const int lane = threadIdx.x & 63;
+  F v[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) v[i] = i < K ? acc[i] : gl::zero<F>();
+#pragma unroll
+  for (int l = 0; l < LP; l++) {
+    const int m = 32 >> l, half = P >> (l + 1);
+    const bool hi = lane & m;
+#pragma unroll
+    for (int j = 0; j < half; j++) {
+      F keep = select(hi, v[j + half], v[j]);
+      F send = select(hi, v[j], v[j + half]);
+      v[j] = gl::add(keep, shfl_xor(send, m));
+    }
+  }
+#pragma unroll
+  for (int m = 32 >> LP; m >= 1; m >>= 1) v[0] = gl::add(v[0], shfl_xor(v[0], m));
+  return v[0];
+}
+
 // Reduces acc over the block; the result is valid in every lane of wave 0.
 //
-// Within a wave the K values are reduced transposed: at the step with lane
-// mask m, lanes with that bit set keep the upper half of their values and
-// lanes without it the lower half, and each receives its partner's copy of
-// the half it keeps.  After log2(P) steps (P = K rounded up to a power of two)
-// every lane holds one value, index lane >> (6 - log2 P), and the remaining
-// masks finish a plain butterfly.  That is P - 1 + 6 - log2 P exchanges
-// instead of 6 K (measured for K = 8 over 1024 threads: 2.0 against 8.8 us).
+// Within a wave the K values are reduced transposed (wave_reduce_t; measured
+// for K = 8 over 1024 threads: 2.0 against 8.8 us for a butterfly per value).
 // One lane per (wave, value) writes LDS; wave 0 sums the waves, lane i value
 // i, and broadcasts with readlane.
 template <class F, int K>
@@ -246,25 +272,10 @@ __device__ __forceinline__ void block_reduce(F* acc) {
       for (int wv = 1; wv < nw; wv++) acc[0] = gl::add(acc[0], gl::load_words<F>(&lds[wv][0]));
     }
   } else {
-    constexpr int P = pow2_ceil(K), LP = ilog2c(P);
-    F v[P];
-#pragma unroll
-    for (int i = 0; i < P; i++) v[i] = i < K ? acc[i] : gl::zero<F>();
-#pragma unroll
-    for (int l = 0; l < LP; l++) {
-      const int m = 32 >> l, half = P >> (l + 1);
-      const bool hi = lane & m;
-#pragma unroll
-      for (int j = 0; j < half; j++) {
-        F keep = select(hi, v[j + half], v[j]);
-        F send = select(hi, v[j], v[j + half]);
-        v[j] = gl::add(keep, shfl_xor(send, m));
-      }
-    }
-#pragma unroll
-    for (int m = 32 >> LP; m >= 1; m >>= 1) v[0] = gl::add(v[0], shfl_xor(v[0], m));
+    constexpr int LP = ilog2c(pow2_ceil(K));
+    const F v0 = wave_reduce_t<F, K>(acc);
     const int idx = lane >> (6 - LP);
-    if ((lane & ((64 >> LP) - 1)) == 0 && idx < K) gl::store_words(v[0], &lds[wave][idx * F::D]);
+    if ((lane & ((64 >> LP) - 1)) == 0 && idx < K) gl::store_words(v0, &lds[wave][idx * F::D]);
     __syncthreads();
     if (wave == 0) {
       const int nw = blockDim.x >> 6;

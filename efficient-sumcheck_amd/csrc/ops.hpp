@@ -194,6 +194,7 @@ struct OpsImpl {
 template <class FI, class F, int K>
 struct DevOpsImpl {
   static int reduce_fs(effsc_hip_prover* h, unsigned g, size_t round) {
+    TRY(fs_flush(h));
     hipLaunchKernelGGL((k_reduce_fs<FI, K>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, h->ext,
                        h->d_polys + round * K * h->ext, h->d_chals + round * h->ext, h->d_cur);
     HIPCHK(hipGetLastError());
@@ -201,12 +202,17 @@ struct DevOpsImpl {
   }
   static int tail(effsc_hip_prover* h, uint64_t* const* src, size_t sst, bool from_input, size_t L, int rounds,
                   bool have, size_t first_round, const uint64_t* cur) {
+    if constexpr (!(std::is_same<FI, Fp>::value && std::is_same<F, Fp>::value && K <= 2)) TRY(fs_flush(h));
     Views<FI, K> in = views<FI, K>(src, sst);
     Views<F, K> work = views<F, K>(h->work, h->work_stride);
     if constexpr (std::is_same<FI, Fp>::value && std::is_same<F, Fp>::value && K <= 2) {
-      // base-field tables of 2^11 .. 2^14 elements (after the pending fold):
-      // the register-resident tail
+      // base-field tables of 4 .. 4096 (K = 2) or 8192 (K = 1) elements after
+      // the pending fold: the pipelined tail; 2^11 .. 2^14: the
+      // register-resident tail
       const size_t L1 = have ? L / 2 : L;
+      if (pipe_tail_ok(h, L1, (size_t)std::max(rounds, 0)))
+        return launch_tail_pipe(h, src, sst, L1, have ? 1 : 0, rounds, first_round, cur);
+      TRY(fs_flush(h));
       if (rounds > 0 && L1 >= 2048 && L1 <= 16384 && !std::getenv("EFFSC_HIP_TAIL_PLAIN")) {
         auto go = [&](auto m0) {
           constexpr int M0 = decltype(m0)::value;

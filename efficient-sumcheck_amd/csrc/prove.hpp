@@ -151,6 +151,7 @@ struct LaneFs {
     if (lane < 25) t->st[lane] = a;
     if (lane == 0) {
       t->pos = pos;
+      t->kind = kind;
       t->rng = rng;
     }
   }
@@ -330,14 +331,16 @@ __device__ __forceinline__ void emit_two_rounds(const Fp* s, LaneFs& fs, DevFs* 
 
 // Two-round pass (pass2.hpp) over base-field tables: sums the pass values,
 // then runs the two transcript steps (emit_two_rounds).
+// The sponge is loaded from t_in (t, or the pinned host staging of a starting
+// proof) and stored to t.
 template <int K>
-__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, DevFs* t, uint64_t* poly,
-                                                       uint64_t* chals, uint64_t* cur) {
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned nblocks, const DevFs* t_in, DevFs* t,
+                                                       uint64_t* poly, uint64_t* chals, uint64_t* cur) {
   constexpr int NV = pass_values<K>();
   __shared__ uint64_t sh[kReduceBlock];
   // the sponge load overlaps the partial sums
   LaneFs fs;
-  if (threadIdx.x < 64) fs.load(t);
+  if (threadIdx.x < 64) fs.load(t_in);
   sum_partials_words<NV>(r.partials, nblocks, sh);
   if (threadIdx.x >= 64) return;
   Fp s[NV];
@@ -661,6 +664,195 @@ __global__ __launch_bounds__(kTailRegThreads) void k_tail_reg(Views<Fp, K> in, V
       if (m < Mc) work.t[i].st(tid + 1024 * m, x[i][m]);
   if (tid == 0) fin[0] = 0;  // more than one element left
   if (tid < 64) fs.store(t);
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined tail for base-field tables (K <= 2) of L0 <= pipe_max<K>()
+// elements after the pending folds.  Round s+1's values are polynomials of
+// degree K in round s's challenge (the two-round pass algebra, pass2.hpp), so
+// while wave 0 runs round s's transcript step the worker waves (1..8) fold the
+// tables with c_{s-1} and accumulate the raw sums R_s of the folded level,
+// whose coefficient part gives round s+1's values at c_s.  A round then costs
+// one transcript step and K polynomial evaluations on wave 0; the fold and
+// the sums run beside the Keccak instead of before it.
+//   init     workers: T_0 (the pending folds applied) into LDS, R_0
+//   step 0   wave 0: round 0's values from R_0, absorb, c_0
+//   step s   wave 0: round s's values = R_{s-1}'s coefficients at c_{s-1},
+//            absorb, c_s;  workers: T_s = fold(T_{s-1}, c_{s-1}), R_s (only
+//            while a round s + 1 follows)
+//   end      workers: the final fold with the last challenge, the tables back
+//            to `work`, the final value
+// One barrier per round.  The tables alternate between two LDS buffers
+// (T_s in A for even s, in B for odd s), the raw sums and challenges between
+// two slots, so a step's reads and writes never meet.  Same round
+// polynomials, challenges, final tables and final value as k_tail_reg.
+// Needs 1 <= rounds <= log2(L0).
+constexpr int kPipeWorkers = 512;
+constexpr int kPipeThreads = 64 + kPipeWorkers;
+template <int K>
+constexpr int pipe_max() {
+  return K == 1 ? 8192 : 4096;
+}
+
+// The workers' raw sums, reduced over each wave into part[wave - 1].
+template <int K>
+__device__ __forceinline__ void pipe_part(const PassAcc<K>& acc, uint64_t (*part)[pass_raw<K>()]) {
+  constexpr int NR = pass_raw<K>(), LP = ilog2c(pow2_ceil(NR));
+  Fp raw[NR];
+#pragma unroll
+  for (int i = 0; i < NR; i++) raw[i] = acc.a[i].reduce();
+  const Fp v = wave_reduce_t<Fp, NR>(raw);
+  const int lane = threadIdx.x & 63, idx = lane >> (6 - LP);
+  if ((lane & ((64 >> LP) - 1)) == 0 && idx < NR) part[(threadIdx.x >> 6) - 1][idx] = v.v;
+}
+
+// EQ (EqFactoredProver, K = 2): table 1 of T_0 is EQ[e] = eql[e >> m_log]
+// eqr[e & (2^m_log - 1)], computed as it is loaded (table 0 is p with its
+// pending folds), so no separate preparation launch writes it first.
+template <int K, bool EQ = false>
+__global__ __launch_bounds__(kPipeThreads) void k_tail_pipe(Views<Fp, K> src, Views<Fp, K> work, int L0, int pend,
+                                                            int rounds, const DevFs* t_in, DevFs* t, uint64_t* polys,
+                                                            uint64_t* chals,
+                                                            const uint64_t* chal_cur, uint64_t* fin,
+                                                            const uint64_t* eql = nullptr,
+                                                            const uint64_t* eqr = nullptr, unsigned m_log = 0) {
+  static_assert(!EQ || K == 2, "EQ: p and EQ");
+  constexpr int NR = pass_raw<K>(), NV = pass_values<K>(), NW = kPipeWorkers / 64, LA = pipe_max<K>();
+  __shared__ uint64_t tabA[K][LA], tabB[K][LA / 2];
+  __shared__ uint64_t part[2][NW][NR];
+  __shared__ uint64_t cs[2];
+  const int tid = threadIdx.x, lane = tid & 63, wk = tid - 64;
+  const bool w0 = tid < 64;
+  LaneFs fs;
+  if (w0) fs.load(t_in);
+  if (!w0) {
+    // T_0: the level-L0 tables from the stored level with pend pending folds
+    auto init = [&](auto pc) {
+      constexpr int P = decltype(pc)::value;
+      const uint64_t c1 = P >= 1 ? chal_cur[0] : 0, c2 = P == 2 ? chal_cur[1] : 0;
+      const uint64_t c12 = P == 2 ? gl::mul(c1, c2) : 0;
+      const int q = L0 / 4;
+      PassAcc<K> acc;
+#pragma unroll
+      for (int i = 0; i < NR; i++) acc.a[i].zero();
+      for (int u0 = wk; u0 < q; u0 += kPipeWorkers) {
+        // the unit index through an empty asm: otherwise loop strength
+        // reduction keeps a 64-bit pointer per region and table live
+        int u = u0;
+        asm volatile("" : "+v"(u));
+        uint64_t E[K][4];
+#pragma unroll
+        for (int i = 0; i < K; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int e = u + j * q;
+            if (EQ && i == 1)
+              E[i][j] = gl::mul(eql[e >> m_log], eqr[e & ((1 << m_log) - 1)]);
+            else
+              E[i][j] = fold_chain<P>(src.t[i].p, e, L0, c1, c2, c12);
+            tabA[i][e] = E[i][j];
+            if constexpr (P > 0) __builtin_amdgcn_sched_barrier(0);
+          }
+        pass_accumulate<K>(E, acc);
+      }
+      pipe_part<K>(acc, part[0]);
+    };
+    if (pend == 0)
+      init(std::integral_constant<int, 0>{});
+    else if (pend == 1)
+      init(std::integral_constant<int, 1>{});
+    else
+      init(std::integral_constant<int, 2>{});
+  }
+  __syncthreads();
+  uint64_t cprev = 0;
+  for (int s = 0; s < rounds; s++) {
+    if (w0) {
+      // raw sums R_{s-1} (R_0 at s = 0): lane i < NR sums value i over the
+      // worker waves, then every lane takes all of them
+      const int slot = s == 0 ? 0 : (s - 1) & 1;
+      Fp sv = gl::zero<Fp>();
+      if (lane < NR)
+#pragma unroll
+        for (int w = 0; w < NW; w++) sv = gl::add(sv, Fp{part[slot][w][lane]});
+      Fp raw[NR], vals[NV], V[K];
+#pragma unroll
+      for (int i = 0; i < NR; i++) raw[i] = readlane(sv, i);
+      pass_values_from_raw<K>(raw, vals);
+      if (s == 0) {
+#pragma unroll
+        for (int i = 0; i < K; i++) V[i] = vals[i];
+      } else {
+        uint64_t cw[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) cw[i] = vals[i].v;
+#pragma unroll
+        for (int i = 0; i < K; i++) V[i] = Fp{pass_eval_next<K>(cw, i, cprev)};
+      }
+      cprev = emit_round_regs<K>(fs, V, polys + (size_t)s * K, chals + s);
+      if (lane == 0) cs[s & 1] = cprev;
+    } else if (s >= 1) {
+      const uint64_t c = cs[(s - 1) & 1];
+      const int Ls = L0 >> s;  // length of T_s; T_{s-1} has 2 Ls
+      // T_{s-1} in A for even s - 1, in B for odd; T_s into the other
+      const bool src_b = (s - 1) & 1;
+      auto ld = [&](int i, int e) -> uint64_t { return src_b ? tabB[i][e] : tabA[i][e]; };
+      auto st = [&](int i, int e, uint64_t x) {
+        if (src_b)
+          tabA[i][e] = x;
+        else
+          tabB[i][e] = x;
+      };
+      if (s + 1 < rounds) {
+        const int q = Ls / 4;
+        PassAcc<K> acc;
+#pragma unroll
+        for (int i = 0; i < NR; i++) acc.a[i].zero();
+        for (int u = wk; u < q; u += kPipeWorkers) {
+          uint64_t E[K][4];
+#pragma unroll
+          for (int i = 0; i < K; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const int e = u + j * q;
+              const uint64_t lo = ld(i, e), hi = ld(i, e + Ls);
+              const uint64_t x = gl::add(lo, gl::mul(c, gl::sub(hi, lo)));
+              E[i][j] = x;
+              st(i, e, x);
+            }
+          pass_accumulate<K>(E, acc);
+        }
+        pipe_part<K>(acc, part[s & 1]);
+      } else {
+        for (int e = wk; e < Ls; e += kPipeWorkers)
+#pragma unroll
+          for (int i = 0; i < K; i++) {
+            const uint64_t lo = ld(i, e), hi = ld(i, e + Ls);
+            st(i, e, gl::add(lo, gl::mul(c, gl::sub(hi, lo))));
+          }
+      }
+    }
+    __syncthreads();
+  }
+  if (w0) {
+    fs.store(t);
+    return;
+  }
+  // the final fold with c_{rounds-1}: T_{rounds-1} (length 2 Lf) -> work
+  const uint64_t c = cs[(rounds - 1) & 1];
+  const int Lf = L0 >> rounds;
+  const bool inB = (rounds - 1) & 1;
+  for (int e = wk; e < Lf; e += kPipeWorkers) {
+    uint64_t prod = 1;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      const uint64_t lo = inB ? tabB[i][e] : tabA[i][e], hi = inB ? tabB[i][e + Lf] : tabA[i][e + Lf];
+      const uint64_t x = gl::add(lo, gl::mul(c, gl::sub(hi, lo)));
+      work.t[i].p[e] = x;
+      prod = i == 0 ? x : gl::mul(prod, x);
+    }
+    if (e == 0) fin[0] = Lf == 1 ? prod : 0;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -80,6 +80,11 @@ struct effsc_hip_prover {
   size_t eq_l_stride = 0, eq_r_stride = 0, eq_l_len = 0, eq_r_len = 0;
   uint64_t* eq_point = nullptr;  // canonical point, v * ext words (rebuilds the eq tables on reset)
   uint64_t* eq_full = nullptr;   // EQ = eq_l x eq_r at the register-tail level (at most 2^14 words)
+  // fresh eq_l / eq_r built behind a device-loop proof (after its proof
+  // words are copied out): reset swaps them in instead of rebuilding
+  uint64_t* eq_l_spare = nullptr;
+  uint64_t* eq_r_spare = nullptr;
+  bool eq_spare_ready = false;
   // eq_r with its coefficient sums for Karatsuba products (extension-field
   // passes): 3 planes over F_p^2, 6 over F_p^3, stride eq_r_stride
   uint64_t* eq_rp = nullptr;
@@ -116,7 +121,16 @@ struct effsc_hip_prover {
   uint64_t* d_fin = nullptr;    // 4 words
   DevFs* d_fs = nullptr;
   uint64_t* d_cur = nullptr;    // 3 words: current challenge (own allocation)
-  uint64_t* h_proof = nullptr;  // pinned staging: polys | chals | fin | fs
+  uint64_t* h_proof = nullptr;  // staging: polys | chals | fin | fs
+  bool h_proof_pinned = false;  // h_proof from hipHostMalloc (device-readable)
+  // set when the proof block's copy to h_proof is already enqueued (work
+  // for the post-state only may follow it on the stream); ev_proof marks it
+  bool proof_copied = false;
+  hipEvent_t ev_proof = nullptr;
+  // the transcript state of a starting device-loop proof, in the pinned
+  // staging: the first transcript kernel loads it from there (fs_in), any
+  // other first user gets it uploaded (fs_flush)
+  const DevFs* fs_pending = nullptr;
   // Kernel timing (effsc_hip_profile): HIP events around each launch on the
   // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold,
   // 3 two-round pass.
@@ -287,6 +301,19 @@ int dispatch(int fi, int fo, int K, Fn&& fn) {
 }
 
 size_t tail_max(const effsc_hip_prover* h);
+// The pipelined tail (k_tail_pipe): base-field tables, K <= 2, L0 (the length
+// after the pending folds) a power of two in [4, pipe_max<K>()], 1 <= rounds
+// <= log2(L0); EFFSC_HIP_TAIL_REG or EFFSC_HIP_TAIL_PLAIN keep the earlier
+// tails.
+bool pipe_tail_ok(const effsc_hip_prover* h, size_t L0, size_t rounds);
+// The sponge a transcript kernel starts from: the pending host staging (taken)
+// or d_fs.
+const DevFs* fs_in(effsc_hip_prover* h);
+// Uploads a pending transcript state to d_fs (before a kernel that only
+// reads d_fs, and before the proof block is copied back).
+int fs_flush(effsc_hip_prover* h);
+int launch_tail_pipe(effsc_hip_prover* h, uint64_t* const* src, size_t sst, size_t L0, int pend, int rounds,
+                     size_t first_round, const uint64_t* cur);
 int src_field(const effsc_hip_prover* h);
 uint64_t* const* src_planes(const effsc_hip_prover* h);
 size_t src_stride(const effsc_hip_prover* h);

@@ -147,11 +147,12 @@ def test_out_of_core_prover_matches_oracle(tmp_path, monkeypatch, kind, v, chunk
 
 def test_out_of_core_rejects_small_and_auto_picks_resident(tmp_path, monkeypatch):
     monkeypatch.delenv("EFFSC_HIP_TAIL", raising=False)
-    fs = files(tmp_path, 2, 1 << 15, 31)
+    # 2^14 elements: n/4 = 4096, the inner product's tail size
+    fs = files(tmp_path, 2, 1 << 14, 31)
     s = [FileStream(p) for p, _ in fs]
     with pytest.raises(N.NativeError):
         inner_product_from_stream(s[0], s[1], mode=OUT_OF_CORE)  # n/4 is not above the tail
     g = inner_product_from_stream(s[0], s[1], mode=AUTO)  # fits: resident, per-round calls allowed
     o = O.Prover.inner_product(fs[0][1], fs[1][1])
-    hooked = sumcheck(g, 15, TestTranscript(3), lambda r, t: None)
-    assert tuple(as_oracle(hooked, 1)) == tuple(o.sumcheck(15, O.Transcript.rng(3)))
+    hooked = sumcheck(g, 14, TestTranscript(3), lambda r, t: None)
+    assert tuple(as_oracle(hooked, 1)) == tuple(o.sumcheck(14, O.Transcript.rng(3)))
