@@ -179,12 +179,15 @@ struct LaneFs {
       if (ext > 2) out[2] = 0;
       return;
     }
+    // pad the sponge itself to the block boundary and permute: the padding
+    // stays in the stream, the next word starts a fresh block
     absorb_word(2);
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t c = a;
-    if (lane == (pos >> 3)) c ^= 0x1F;  // SHAKE padding
-    if (lane == (kShakeRate - 1) / 8) c ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) % 8));
-    c = keccak_lanes(c);
+    if (lane == (pos >> 3)) a ^= 0x1F;  // SHAKE padding
+    if (lane == (kShakeRate - 1) / 8) a ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) % 8));
+    a = keccak_lanes(a);
+    pos = 0;
+    const uint64_t c = a;
     uint64_t w0 = shfl64(c, 0), w1 = shfl64(c, 1), w2 = shfl64(c, 2), w3 = shfl64(c, 3), w4 = shfl64(c, 4),
              w5 = shfl64(c, 5);
     out[0] = uniform64(gl::reduce128(w0, w1));

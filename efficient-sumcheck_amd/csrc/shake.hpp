@@ -5,11 +5,16 @@
 // crate, src/transcript/spongefish.rs:17-44, whose wire encoding is not part
 // of the reference; this library fixes its own).  Every item is a whole number
 // of little-endian u64 words, so the sponge absorbs whole lanes:
-//   "effsc-hip/fs/v2\0" (2 words) || len(domsep) || domsep zero-padded to 8 bytes
+//   "effsc-hip/fs/v3\0" (2 words) || len(domsep) || domsep zero-padded to 8 bytes
 //   send(x)      : word 1, then each canonical component
 //   challenge(d) : word 2 is appended; the first 16 d bytes of
-//                  SHAKE256(stream) give d little-endian u128 values mod p.
-// The sponge is kept incrementally, so each operation costs O(1) permutations.
+//                  SHAKE256(stream) give d little-endian u128 values mod p;
+//                  then the stream is padded to the next 136-byte boundary
+//                  with SHAKE's padding (0x1F, zeros, 0x80).
+// The sponge is kept incrementally.  Because the padding stays in the stream,
+// a challenge pads and permutes the sponge itself and reads its output from
+// the permuted state: one permutation per challenge, plus one per 136 bytes
+// absorbed within a round.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -82,23 +87,7 @@ struct Sponge {
   SKHD void absorb_u64(uint64_t x) {
     for (int j = 0; j < 8; j++) absorb_byte((uint8_t)(x >> (8 * j)));
   }
-  // Pads a copy with the SHAKE suffix and squeezes up to one rate block.
-  SKHD void squeeze_copy(uint8_t* out, int outlen) const {
-    Sponge c = *this;
-    c.st[c.pos >> 3] ^= (uint64_t)0x1F << (8 * (c.pos & 7));
-    c.st[(kShakeRate - 1) >> 3] ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) & 7));
-    keccak_f1600(c.st);
-    for (int i = 0; i < outlen; i++) out[i] = (uint8_t)(c.st[i >> 3] >> (8 * (i & 7)));
-  }
 };
-
-// u128 (little-endian bytes) mod p
-SKHD uint64_t u128_mod_p(const uint8_t* b) {
-  uint64_t lo = 0, hi = 0;
-  for (int j = 0; j < 8; j++) lo |= (uint64_t)b[j] << (8 * j);
-  for (int j = 0; j < 8; j++) hi |= (uint64_t)b[8 + j] << (8 * j);
-  return gl::reduce128(lo, hi);
-}
 
 struct FsTranscript {
   Sponge sp;
@@ -112,7 +101,7 @@ struct FsTranscript {
     }
   }
   SKHD void init(const uint8_t* domsep, size_t len) {
-    const char tag[16] = "effsc-hip/fs/v2";
+    const char tag[16] = "effsc-hip/fs/v3";
     sp.init();
     sp.absorb(reinterpret_cast<const uint8_t*>(tag), 16);
     sp.absorb_u64((uint64_t)len);
@@ -125,9 +114,11 @@ struct FsTranscript {
   }
   SKHD void challenge(int d, uint64_t* out) {
     absorb_word(2);
-    uint8_t buf[48];
-    sp.squeeze_copy(buf, 16 * d);
-    for (int i = 0; i < d; i++) out[i] = u128_mod_p(buf + 16 * i);
+    sp.st[sp.pos >> 3] ^= (uint64_t)0x1F << (8 * (sp.pos & 7));
+    sp.st[(kShakeRate - 1) >> 3] ^= (uint64_t)0x80 << (8 * ((kShakeRate - 1) & 7));
+    keccak_f1600(sp.st);
+    sp.pos = 0;
+    for (int i = 0; i < d; i++) out[i] = gl::reduce128(sp.st[2 * i], sp.st[2 * i + 1]);
   }
 };
 

@@ -87,7 +87,8 @@ def uniform_table(n, seed):
 
 
 # ----------------------------------------------------------------- transcripts
-FS_TAG = b"effsc-hip/fs/v2\x00"  # 16 bytes
+FS_TAG = b"effsc-hip/fs/v3\x00"  # 16 bytes
+SHAKE_RATE = 136  # SHAKE256 block size in bytes
 
 
 def _pad8(bs):
@@ -96,7 +97,10 @@ def _pad8(bs):
 
 class ShakeTranscript:
     """Byte-stream definition of the Fiat-Shamir transcript; every item is a
-    whole number of little-endian 64-bit words."""
+    whole number of little-endian 64-bit words.  A challenge is the SHAKE256
+    of the stream so far; the stream is then padded to the next block
+    boundary with SHAKE's own padding (0x1F, zeros, 0x80), so the sponge that
+    produced the challenge is the sponge of the padded stream."""
 
     def __init__(self, domsep: bytes):
         self.stream = bytearray(FS_TAG + len(domsep).to_bytes(8, "little") + _pad8(domsep))
@@ -107,6 +111,10 @@ class ShakeTranscript:
     def challenge(self, d):
         self.stream += (2).to_bytes(8, "little")
         out = hashlib.shake_256(bytes(self.stream)).digest(16 * d)
+        pad = bytearray(SHAKE_RATE - len(self.stream) % SHAKE_RATE)
+        pad[0] ^= 0x1F
+        pad[-1] ^= 0x80
+        self.stream += pad
         return tuple(int.from_bytes(out[16 * i:16 * i + 16], "little") % P for i in range(d))
 
 

@@ -279,7 +279,7 @@ struct or_transcript {
     sponge sp;
     uint64_t rng;
 };
-static const char FS_TAG[16] = "effsc-hip/fs/v2"; /* 15 chars + NUL = 16 bytes */
+static const char FS_TAG[16] = "effsc-hip/fs/v3"; /* 15 chars + NUL = 16 bytes */
 
 static void sp_absorb_u64(sponge *s, uint64_t x) {
     uint8_t b[8];
@@ -318,10 +318,14 @@ void or_tr_challenge(or_transcript *t, int d, uint64_t *out) {
         for (int i = 1; i < d; i++) out[i] = 0;
         return;
     }
+    /* the stream so far is padded to the block boundary (pad10*1 with the
+       SHAKE suffix 0x1F) and the padded block absorbed: the output is the
+       first 16 d bytes of SHAKE256 of the stream, and the padding stays in
+       the stream, so the next item starts a fresh block */
     sp_absorb_u64(&t->sp, 2);
-    sponge c = t->sp;
     uint8_t buf[48];
-    sp_finish(&c, buf, (size_t)16 * d);
+    sp_finish(&t->sp, buf, (size_t)16 * d);
+    t->sp.pos = 0;
     for (int i = 0; i < d; i++) {
         uint64_t lo = 0, hi = 0;
         for (int j = 0; j < 8; j++) lo |= (uint64_t)buf[16 * i + j] << (8 * j);

@@ -45,10 +45,13 @@ void or_shake256(const uint8_t *msg, size_t len, uint8_t *out, size_t outlen);
 /* ---- transcripts ---- */
 typedef struct or_transcript or_transcript;
 /* Fiat-Shamir transcript: SHAKE256 over a stream of little-endian u64 words
- *   "effsc-hip/fs/v2\0" (2 words) || len || domsep zero-padded to 8 bytes
+ *   "effsc-hip/fs/v3\0" (2 words) || len || domsep zero-padded to 8 bytes
  *   send(x):      word 1, then the canonical components
  *   challenge(d): word 2 is appended; output = d x (u128le mod p) from the
- *                 first 16 d bytes of SHAKE256(stream). */
+ *                 first 16 d bytes of SHAKE256(stream); then the stream is
+ *                 padded to the next 136-byte boundary with the SHAKE
+ *                 padding (0x1F, zeros, 0x80), so a challenge costs one
+ *                 permutation. */
 or_transcript *or_tr_new_shake(const uint8_t *domsep, size_t len);
 /* Test transcript: challenge = from_u64(splitmix64 next) lifted into ext. */
 or_transcript *or_tr_new_rng(uint64_t seed);

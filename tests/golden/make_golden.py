@@ -59,6 +59,20 @@ def transcript_vectors():
                 ops.append(["challenge", list(t.challenge(ext))])
         ops.append(["challenge", list(t.challenge(ext))])
         out.append({"domsep": ds.hex(), "ext": ext, "ops": ops})
+    # block-boundary cases of the padded stream: after a challenge the stream
+    # is block-aligned; eight base-field sends (16 bytes each) and the
+    # challenge word end exactly on the next boundary (a whole padding block
+    # follows), and twelve F_p^3 sends cross two boundaries inside one round
+    for ext, sends in ((1, (1, 8, 8, 7)), (3, (1, 12, 2))):
+        t = B.ShakeTranscript(b"effsc-hip/blocks")
+        ops = []
+        for k, ns in enumerate(sends):
+            for i in range(ns):
+                v = tuple(B.uniform(98, 100 * k + 3 * i + j) for j in range(ext))
+                t.send(v)
+                ops.append(["send", list(v)])
+            ops.append(["challenge", list(t.challenge(ext))])
+        out.append({"domsep": b"effsc-hip/blocks".hex(), "ext": ext, "ops": ops})
     rng_t = B.RngTranscript(42)
     out.append({"rng_seed": 42, "challenges": [list(rng_t.challenge(1)) for _ in range(5)]})
     return out
