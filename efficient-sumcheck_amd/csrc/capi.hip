@@ -175,7 +175,8 @@ const DevFs* fs_in(effsc_hip_prover* h) {
 
 int fs_flush(effsc_hip_prover* h) {
   if (h->fs_pending) {
-    HIPCHK(hipMemcpyAsync(h->d_fs, h->fs_pending, sizeof(DevFs), hipMemcpyHostToDevice, h->stream));
+    // from the pinned staging or from d_fs_alt
+    HIPCHK(hipMemcpyAsync(h->d_fs, h->fs_pending, sizeof(DevFs), hipMemcpyDefault, h->stream));
     h->fs_pending = nullptr;
   }
   return EFFSC_HIP_OK;
@@ -510,6 +511,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_r) (void)hipFree(h->eq_r);
   if (h->eq_point) (void)hipFree(h->eq_point);
   if (h->eq_full) (void)hipFree(h->eq_full);
+  if (h->fuse_parts) (void)hipFree(h->fuse_parts);  // d_fs_alt: inside fuse_parts
   if (h->eq_l_spare) (void)hipFree(h->eq_l_spare);
   if (h->eq_r_spare) (void)hipFree(h->eq_r_spare);
   if (h->eq_rp) (void)hipFree(h->eq_rp);

@@ -101,6 +101,8 @@ int ensure_dev_bufs(effsc_hip_prover* h) {
   h->d_fin = h->d_chals + kMaxRounds * 3;
   h->d_fs = reinterpret_cast<DevFs*>(h->d_fin + 4);
   TRY(alloc_dev(&h->d_cur, 8));  // two pending challenges of up to 3 words
+  TRY(alloc_dev(&h->fuse_parts, (size_t)kFuseMaxGrid * 8 + (sizeof(DevFs) + 7) / 8));
+  h->d_fs_alt = reinterpret_cast<DevFs*>(h->fuse_parts + (size_t)kFuseMaxGrid * 8);
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h->h_proof), kProofWords * 8, hipHostMallocDefault));
   h->h_proof_pinned = true;
   return EFFSC_HIP_OK;
@@ -114,6 +116,42 @@ int copy_proof_out(effsc_hip_prover* h) {
   return EFFSC_HIP_OK;
 }
 
+
+// A base-field fold pass that can take the previous pass's transcript step
+// into its own launch (k_pass_fs2): two pending folds, vector-aligned
+// planes, and a grid small enough to be resident at once.
+bool fuse_ok(const effsc_hip_prover* h, int p, size_t q) {
+  if (!h->fuse_parts || h->ext != 1 || h->K > 2 || p != 2 || std::getenv("EFFSC_HIP_NO_FUSE")) return false;
+  if (q < 2 || q % 2 || src_stride(h) % 2 || h->work_stride % 2) return false;
+  return (q / 2 + kBlock - 1) / kBlock <= kFuseMaxGrid;
+}
+
+// k_pass_fs2 over level 4q -> q units: runs the transcript step of the pass
+// whose values are in (rin, nin) as rounds r, r + 1, then this pass; its
+// partials go to the other buffer, returned in rout / g.
+int launch_pass_fs(effsc_hip_prover* h, size_t q, Reduce rin, unsigned nin, size_t r, Reduce* rout, unsigned* g) {
+  uint64_t* const* src = src_planes(h);
+  const size_t sst = src_stride(h);
+  const unsigned grid = (unsigned)((q / 2 + kBlock - 1) / kBlock);
+  const Reduce ro = rin.partials == h->fuse_parts ? h->red : Reduce{h->fuse_parts, h->red.result};
+  const DevFs* tin = fs_in(h);
+  DevFs* tout = tin == h->d_fs ? h->d_fs_alt : h->d_fs;
+  auto go = [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    hipLaunchKernelGGL((k_pass_fs2<K>), dim3(grid), dim3(kBlock), 0, h->stream, views<Fp, K>(src, sst),
+                       views<Fp, K>(h->work, h->work_stride), q, rin, nin, ro, tin, tout, h->d_polys + K * r,
+                       h->d_chals + r, h->d_cur);
+    HIPCHK(hipGetLastError());
+    return EFFSC_HIP_OK;
+  };
+  TRY(h->K == 1 ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, 2>{}));
+  // the next transcript kernel starts from d_fs_alt (or it is copied to d_fs)
+  if (tout == h->d_fs_alt) h->fs_pending = h->d_fs_alt;
+  *rout = ro;
+  *g = grid;
+  h->last_grid = grid;
+  return EFFSC_HIP_OK;
+}
 
 // Runs rounds [0, num_rounds), finalize and final_value on the device with
 // the transcript state copied in and out of `t`.  Words out are canonical.
@@ -147,12 +185,38 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   if (h->twostep || ext2) {
     // two rounds per pass while the tables are large (base field: pass2.hpp;
     // extension fields: pass_ext.hip)
-    while (num_rounds - r >= 2) {
+    // a base-field pass's transcript step waits for the next pass, which
+    // takes it into its launch when it is small enough (k_pass_fs2)
+    bool deferred = false;
+    Reduce dred{};
+    unsigned dg = 0;
+    auto reduce_now = [&](Reduce red, unsigned g) -> int {
+      if (K == 1)
+        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
+                           h->d_polys + r, h->d_chals + r, h->d_cur);
+      else
+        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
+                           h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
+      HIPCHK(hipGetLastError());
+      r += 2;
+      return EFFSC_HIP_OK;
+    };
+    while (num_rounds >= r + (deferred ? 2 : 0) + 2) {
       size_t L = h->len >> p;
       if (L <= tail_max(h) || L < 4) break;
       Reduce red = h->red;
       unsigned g = 0;
-      if (p == 0 && h->pass0_ready && !h->in_work && h->len == h->n) {
+      const bool from_pass0 = p == 0 && h->pass0_ready && !h->in_work && h->len == h->n;
+      const bool plain = !from_pass0 && !(h->ooc && !h->in_work) && !ext2;
+      if (deferred && !(plain && fuse_ok(h, p, L / 4))) {
+        TRY(reduce_now(dred, dg));
+        deferred = false;
+      }
+      if (deferred) {
+        TRY(launch_pass_fs(h, L / 4, dred, dg, r, &red, &g));
+        r += 2;
+        deferred = false;
+      } else if (from_pass0) {
         // the pass over the input was accumulated while it streamed in
         red = Reduce{h->d_pass0, h->red.result};
         g = 1;
@@ -175,17 +239,17 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
       if (ext2) {
         TRY(fs_flush(h));
         TRY(launch_reduce_fs2_ext(h, red, g, r));
+        r += 2;
+      } else if (plain) {
+        deferred = true;
+        dred = red;
+        dg = g;
+      } else {
+        TRY(reduce_now(red, g));
       }
-      else if (K == 1)
-        hipLaunchKernelGGL(k_reduce_fs2<1>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
-                           h->d_polys + r, h->d_chals + r, h->d_cur);
-      else
-        hipLaunchKernelGGL(k_reduce_fs2<2>, dim3(1), dim3(kReduceBlock), 0, h->stream, red, g, fs_in(h), h->d_fs,
-                           h->d_polys + 2 * r, h->d_chals + r, h->d_cur);
-      HIPCHK(hipGetLastError());
-      r += 2;
       p = 2;
     }
+    if (deferred) TRY(reduce_now(dred, dg));
   } else {
     while (r < num_rounds) {
       size_t M = h->len;

@@ -298,12 +298,14 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs(Reduce r, unsigned n
 // whose lane 0 holds the reduced pass values: round s values, challenge x,
 // round s+1 values (the pass polynomials at x), challenge y.  poly receives
 // 2K words, chals (x, y), cur (x, y): the pending folds of the next pass.
-// `fs` holds the sponge loaded from t (callers issue that load early).
+// `fs` holds the sponge loaded from t (callers issue that load early).  With
+// write false nothing goes to memory but (x, y) to xy_sh (when given), which
+// lane 0 writes in either case.
 template <int K>
 __device__ __forceinline__ void emit_two_rounds(const Fp* s, LaneFs& fs, DevFs* t, uint64_t* poly, uint64_t* chals,
-                                                uint64_t* cur) {
+                                                uint64_t* cur, bool write = true, uint64_t* xy_sh = nullptr) {
   constexpr int NV = pass_values<K>();
-  const bool lane0 = (threadIdx.x & 63) == 0;
+  const bool lane0 = (threadIdx.x & 63) == 0 && write;
   uint64_t v[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) v[i] = uniform64(s[i].v);
@@ -329,7 +331,11 @@ __device__ __forceinline__ void emit_two_rounds(const Fp* s, LaneFs& fs, DevFs* 
     cur[0] = x[0];
     cur[1] = y[0];
   }
-  fs.store(t);
+  if (xy_sh && (threadIdx.x & 63) == 0) {
+    xy_sh[0] = x[0];
+    xy_sh[1] = y[0];
+  }
+  if (write) fs.store(t);
 }
 
 // Two-round pass (pass2.hpp) over base-field tables: sums the pass values,
@@ -350,6 +356,71 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
 #pragma unroll
   for (int i = 0; i < NV; i++) s[i] = Fp{sh[i]};
   emit_two_rounds<K>(s, fs, t, poly, chals, cur);
+}
+
+// A small two-round fold pass (pass2.hpp) with the previous pass's transcript
+// step in the same launch, for grids of at most kFuseMaxGrid workgroups (all
+// resident at once, so the step below costs the launch its latency once).
+// Every workgroup issues its units' source loads first, sums the previous
+// pass's partials (rin, nin blocks) and runs the two transcript steps on wave
+// 0 (the steps of k_reduce_fs2, so every workgroup derives the same two
+// challenges), then applies both folds, stores the folded level and
+// accumulates this pass's values into rout.  Workgroup 0 alone writes the
+// proof words, the challenges, the pending folds and the sponge; the sponge
+// goes to t_out and the partials to rout, neither of which this launch reads.
+// One unit pair per thread: the grid covers q / 2 threads.
+constexpr unsigned kFuseMaxGrid = 512;
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_pass_fs2(Views<Fp, K> src, Views<Fp, K> dst, size_t q, Reduce rin,
+                                                     unsigned nin, Reduce rout, const DevFs* t_in, DevFs* t_out,
+                                                     uint64_t* poly, uint64_t* chals, uint64_t* cur) {
+  constexpr int NV = pass_values<K>(), NR = pass_raw<K>();
+  __shared__ uint64_t sh[kBlock];
+  __shared__ uint64_t xy[2];
+  const size_t L = 4 * q, u = gtid() * 2;
+  const bool act = u < q;
+  // level-4L source: element e + kL (k < 4) of region j for units u, u + 1
+  v2u64 X[K][4][4];
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) X[i][j][k] = *reinterpret_cast<const v2u64*>(src.t[i].p + u + j * q + k * L);
+  }
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t_in);
+  sum_partials_words<NV>(rin.partials, nin, sh);
+  if (threadIdx.x < 64) {
+    Fp s[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) s[i] = Fp{sh[i]};
+    emit_two_rounds<K>(s, fs, t_out, poly, chals, cur, blockIdx.x == 0, xy);
+  }
+  __syncthreads();
+  const uint64_t c1 = xy[0], c2 = xy[1], c12 = gl::mul(c1, c2);
+  PassAcc<K> acc;
+#pragma unroll
+  for (int i = 0; i < NR; i++) acc.a[i].zero();
+  if (act) {
+    uint64_t E0[K][4], E1[K][4];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        E0[i][j] = gl::fold2(X[i][j][0].x, X[i][j][1].x, X[i][j][2].x, X[i][j][3].x, c1, c2, c12);
+        E1[i][j] = gl::fold2(X[i][j][0].y, X[i][j][1].y, X[i][j][2].y, X[i][j][3].y, c1, c2, c12);
+        *reinterpret_cast<ulonglong2*>(dst.t[i].p + u + j * q) = make_ulonglong2(E0[i][j], E1[i][j]);
+      }
+    pass_accumulate<K>(E0, acc);
+    pass_accumulate<K>(E1, acc);
+  }
+  Fp raw[NR], vals[NV];
+#pragma unroll
+  for (int i = 0; i < NR; i++) raw[i] = acc.a[i].reduce();
+  pass_values_from_raw<K>(raw, vals);
+  finish<Fp, NV>(vals, rout);
 }
 
 constexpr int kTailBlock = 1024;

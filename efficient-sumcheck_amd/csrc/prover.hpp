@@ -127,10 +127,16 @@ struct effsc_hip_prover {
   // for the post-state only may follow it on the stream); ev_proof marks it
   bool proof_copied = false;
   hipEvent_t ev_proof = nullptr;
-  // the transcript state of a starting device-loop proof, in the pinned
-  // staging: the first transcript kernel loads it from there (fs_in), any
-  // other first user gets it uploaded (fs_flush)
+  // where the current transcript state is when it is not in d_fs: the pinned
+  // staging of a starting device-loop proof, or d_fs_alt after a fused pass;
+  // the next transcript kernel loads it from there (fs_in), any other next
+  // user gets it copied to d_fs (fs_flush)
   const DevFs* fs_pending = nullptr;
+  // fused passes (k_pass_fs2): the second sponge slot and partials buffer
+  // (kFuseMaxGrid blocks of up to 8 words), allocated with the proof block;
+  // null (no fused passes) for the allocation-free entry point
+  DevFs* d_fs_alt = nullptr;
+  uint64_t* fuse_parts = nullptr;
   // Kernel timing (effsc_hip_profile): HIP events around each launch on the
   // prover's stream; slot 0 fused fold+evaluate, 1 evaluate, 2 fold,
   // 3 two-round pass.
@@ -306,11 +312,11 @@ size_t tail_max(const effsc_hip_prover* h);
 // <= log2(L0); EFFSC_HIP_TAIL_REG or EFFSC_HIP_TAIL_PLAIN keep the earlier
 // tails.
 bool pipe_tail_ok(const effsc_hip_prover* h, size_t L0, size_t rounds);
-// The sponge a transcript kernel starts from: the pending host staging (taken)
-// or d_fs.
+// The sponge a transcript kernel starts from: the pending state (taken; the
+// kernel writes its result to d_fs) or d_fs.
 const DevFs* fs_in(effsc_hip_prover* h);
-// Uploads a pending transcript state to d_fs (before a kernel that only
-// reads d_fs, and before the proof block is copied back).
+// Copies a pending transcript state to d_fs (before a kernel that only reads
+// d_fs, and before the proof block is copied back).
 int fs_flush(effsc_hip_prover* h);
 int launch_tail_pipe(effsc_hip_prover* h, uint64_t* const* src, size_t sst, size_t L0, int pend, int rounds,
                      size_t first_round, const uint64_t* cur);

@@ -493,6 +493,27 @@ def test_register_tail_parity(kind, v, tail, monkeypatch):
             check_proof(kind, 1, 1, n, v, 4000 + v, transcript=tr, rounds=rounds)
 
 
+@pytest.mark.parametrize("tail", [None, 2, 1 << 9])
+@pytest.mark.parametrize("kind", ["ml", "ip"])
+@pytest.mark.parametrize("v", [9, 12, 17, 20])
+def test_fused_small_passes(kind, v, tail, monkeypatch):
+    # fold passes of at most 512 workgroups take the previous pass's transcript
+    # step into their launch (k_pass_fs2): every workgroup runs the step, the
+    # first writes the proof words; the sponge and the partials alternate
+    # between two buffers.  Against the oracle, partial runs included, and the
+    # same proof with EFFSC_HIP_NO_FUSE (one transcript launch per pass)
+    if tail is not None:
+        monkeypatch.setenv("EFFSC_HIP_TAIL", str(tail))
+    n = 1 << v
+    for rounds in sorted({2, 5, v - 1, v}):
+        for tr in ("shake", "rng"):
+            _, fused = check_proof(kind, 1, 1, n, v, 4800 + v, transcript=tr, rounds=rounds)
+            monkeypatch.setenv("EFFSC_HIP_NO_FUSE", "1")
+            _, plain = check_proof(kind, 1, 1, n, v, 4800 + v, transcript=tr, rounds=rounds)
+            monkeypatch.delenv("EFFSC_HIP_NO_FUSE")
+            assert plain.round_polys == fused.round_polys and plain.final_value == fused.final_value
+
+
 @pytest.mark.parametrize("tail", [None, 1 << 9, 1 << 10])
 @pytest.mark.parametrize("kind,input_ext", [("p3", 1), ("p3", 3), ("p4", 3)])
 @pytest.mark.parametrize("v", [10, 11, 12])
