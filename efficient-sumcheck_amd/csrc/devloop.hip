@@ -119,9 +119,11 @@ int copy_proof_out(effsc_hip_prover* h) {
 
 // A base-field fold pass that can take the previous pass's transcript step
 // into its own launch (k_pass_fs2): two pending folds, vector-aligned
-// planes, and a grid small enough to be resident at once.
-bool fuse_ok(const effsc_hip_prover* h, int p, size_t q) {
-  if (!h->fuse_parts || h->ext != 1 || h->K > 2 || p != 2 || std::getenv("EFFSC_HIP_NO_FUSE")) return false;
+// planes, a grid small enough to be resident at once and a previous pass of
+// at most kFuseMaxIn workgroups (nin).
+bool fuse_ok(const effsc_hip_prover* h, int p, size_t q, unsigned nin) {
+  if (!h->fuse_parts || h->ext != 1 || h->K > 2 || p != 2 || nin > kFuseMaxIn || std::getenv("EFFSC_HIP_NO_FUSE"))
+    return false;
   if (q < 2 || q % 2 || src_stride(h) % 2 || h->work_stride % 2) return false;
   return (q / 2 + kBlock - 1) / kBlock <= kFuseMaxGrid;
 }
@@ -208,7 +210,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
       unsigned g = 0;
       const bool from_pass0 = p == 0 && h->pass0_ready && !h->in_work && h->len == h->n;
       const bool plain = !from_pass0 && !(h->ooc && !h->in_work) && !ext2;
-      if (deferred && !(plain && fuse_ok(h, p, L / 4))) {
+      if (deferred && !(plain && fuse_ok(h, p, L / 4, dg))) {
         TRY(reduce_now(dred, dg));
         deferred = false;
       }

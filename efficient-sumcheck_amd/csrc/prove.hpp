@@ -368,8 +368,12 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_fs2(Reduce r, unsigned 
 // accumulates this pass's values into rout.  Workgroup 0 alone writes the
 // proof words, the challenges, the pending folds and the sponge; the sponge
 // goes to t_out and the partials to rout, neither of which this launch reads.
-// One unit pair per thread: the grid covers q / 2 threads.
-constexpr unsigned kFuseMaxGrid = 512;
+// One unit pair per thread: the grid covers q / 2 threads.  Every workgroup
+// reads all nin partials, so the previous pass's grid is bounded too
+// (kFuseMaxIn): the inner product's 2^20-element level (512 workgroups after
+// a 2048-block pass) took 38.8 us fused against 36.5 us as a pass plus a
+// transcript launch, the 2^18 level 21.3 against 28.8 us.
+constexpr unsigned kFuseMaxGrid = 128, kFuseMaxIn = 1024;
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_pass_fs2(Views<Fp, K> src, Views<Fp, K> dst, size_t q, Reduce rin,
                                                      unsigned nin, Reduce rout, const DevFs* t_in, DevFs* t_out,

@@ -497,7 +497,7 @@ def test_register_tail_parity(kind, v, tail, monkeypatch):
 @pytest.mark.parametrize("kind", ["ml", "ip"])
 @pytest.mark.parametrize("v", [9, 12, 17, 20])
 def test_fused_small_passes(kind, v, tail, monkeypatch):
-    # fold passes of at most 512 workgroups take the previous pass's transcript
+    # fold passes of at most 128 workgroups take the previous pass's transcript
     # step into their launch (k_pass_fs2): every workgroup runs the step, the
     # first writes the proof words; the sponge and the partials alternate
     # between two buffers.  Against the oracle, partial runs included, and the
