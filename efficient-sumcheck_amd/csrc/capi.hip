@@ -513,6 +513,7 @@ void destroy(effsc_hip_prover* h) {
   if (h->eq_full) (void)hipFree(h->eq_full);
   if (h->fuse_parts) (void)hipFree(h->fuse_parts);  // d_fs_alt: inside fuse_parts
   if (h->eq_l_spare) (void)hipFree(h->eq_l_spare);
+  if (h->eq_l_alt) (void)hipFree(h->eq_l_alt);
   if (h->eq_r_spare) (void)hipFree(h->eq_r_spare);
   if (h->eq_rp) (void)hipFree(h->eq_rp);
   if (h->d_pass0) (void)hipFree(h->d_pass0);

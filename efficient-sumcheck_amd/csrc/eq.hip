@@ -330,6 +330,88 @@ __global__ __launch_bounds__(kBlock) void k_eq_pass_sv(const uint64_t* src, uint
   }
 }
 
+// k_eq_pass_sv<2, false> with the previous pass's transcript step in the same
+// launch (as k_pass_fs2, csrc/prove.hpp): every workgroup issues its loads,
+// sums the previous pass's partials (rin, nin blocks) and runs the two
+// transcript steps on wave 0, deriving the same (x, y) as k_eq_reduce_fs2.
+// Both variables of that step are left ones, so this pass's eq_l is the
+// double fold of the current table eql (4 h1 entries) with x then y, entry i
+// from eql[i + k h1] (k < 4), and eq_r is unchanged; each workgroup folds the
+// four entries it uses, and workgroup 0 writes the whole folded table to
+// eql_out (the next table; eql itself is read by every workgroup).  Workgroup
+// 0 alone writes the proof words, the challenges and the sponge (t_out).
+// Chunks of 512 units: one unit pair per thread.
+__global__ __launch_bounds__(kBlock) void k_eq_pass_sv_fs(const uint64_t* src, uint64_t* dst, size_t q,
+                                                          const uint64_t* eql, size_t h1, uint64_t* eql_out,
+                                                          const uint64_t* eqr, unsigned m_log, Reduce rin,
+                                                          unsigned nin, Reduce rout, const DevFs* t_in, DevFs* t_out,
+                                                          uint64_t* poly, uint64_t* chals, uint64_t* cur) {
+  constexpr int NV = pass_values<2>();
+  __shared__ uint64_t sh[kBlock];
+  __shared__ uint64_t xy[2];
+  const size_t L = 4 * q, mask = ((size_t)1 << m_log) - 1;
+  const size_t u0 = (size_t)blockIdx.x * (2 * kBlock), u = u0 + threadIdx.x * 2;
+  const size_t b = u0 >> m_log, qs = q >> m_log;
+  // p at level 4L, both units of the pair; the eq_r pair; thread 0: the eq_l
+  // entries under L_j
+  v2u64 X[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) X[j][k] = *reinterpret_cast<const v2u64*>(src + u + j * q + k * L);
+  const v2u64 R = *reinterpret_cast<const v2u64*>(eqr + (u & mask));
+  uint64_t El[4][4];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) El[j][k] = eql[b + j * qs + k * h1];
+  LaneFs fs;
+  if (threadIdx.x < 64) fs.load(t_in);
+  sum_partials_words<NV>(rin.partials, nin, sh);
+  if (threadIdx.x < 64) {
+    Fp sv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) sv[i] = Fp{sh[i]};
+    emit_two_rounds<2>(sv, fs, t_out, poly, chals, cur, blockIdx.x == 0, xy);
+  }
+  __syncthreads();
+  const uint64_t c1 = xy[0], c2 = xy[1], c12 = gl::mul(c1, c2);
+  if (blockIdx.x == 0)
+    for (size_t i = threadIdx.x; i < h1; i += blockDim.x)
+      eql_out[i] = gl::fold2(eql[i], eql[i + h1], eql[i + 2 * h1], eql[i + 3 * h1], c1, c2, c12);
+  Acc128 T[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint64_t e0 = gl::fold2(X[j][0].x, X[j][1].x, X[j][2].x, X[j][3].x, c1, c2, c12);
+    const uint64_t e1 = gl::fold2(X[j][0].y, X[j][1].y, X[j][2].y, X[j][3].y, c1, c2, c12);
+    *reinterpret_cast<ulonglong2*>(dst + u + j * q) = make_ulonglong2(e0, e1);
+    T[j].zero();
+    T[j].mac(e0, R.x);
+    T[j].mac(e1, R.y);
+  }
+  Fp t[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) t[j] = T[j].reduce();
+  block_reduce<Fp, 4>(t);
+  if (threadIdx.x == 0) {
+    uint64_t l[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) l[j] = gl::fold2(El[j][0], El[j][1], El[j][2], El[j][3], c1, c2, c12);
+    auto m = [](uint64_t x, uint64_t y) { return Fp{gl::mul(x, y)}; };
+    const uint64_t l20 = gl::sub(l[2], l[0]), l31 = gl::sub(l[3], l[1]), l10 = gl::sub(l[1], l[0]),
+                   l32 = gl::sub(l[3], l[2]);
+    const uint64_t t20 = gl::sub(t[2].v, t[0].v), t31 = gl::sub(t[3].v, t[1].v), t10 = gl::sub(t[1].v, t[0].v),
+                   t32 = gl::sub(t[3].v, t[2].v);
+    const Fp raw[8] = {m(l[0], t[0].v), m(l[1], t[1].v), m(l20, t20),         m(l31, t31),
+                       m(l[2], t[2].v), m(l10, t10),     m(l32, t32),         m(gl::sub(l31, l20), gl::sub(t31, t20))};
+    Fp vals[NV];
+    pass_values_from_raw<2>(raw, vals);
+#pragma unroll
+    for (int i = 0; i < NV; i++) rout.partials[(size_t)blockIdx.x * NV + i] = vals[i].v;
+  }
+}
+
 // Two transcript rounds from the pass values, then the MSB folds of the eq
 // table of round s's variable (t0, half h0) with x and of round s+1's (t1,
 // half h1) with y; t0 and t1 may be the same table (h1 = h0 / 2 then).
@@ -681,7 +763,25 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
     if (!std::getenv("EFFSC_HIP_ONE_ROUND")) {
       // two rounds per pass while the level evaluated is above the tail
       int pend = 0;
-      while (j + 2 <= v) {
+      // a pass's transcript step and eq folds wait for the next pass, which
+      // takes them into its launch when it is small (k_eq_pass_sv_fs); the
+      // eq lengths below already count the deferred folds
+      struct Deferred {
+        Reduce red;
+        unsigned g;
+        uint64_t *t0, *t1;
+        size_t h0, h1;
+      } dfr{};
+      bool deferred = false;
+      auto reduce_now = [&]() -> int {
+        hipLaunchKernelGGL(k_eq_reduce_fs2, dim3(1), dim3(kReduceBlock), 0, h->stream, dfr.red, dfr.g, fs_in(h),
+                           h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, dfr.t0, dfr.h0, dfr.t1, dfr.h1);
+        HIPCHK(hipGetLastError());
+        j += 2;
+        deferred = false;
+        return EFFSC_HIP_OK;
+      };
+      while (j + (deferred ? 2 : 0) + 2 <= v) {
         const size_t Lv = L >> pend;
         if (Lv <= tail || Lv < 8) break;
         size_t m_log = 0;
@@ -701,6 +801,29 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
                                     1, std::min<size_t>((q / 2 + kBlock - 1) / kBlock, gmax * kPassGridScale));
         const uint64_t* src = h->in_work ? h->work[0] : h->in[0];
         const bool nt = 8.0 * (double)L >= kNtMinBytes;
+        // fused with the deferred step: both of its variables are left ones
+        // (one eq_l buffer folded into the other), a split-value pass of
+        // one-step workgroups, few enough to be resident together
+        const bool fuse = deferred && h->fuse_parts && sv && pend == 2 && !nt && chunk == 2 * (size_t)kBlock && g <= kFuseMaxGrid &&
+                          dfr.g <= kFuseMaxIn && dfr.t0 == h->eq_l && dfr.t1 == h->eq_l &&
+                          !std::getenv("EFFSC_HIP_NO_FUSE");
+        if (deferred && !fuse) TRY(reduce_now());
+        if (fuse && !h->eq_l_alt) TRY(alloc_dev(&h->eq_l_alt, h->eq_l_stride * (size_t)h->ext));
+        Reduce pred = h->red;
+        if (fuse) {
+          const Reduce ro = dfr.red.partials == h->fuse_parts ? h->red : Reduce{h->fuse_parts, h->red.result};
+          const DevFs* tin = fs_in(h);
+          DevFs* tout = tin == h->d_fs ? h->d_fs_alt : h->d_fs;
+          hipLaunchKernelGGL(k_eq_pass_sv_fs, dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], q, h->eq_l, dfr.h1,
+                             h->eq_l_alt, h->eq_r, (unsigned)m_log, dfr.red, dfr.g, ro, tin, tout, h->d_polys + j * 2,
+                             h->d_chals + j, h->d_cur);
+          HIPCHK(hipGetLastError());
+          if (tout == h->d_fs_alt) h->fs_pending = h->d_fs_alt;
+          std::swap(h->eq_l, h->eq_l_alt);
+          j += 2;
+          deferred = false;
+          pred = ro;
+        } else {
         TRY(prof_begin(h));
         if (sv && pend == 0)
           hipLaunchKernelGGL((k_eq_pass_sv<0, true>), dim3(g), dim3(kBlock), 0, h->stream, src, h->work[0], h->d_cur,
@@ -723,6 +846,7 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         HIPCHK(hipGetLastError());
         TRY(prof_end(h, 3, 8.0 * ((double)L + (pend ? (double)Lv : 0.0)),
                      (double)q * ((sv ? 4.0 : 8.0 + 8.0) + (pend ? 12.0 : 0.0))));
+        }
         if (pend) {
           L = Lv;
           h->len = Lv;
@@ -737,12 +861,13 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         uint64_t* t1 = j + 1 < v_l ? h->eq_l : h->eq_r;
         const size_t h1 = *l1 >> 1;
         *l1 = h1;
-        hipLaunchKernelGGL(k_eq_reduce_fs2, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, fs_in(h), h->d_fs,
-                           h->d_polys + j * 2, h->d_chals + j, h->d_cur, t0, h0, t1, h1);
-        HIPCHK(hipGetLastError());
-        j += 2;
+        dfr = Deferred{pred, g, t0, t1, h0, h1};
+        deferred = true;
         pend = 2;
       }
+      if (deferred) TRY(reduce_now());
+      // a fused pass leaves eq_l in the other buffer
+      eql = View<F>{h->eq_l, h->eq_l_stride};
       const size_t Lt = L >> pend;
       if (pend == 2 && Lt >= 2048 && Lt <= 16384 && !std::getenv("EFFSC_HIP_EQ_TAIL")) {
         // the remaining v - j rounds in the register tail over p and EQ

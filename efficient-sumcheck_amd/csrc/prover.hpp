@@ -85,6 +85,9 @@ struct effsc_hip_prover {
   uint64_t* eq_l_spare = nullptr;
   uint64_t* eq_r_spare = nullptr;
   bool eq_spare_ready = false;
+  // the other eq_l buffer of the fused base-field passes (k_eq_pass_sv_fs
+  // writes the folded table there; the two swap after the launch)
+  uint64_t* eq_l_alt = nullptr;
   // eq_r with its coefficient sums for Karatsuba products (extension-field
   // passes): 3 planes over F_p^2, 6 over F_p^3, stride eq_r_stride
   uint64_t* eq_rp = nullptr;

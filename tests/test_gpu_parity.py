@@ -211,6 +211,30 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     assert tuple(as_oracle(rest, ext)) == tuple(po2)
 
 
+@pytest.mark.parametrize("v", [14, 17, 20, 23])
+def test_eq_factored_fused_small_passes(v, monkeypatch):
+    # base-field split-value passes of at most 128 workgroups take the previous
+    # pass's transcript step and eq_l folds into their launch
+    # (k_eq_pass_sv_fs): against the oracle and the unfused loop, with both
+    # transcripts and a second proof after reset (the eq_l buffers swap)
+    wpt = O.uniform(v, 970 + v)
+    tab = O.uniform(1 << v, 971 + v)
+    for tr in ("shake", "rng"):
+        g = EqFactoredProver(wpt, tab)
+        o = O.Prover.eq_factored(wpt, tab)
+        mk = (lambda: Transcript(b"eq-fused")) if tr == "shake" else (lambda: TestTranscript(5))
+        po = o.sumcheck(v, O.Transcript.shake(b"eq-fused") if tr == "shake" else O.Transcript.rng(5))
+        first = sumcheck(g, v, mk())
+        assert tuple(as_oracle(first, 1)) == tuple(po)
+        g.reset()
+        again = sumcheck(g, v, mk())
+        assert again.round_polys == first.round_polys and again.final_value == first.final_value
+        monkeypatch.setenv("EFFSC_HIP_NO_FUSE", "1")
+        plain = sumcheck(EqFactoredProver(wpt, tab), v, mk())
+        monkeypatch.delenv("EFFSC_HIP_NO_FUSE")
+        assert plain.round_polys == first.round_polys and plain.final_value == first.final_value
+
+
 def test_eq_factored_short_input_is_zero_padded():
     check_proof("eq", 1, 1, 37, 7, 4242)
 
