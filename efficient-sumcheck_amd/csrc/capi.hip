@@ -196,9 +196,14 @@ int launch_tail_pipe(effsc_hip_prover* h, uint64_t* const* src, size_t sst, size
   auto go = [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     const Views<Fp, K> in = views<Fp, K>(src, sst), work = views<Fp, K>(h->work, h->work_stride);
+    // the tail also copies the proof block into a pinned staging
+    // (EFFSC_HIP_PROOF_COPY=1: a separate copy after the tail)
+    uint64_t* ph = h->h_proof_pinned && !std::getenv("EFFSC_HIP_PROOF_COPY") ? h->h_proof : nullptr;
     hipLaunchKernelGGL((k_tail_pipe<K>), dim3(1), dim3(kPipeThreads), 0, h->stream, in, work, (int)L0, pend, rounds,
-                       fs_in(h), h->d_fs, h->d_polys + first_round * K, h->d_chals + first_round, cur, h->d_fin);
+                       fs_in(h), h->d_fs, h->d_polys + first_round * K, h->d_chals + first_round, cur, h->d_fin,
+                       nullptr, nullptr, 0u, ph, h->d_polys, (int)kProofWords);
     HIPCHK(hipGetLastError());
+    h->proof_by_tail = ph != nullptr;
     return EFFSC_HIP_OK;
   };
   if (h->K == 1) return go(std::integral_constant<int, 1>{});

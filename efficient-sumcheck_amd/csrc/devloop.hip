@@ -174,6 +174,7 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   // a pinned staging is read by the first transcript kernel itself (fs_in);
   // otherwise (a caller's host buffer) the state is uploaded now
   h->fs_pending = hs;
+  h->proof_by_tail = false;
   if (!h->h_proof_pinned) TRY(fs_flush(h));
 
   // host-side pending folds are applied first; from here on the pending
@@ -331,7 +332,10 @@ int prove_device(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript* t
   uint64_t* hp = h->h_proof;
   uint64_t* hc = hp + kMaxRounds * kMaxK * 3;
   uint64_t* hf = hc + kMaxRounds * 3;
-  TRY(copy_proof_out(h));
+  if (h->proof_by_tail)
+    h->proof_by_tail = false;  // the pipelined tail wrote the block into h_proof
+  else
+    TRY(copy_proof_out(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   TRY(prof_collect(h));
   std::copy(hp, hp + num_rounds * K * E, round_polys);

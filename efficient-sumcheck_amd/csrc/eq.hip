@@ -887,9 +887,13 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
           Views<Fp, 2> sv{};
           sv.t[0] = View<Fp>{const_cast<uint64_t*>(src), 0};
           sv.t[1] = View<Fp>{h->eq_full, 0};
+          // the tail also copies the proof block into a pinned staging
+          // (EFFSC_HIP_PROOF_COPY=1: a separate copy after the tail)
+          uint64_t* ph = h->h_proof_pinned && !std::getenv("EFFSC_HIP_PROOF_COPY") ? h->h_proof : nullptr;
           hipLaunchKernelGGL((k_tail_pipe<2, true>), dim3(1), dim3(kPipeThreads), 0, h->stream, sv, tv, (int)Lt, 2,
-                             rounds, fs_in(h), h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin, h->eq_l, h->eq_r,
-                             (unsigned)m_log);
+                             rounds, fs_in(h), h->d_fs, h->d_polys + j * 2, h->d_chals + j, h->d_cur, h->d_fin, h->eq_l,
+                             h->eq_r, (unsigned)m_log, ph, h->d_polys, (int)kProofWords);
+          h->proof_by_tail = ph != nullptr;
         } else {
         TRY(fs_flush(h));
         hipLaunchKernelGGL(k_eq_tail_prep, dim3(blocks_for(h, Lt)), dim3(kBlock), 0, h->stream, src, h->work[0],
@@ -914,10 +918,14 @@ int eq_device_run(effsc_hip_prover* h, size_t v) {
         }
         }
         HIPCHK(hipGetLastError());
-        // the proof block is complete: copy it out first, then fold eq_l and
-        // eq_r for the post-state (no proof word depends on them), so the
-        // caller waits for the copy only
-        TRY(copy_proof_out(h));
+        // the proof block is complete: copy it out first (unless the tail
+        // wrote it to the staging), then fold eq_l and eq_r for the
+        // post-state (no proof word depends on them), so the caller waits for
+        // the copy only
+        if (h->proof_by_tail)
+          h->proof_by_tail = false;
+        else
+          TRY(copy_proof_out(h));
         if (!h->ev_proof) HIPCHK(hipEventCreateWithFlags(&h->ev_proof, hipEventDisableTiming));
         HIPCHK(hipEventRecord(h->ev_proof, h->stream));
         h->proof_copied = true;
@@ -1146,6 +1154,7 @@ int prove_device_eq(effsc_hip_prover* h, size_t num_rounds, effsc_hip_transcript
   // otherwise the state is uploaded now
   h->fs_pending = hs;
   h->proof_copied = false;
+  h->proof_by_tail = false;
   if (!h->h_proof_pinned) TRY(fs_flush(h));
   if (E == 1)
     TRY((eq_device_run<Fp, Fp>(h, num_rounds)));

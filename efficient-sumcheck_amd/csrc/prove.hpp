@@ -793,7 +793,10 @@ __global__ __launch_bounds__(kPipeThreads) void k_tail_pipe(Views<Fp, K> src, Vi
                                                             uint64_t* chals,
                                                             const uint64_t* chal_cur, uint64_t* fin,
                                                             const uint64_t* eql = nullptr,
-                                                            const uint64_t* eqr = nullptr, unsigned m_log = 0) {
+                                                            const uint64_t* eqr = nullptr, unsigned m_log = 0,
+                                                            uint64_t* proof_host = nullptr,
+                                                            const uint64_t* proof_dev = nullptr,
+                                                            int proof_words = 0) {
   static_assert(!EQ || K == 2, "EQ: p and EQ");
   constexpr int NR = pass_raw<K>(), NV = pass_values<K>(), NW = kPipeWorkers / 64, LA = pipe_max<K>();
   __shared__ uint64_t tabA[K][LA], tabB[K][LA / 2];
@@ -914,22 +917,28 @@ __global__ __launch_bounds__(kPipeThreads) void k_tail_pipe(Views<Fp, K> src, Vi
   }
   if (w0) {
     fs.store(t);
-    return;
-  }
-  // the final fold with c_{rounds-1}: T_{rounds-1} (length 2 Lf) -> work
-  const uint64_t c = cs[(rounds - 1) & 1];
-  const int Lf = L0 >> rounds;
-  const bool inB = (rounds - 1) & 1;
-  for (int e = wk; e < Lf; e += kPipeWorkers) {
-    uint64_t prod = 1;
+  } else {
+    // the final fold with c_{rounds-1}: T_{rounds-1} (length 2 Lf) -> work
+    const uint64_t c = cs[(rounds - 1) & 1];
+    const int Lf = L0 >> rounds;
+    const bool inB = (rounds - 1) & 1;
+    for (int e = wk; e < Lf; e += kPipeWorkers) {
+      uint64_t prod = 1;
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-      const uint64_t lo = inB ? tabB[i][e] : tabA[i][e], hi = inB ? tabB[i][e + Lf] : tabA[i][e + Lf];
-      const uint64_t x = gl::add(lo, gl::mul(c, gl::sub(hi, lo)));
-      work.t[i].p[e] = x;
-      prod = i == 0 ? x : gl::mul(prod, x);
+      for (int i = 0; i < K; i++) {
+        const uint64_t lo = inB ? tabB[i][e] : tabA[i][e], hi = inB ? tabB[i][e + Lf] : tabA[i][e + Lf];
+        const uint64_t x = gl::add(lo, gl::mul(c, gl::sub(hi, lo)));
+        work.t[i].p[e] = x;
+        prod = i == 0 ? x : gl::mul(prod, x);
+      }
+      if (e == 0) fin[0] = Lf == 1 ? prod : 0;
     }
-    if (e == 0) fin[0] = Lf == 1 ? prod : 0;
+  }
+  if (proof_host) {
+    // the proof block (this launch's words and the earlier launches') straight
+    // into the pinned host staging, so the host waits for this kernel only
+    __syncthreads();
+    for (int i = tid; i < proof_words; i += kPipeThreads) proof_host[i] = proof_dev[i];
   }
 }
 

@@ -126,6 +126,9 @@ struct effsc_hip_prover {
   uint64_t* d_cur = nullptr;    // 3 words: current challenge (own allocation)
   uint64_t* h_proof = nullptr;  // staging: polys | chals | fin | fs
   bool h_proof_pinned = false;  // h_proof from hipHostMalloc (device-readable)
+  // set when the last kernel of a device-loop proof wrote the proof block into
+  // h_proof itself (k_tail_pipe), so no copy is needed
+  bool proof_by_tail = false;
   // set when the proof block's copy to h_proof is already enqueued (work
   // for the post-state only may follow it on the stream); ev_proof marks it
   bool proof_copied = false;

@@ -211,6 +211,20 @@ def test_eq_factored_device_loop(ext, v, monkeypatch):
     assert tuple(as_oracle(rest, ext)) == tuple(po2)
 
 
+@pytest.mark.parametrize("kind", ["ml", "ip", "eq"])
+@pytest.mark.parametrize("v", [12, 16, 20])
+def test_proof_block_from_the_tail(kind, v, monkeypatch):
+    # the pipelined tail writes the proof block into the pinned host staging
+    # itself; EFFSC_HIP_PROOF_COPY=1 keeps the separate copy: both give the
+    # oracle's proof, twice in a row on one prover
+    for copy in (None, "1"):
+        if copy:
+            monkeypatch.setenv("EFFSC_HIP_PROOF_COPY", copy)
+        for tr in ("shake", "rng"):
+            g, _ = check_proof(kind, 1, 1, 1 << v, v, 5600 + v, transcript=tr)
+        monkeypatch.delenv("EFFSC_HIP_PROOF_COPY", raising=False)
+
+
 @pytest.mark.parametrize("v", [14, 17, 20, 23])
 def test_eq_factored_fused_small_passes(v, monkeypatch):
     # base-field split-value passes of at most 128 workgroups take the previous
