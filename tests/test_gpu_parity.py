@@ -659,8 +659,9 @@ def test_measurement_probes_report_rates_and_reject_bad_arguments():
 
 def test_full_size_inner_product_matches_oracle():
     # the benchmark configuration itself: two 2^28-element tables from the
-    # counter-based generator, proved on the GPU (two-round passes, register
-    # tail, device transcript) and by the oracle's threaded restatement of the
+    # counter-based generator, proved on the GPU (two-round passes, fused
+    # small passes, pipelined tail, device transcript) and by the oracle's
+    # threaded restatement of the
     # reference prover (two folds per round, serial evaluate): bit-identical
     # round polynomials, challenges and final value
     v = 28
