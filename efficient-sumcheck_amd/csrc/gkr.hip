@@ -306,15 +306,52 @@ __global__ __launch_bounds__(kBlock) void k_gkr_eval_fw(Views<FI, 2> t, View<FI>
   finish<FI, 2>(acc, r);
 }
 
+// The previous round's transcript step taken into a k_gkr_fused_fw launch
+// (FS = true): the partial wires of the previous launch (rin, nin blocks),
+// the sponge (from t_in, to t_out) and the proof words, written by
+// workgroup 0 alone; the partials and the sponge alternate between two
+// buffers, so no launch reads what it writes.
+struct GkrStep {
+  Reduce rin;
+  unsigned nin;
+  const DevFs* t_in;
+  DevFs* t_out;
+  uint64_t *poly, *chals, *cur;
+};
+
 // From level 4q with j folds applied to level 2q with j + 1 (j + 1 <= k):
 // bsrc holds B_j (2 hb elements), bdst receives B_{j+1} (hb elements).
 // cconst: the level-2q pairs bind a b variable (j + 1 < k), so w_c is the
-// constant of each pair; else w_b is (B_k has one element).
-template <class FI, class FW, class F>
+// constant of each pair; else w_b is (B_k has one element).  With FS the
+// launch first runs the previous round's transcript step (every workgroup
+// derives the same challenge, as k_gkr_wire_fs), base field only.
+template <class FI, class FW, class F, bool FS = false>
 __global__ __launch_bounds__(kBlock) void k_gkr_fused_fw(Views<FI, 2> in, Views<F, 2> out, View<FI> bsrc,
                                                          View<F> bdst, size_t hb, View<FW> wit, unsigned k, int cconst,
-                                                         const uint64_t* cptr, size_t q, uint64_t w, Reduce r) {
-  const F c = gl::load_words<F>(cptr);
+                                                         const uint64_t* cptr, size_t q, uint64_t w, Reduce r,
+                                                         GkrStep stp = {}) {
+  F c;
+  if constexpr (FS) {
+    static_assert(F::D == 1, "fused transcript step: base field");
+    __shared__ uint64_t sh[kBlock];
+    __shared__ uint64_t csh[3];
+    LaneFs fs;
+    if (threadIdx.x < 64) fs.load(stp.t_in);
+    sum_partials_words<2>(stp.rin.partials, stp.nin, sh);
+    if (threadIdx.x < 64) {
+      const F s2[2] = {F{sh[0]}, F{sh[1]}};
+      const bool w0 = blockIdx.x == 0;
+      emit_round_fs<F, 2>(s2, 1, fs, stp.poly, stp.chals, csh, w0);
+      if (w0) {
+        if ((threadIdx.x & 63) == 0) stp.cur[0] = csh[0];
+        fs.store(stp.t_out);
+      }
+    }
+    __syncthreads();
+    c = gl::load_words<F>(csh);
+  } else {
+    c = gl::load_words<F>(cptr);
+  }
   const size_t mask = ((size_t)1 << k) - 1;
   const size_t step = (size_t)gridDim.x * blockDim.x;
   for (size_t i = gtid(); i < hb; i += step) bdst.st(i, fold1<F>(bsrc.ld(i), bsrc.ld(i + hb), c, w));
@@ -371,13 +408,19 @@ __global__ __launch_bounds__(kBlock) void k_gkr_fused_fw(Views<FI, 2> in, Views<
 
 // Sums the per-workgroup wires and runs the transcript step.
 template <class F>
-__global__ __launch_bounds__(kReduceBlock) void k_gkr_wire_fs(Reduce r, unsigned nblocks, DevFs* t, int ext,
-                                                              uint64_t* poly, uint64_t* chal_out, uint64_t* chal_cur) {
+// The sponge is loaded from t_in (t, or the other buffer after a fused
+// launch) and stored to t.
+__global__ __launch_bounds__(kReduceBlock) void k_gkr_wire_fs(Reduce r, unsigned nblocks, const DevFs* t_in, DevFs* t,
+                                                              int ext, uint64_t* poly, uint64_t* chal_out,
+                                                              uint64_t* chal_cur) {
   F s[2] = {gl::zero<F>(), gl::zero<F>()};
   sum_partials<F, 2>(r.partials, nblocks, s);
   block_reduce<F, 2>(s);
   if (threadIdx.x >= 64) return;
-  emit_round<F, 2>(s, ext, t, poly, chal_out, chal_cur);
+  LaneFs fs;
+  fs.load(t_in);
+  emit_round_fs<F, 2>(s, ext, fs, poly, chal_out, chal_cur);
+  fs.store(t);
 }
 
 // The level-M broadcast tables from the factored witness (j >= 1 folds).
@@ -449,34 +492,67 @@ int gkr_device_run(effsc_hip_prover* h, size_t num_rounds, bool* tail_used) {
     hipLaunchKernelGGL(k_gkr_eval_fw<FI>, dim3(g), dim3(kBlock), 0, h->stream, in2, wit, kb, M / 2, h->w, h->red);
     HIPCHK(hipGetLastError());
     TRY(prof_end(h, 1, 8.0 * 2 * (double)M * FI::D, 0.0));
-    hipLaunchKernelGGL(k_gkr_wire_fs<FI>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
-                       h->d_polys, h->d_chals, h->d_cur);
-    HIPCHK(hipGetLastError());
-    r = 1;
+    // a round's transcript step waits for the next launch, which takes it
+    // into its own when that launch is small (base field, k_gkr_fused_fw<FS>)
+    Reduce dred = h->red;
+    unsigned dg = g;
+    bool deferred = true;
+    auto wire_now = [&]() -> int {
+      if (r == 0)
+        hipLaunchKernelGGL(k_gkr_wire_fs<FI>, dim3(1), dim3(kReduceBlock), 0, h->stream, dred, dg, fs_in(h), h->d_fs,
+                           E, h->d_polys, h->d_chals, h->d_cur);
+      else
+        hipLaunchKernelGGL(k_gkr_wire_fs<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, dred, dg, fs_in(h), h->d_fs,
+                           E, h->d_polys + r * 2 * E, h->d_chals + r * E, h->d_cur);
+      HIPCHK(hipGetLastError());
+      r++;
+      deferred = false;
+      return EFFSC_HIP_OK;
+    };
     have = true;
     size_t j = 0;  // folds applied to the tables
-    while (r < num_rounds && h->len > tail && h->len >= 4 && j + 1 <= kb) {
+    while (r + (deferred ? 1 : 0) < num_rounds && h->len > tail && h->len >= 4 && j + 1 <= kb) {
       M = h->len;
       g = blocks_for(h, M / 8);
       const size_t hb = ((size_t)1 << kb) >> (j + 1);
-      TRY(prof_begin(h));
       const int cconst = j + 1 < kb ? 1 : 0;
-      if (j == 0)
-        hipLaunchKernelGGL((k_gkr_fused_fw<FI, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, in2, w2, wit,
-                           bplanes(1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
-      else
-        hipLaunchKernelGGL((k_gkr_fused_fw<F, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, w2, w2, bplanes(j),
-                           bplanes(j + 1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
-      HIPCHK(hipGetLastError());
-      const int din = j == 0 ? FI::D : F::D;
-      TRY(prof_end(h, 0, 8.0 * 2 * ((double)M * din + (double)(M / 2) * F::D), 0.0));
+      bool fused = false;
+      Reduce red = h->red;
+      if constexpr (std::is_same<FI, Fp>::value && std::is_same<F, Fp>::value) {
+        if (deferred && j >= 1 && h->fuse_parts && g <= kFuseMaxGrid && dg <= kFuseMaxIn &&
+            !std::getenv("EFFSC_HIP_NO_FUSE")) {
+          red = dred.partials == h->fuse_parts ? h->red : Reduce{h->fuse_parts, h->red.result};
+          GkrStep stp{dred, dg, fs_in(h), nullptr, h->d_polys + r * 2, h->d_chals + r, h->d_cur};
+          stp.t_out = stp.t_in == h->d_fs ? h->d_fs_alt : h->d_fs;
+          hipLaunchKernelGGL((k_gkr_fused_fw<F, FI, F, true>), dim3(g), dim3(kBlock), 0, h->stream, w2, w2,
+                             bplanes(j), bplanes(j + 1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, red, stp);
+          HIPCHK(hipGetLastError());
+          if (stp.t_out == h->d_fs_alt) h->fs_pending = h->d_fs_alt;
+          r++;
+          deferred = false;
+          fused = true;
+        }
+      }
+      if (!fused) {
+        if (deferred) TRY(wire_now());
+        TRY(prof_begin(h));
+        if (j == 0)
+          hipLaunchKernelGGL((k_gkr_fused_fw<FI, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, in2, w2, wit,
+                             bplanes(1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
+        else
+          hipLaunchKernelGGL((k_gkr_fused_fw<F, FI, F>), dim3(g), dim3(kBlock), 0, h->stream, w2, w2, bplanes(j),
+                             bplanes(j + 1), hb, wit, kb, cconst, h->d_cur, M / 4, h->w, h->red);
+        HIPCHK(hipGetLastError());
+        const int din = j == 0 ? FI::D : F::D;
+        TRY(prof_end(h, 0, 8.0 * 2 * ((double)M * din + (double)(M / 2) * F::D), 0.0));
+      }
       h->len = M / 2;
       j++;
-      hipLaunchKernelGGL(k_gkr_wire_fs<F>, dim3(1), dim3(kReduceBlock), 0, h->stream, h->red, g, h->d_fs, E,
-                         h->d_polys + r * 2 * E, h->d_chals + r * E, h->d_cur);
-      HIPCHK(hipGetLastError());
-      r++;
+      dred = red;
+      dg = g;
+      deferred = true;
     }
+    if (deferred) TRY(wire_now());
     if (j >= 1) {
       hipLaunchKernelGGL((k_gkr_expand<FI, F>), dim3(blocks_for(h, h->len)), dim3(kBlock), 0, h->stream, bplanes(j),
                          wit, kb, h->len, View<F>{h->work[2], h->work_stride}, View<F>{h->work[3], h->work_stride});

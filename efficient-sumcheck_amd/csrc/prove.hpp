@@ -201,7 +201,9 @@ struct LaneFs {
 // challenge into chal_out and chal_cur; lane 0 does the stores.
 template <class F, int K>
 __device__ __forceinline__ void emit_round_fs(const F* s, int ext, LaneFs& fs, uint64_t* poly, uint64_t* chal_out,
-                                              uint64_t* chal_cur) {
+                                              uint64_t* chal_cur, bool write = true) {
+  // write false: only chal_cur receives the challenge (poly and chal_out
+  // are left alone)
   const bool lane0 = (threadIdx.x & 63) == 0;
 #pragma unroll
   for (int i = 0; i < K; i++) {
@@ -209,7 +211,7 @@ __device__ __forceinline__ void emit_round_fs(const F* s, int ext, LaneFs& fs, u
     gl::store_words(s[i], wd);
 #pragma unroll
     for (int c = 0; c < 3; c++) wd[c] = uniform64(wd[c]);
-    if (lane0)
+    if (lane0 && write)
 #pragma unroll
       for (int c = 0; c < 3; c++)
         if (c < ext) poly[i * ext + c] = wd[c];
@@ -222,7 +224,7 @@ __device__ __forceinline__ void emit_round_fs(const F* s, int ext, LaneFs& fs, u
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       if (c < ext) {
-        chal_out[c] = ch[c];
+        if (write) chal_out[c] = ch[c];
         chal_cur[c] = ch[c];
       }
     }

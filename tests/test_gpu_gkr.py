@@ -132,6 +132,16 @@ def test_gkr_device_loop(k, ext, input_ext, monkeypatch):
             assert ph.round_polys == pg.round_polys and ph.final_value == pg.final_value
             if rounds == 2 * k:
                 assert g.claimed_w_values() == h.claimed_w_values()
+            if ext == 1:
+                # one transcript launch per round instead of the small
+                # factored launches taking the previous round's step
+                monkeypatch.setenv("EFFSC_HIP_NO_FUSE", "1")
+                pl = GkrProver(add, mult, wit, ext=ext, input_ext=input_ext)
+                pp = sumcheck(pl, rounds, mk())
+                monkeypatch.delenv("EFFSC_HIP_NO_FUSE")
+                assert pp.round_polys == pg.round_polys and pp.final_value == pg.final_value
+                for i in range(4):
+                    assert np.array_equal(pl.table_words(i), g.table_words(i)), f"table {i}"
             # and so does the device loop over the four broadcast tables
             monkeypatch.setenv("EFFSC_HIP_GKR_BROADCAST", "1")
             f = GkrProver(add, mult, wit, ext=ext, input_ext=input_ext)
